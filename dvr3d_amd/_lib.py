@@ -1,0 +1,225 @@
+"""ctypes binding of libdvr3d_amd.so (include/dvr3d_amd.h).
+
+The library is built in-tree by ``dvr3d_amd.build.build()``; loading fails
+loudly when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdvr3d_amd.so")
+
+DVR_OK = 0
+DVR_ERR_ARG = 1
+DVR_ERR_HIP = 2
+DVR_ERR_CORE_2D = 3
+DVR_ERR_CORE_3D = 4
+DVR_ERR_NOCONV = 5
+DVR_SOLVER_DENSE = 0
+DVR_SOLVER_DAVIDSON = 1
+DVR_NSTAGE = 7
+STAGE_NAMES = ("1d", "2d_build", "2d_eig", "2d_vec", "3d_build", "3d_eig", "3d_vec")
+
+EXPORTS = (
+    "dvr_ctx_create", "dvr_ctx_destroy", "dvr_last_error", "dvr_set_basis",
+    "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_get_sizes",
+    "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
+    "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
+    "dvr_diag", "dvr_syev_batch",
+)
+
+
+class Dims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "npnta", "npntb", "npntc", "max2d", "max3d", "neval", "zcut", "zall",
+        "zr2r1", "idvr", "npnt1", "npnt2", "nvec", "solver")]
+
+
+class DvrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not built: run dvr3d_amd.build.build() (HIP library required, no CPU fallback)")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    I = C.POINTER(C.c_int32)
+    lib.dvr_ctx_create.argtypes = [C.POINTER(Dims), C.c_int32, C.POINTER(P)]
+    lib.dvr_ctx_destroy.argtypes = [P]
+    lib.dvr_ctx_destroy.restype = None
+    lib.dvr_last_error.argtypes = [P]
+    lib.dvr_last_error.restype = C.c_char_p
+    lib.dvr_set_basis.argtypes = [P, D, D, D, D, D, C.c_double, C.c_double]
+    lib.dvr_set_potential.argtypes = [P, D]
+    lib.dvr_eval_potential_ab2.argtypes = [P, D, D, D, D]
+    lib.dvr_run.argtypes = [P]
+    lib.dvr_get_sizes.argtypes = [P, I, I, I]
+    lib.dvr_get_eigenvalues.argtypes = [P, D, I]
+    lib.dvr_get_1d_eigenvalues.argtypes = [P, D]
+    lib.dvr_get_2d_eigenvalues.argtypes = [P, D, I]
+    lib.dvr_get_wavefunctions.argtypes = [P, D]
+    lib.dvr_get_vectors.argtypes = [P, D]
+    lib.dvr_get_stage_times.argtypes = [P, D]
+    lib.dvr_diag.argtypes = [D, C.c_int32, C.c_int32, D]
+    lib.dvr_syev_batch.argtypes = [C.c_int32, I, D, I, D, D]
+    for name in EXPORTS:
+        if name not in ("dvr_ctx_destroy", "dvr_last_error"):
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _dp(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context: the jhmain stage sequence behind dvr_run."""
+
+    def __init__(self, dims: dict, device: int = 0):
+        self.lib = load()
+        self.dims = dict(dims)
+        d = Dims(**{k: int(v) for k, v in dims.items()})
+        self._d = d
+        h = C.c_void_p()
+        rc = self.lib.dvr_ctx_create(C.byref(d), device, C.byref(h))
+        if rc != DVR_OK:
+            raise DvrError(rc, "dvr_ctx_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dvr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != DVR_OK:
+            msg = self.lib.dvr_last_error(self.h)
+            raise DvrError(rc, msg.decode() if msg else "")
+
+    def set_basis(self, xlmatr, xkb, xkc, wsum, wterm, emax1_au, emax2_au):
+        self._keep = [_f64(x) for x in (xlmatr, xkb, xkc, wsum)]
+        wt = None if wterm is None else _f64(wterm)
+        self._check(self.lib.dvr_set_basis(
+            self.h, *[_dp(x) for x in self._keep], _dp(wt), float(emax1_au), float(emax2_au)))
+
+    def set_potential(self, v):
+        v = _f64(v)
+        self._check(self.lib.dvr_set_potential(self.h, _dp(v)))
+
+    def eval_potential_ab2(self, rc, rb, xa, par):
+        arrs = [_f64(x) for x in (rc, rb, xa, par)]
+        self._check(self.lib.dvr_eval_potential_ab2(self.h, *[_dp(x) for x in arrs]))
+
+    def run(self):
+        self._check(self.lib.dvr_run(self.h))
+
+    def sizes(self):
+        dd = self.dims
+        iv1 = np.zeros(dd["npntc"] * dd["npntb"], dtype=np.int32)
+        iv2 = np.zeros(dd["npntc"], dtype=np.int32)
+        n3 = C.c_int32()
+        self._check(self.lib.dvr_get_sizes(self.h, _ip(iv1), _ip(iv2), C.byref(n3)))
+        # iv1 comes back as the reference's iv1(ndimc, ndimb), column-major
+        return iv1.reshape(dd["npntb"], dd["npntc"]).T.copy(), iv2, int(n3.value)
+
+    def eigenvalues(self):
+        m = C.c_int32()
+        self._check(self.lib.dvr_get_eigenvalues(self.h, None, C.byref(m)))
+        out = np.zeros(m.value)
+        self._check(self.lib.dvr_get_eigenvalues(self.h, _dp(out), C.byref(m)))
+        return out
+
+    def eigenvalues_1d(self):
+        dd = self.dims
+        out = np.zeros(dd["npntc"] * dd["npntb"] * dd["npnta"])
+        self._check(self.lib.dvr_get_1d_eigenvalues(self.h, _dp(out)))
+        return out.reshape(dd["npntc"], dd["npntb"], dd["npnta"])
+
+    def eigenvalues_2d(self):
+        t = C.c_int32()
+        self._check(self.lib.dvr_get_2d_eigenvalues(self.h, None, C.byref(t)))
+        out = np.zeros(t.value)
+        self._check(self.lib.dvr_get_2d_eigenvalues(self.h, _dp(out), C.byref(t)))
+        return out
+
+    def wavefunctions(self, nvec: int):
+        dd = self.dims
+        out = np.zeros(nvec * dd["idvr"] * dd["npnt1"] * dd["npnt2"])
+        self._check(self.lib.dvr_get_wavefunctions(self.h, _dp(out)))
+        # record l is psi(idvr, npnt1, npnt2) in Fortran order
+        return out.reshape(nvec, dd["npnt2"], dd["npnt1"], dd["idvr"])
+
+    def vectors(self, n3: int, nvec: int):
+        out = np.zeros(n3 * nvec)
+        self._check(self.lib.dvr_get_vectors(self.h, _dp(out)))
+        return out.reshape(nvec, n3).T
+
+    def stage_times(self):
+        out = np.zeros(DVR_NSTAGE)
+        self._check(self.lib.dvr_get_stage_times(self.h, _dp(out)))
+        return dict(zip(STAGE_NAMES, out.tolist()))
+
+
+def syev_batch(mats, nvec=None, vectors=True):
+    """Lowest eigenpairs of several symmetric matrices (lower triangle read)."""
+    lib = load()
+    n = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    nv = n.copy() if nvec is None else np.array(nvec, dtype=np.int32)
+    a = np.concatenate([np.asfortranarray(m, dtype=np.float64).ravel(order="F") for m in mats])
+    w = np.zeros(int(nv.sum()))
+    z = np.zeros(int((n.astype(np.int64) * nv).sum())) if vectors else None
+    rc = lib.dvr_syev_batch(len(mats), _ip(n), _dp(a), _ip(nv), _dp(w), _dp(z))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_syev_batch failed")
+    ws, zs, wo, zo = [], [], 0, 0
+    for p in range(len(mats)):
+        ws.append(w[wo:wo + nv[p]])
+        wo += nv[p]
+        if vectors:
+            zs.append(z[zo:zo + n[p] * nv[p]].reshape(nv[p], n[p]).T)
+            zo += n[p] * nv[p]
+    return (ws, zs) if vectors else ws
+
+
+def diag(ham: np.ndarray):
+    """dvr_diag on a square column-major copy of ham: returns (eig, vectors)."""
+    lib = load()
+    h = np.asfortranarray(ham, dtype=np.float64).copy(order="F")
+    n = h.shape[0]
+    e = np.zeros(n)
+    rc = lib.dvr_diag(h.ctypes.data_as(C.POINTER(C.c_double)), n, n, _dp(e))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_diag failed")
+    return e, h

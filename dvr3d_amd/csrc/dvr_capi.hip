@@ -1,0 +1,758 @@
+// C ABI and stage orchestration of the theta-first DVR3DRJZ path on one GPU.
+//
+// The stage sequence is that of jhmain (source/dvr3drjz_segmented.f90:
+// 1915-2118); every intermediate stays in device memory instead of the
+// reference's scratch units (IVECS1/IEIGS1/INTVEC/IVECS2/IVINT/IBAND).  The
+// host only sees the small integer truncation data (iv1, iv2), which it needs
+// to size the following launches, and the 2D eigenvalues for the global
+// selection (choose, D:2767-2867).
+#include "../../include/dvr3d_amd.h"
+#include "dvr_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace dvr;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  void ensure(size_t b) {
+    if (b <= bytes && p) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) b = 8;
+    DVR_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <class T>
+T* upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (!h.empty())
+    DVR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(),
+                           hipMemcpyHostToDevice, s));
+  return d.as<T>();
+}
+
+// Working storage of one batched eigensolve.
+struct EigWork {
+  DevBuf n, aoff, voff, d, e, e2, tau, vb0, vb1, wb, pb, gl, gu, tn, piv;
+  DevBuf il, kofs, eoff, eval;
+  DevBuf zoff, m, clus, scr, tbuf, toff;
+  EigBatch b{};
+  std::vector<int> hn;
+  std::vector<long> hvoff;
+  std::vector<int> hcnt;
+  std::vector<long> heoff;
+  std::vector<double> htn;
+  std::vector<double> hval;  // concatenated eigenvalues (heoff offsets)
+
+  // Reduce A_p (at A + aoff[p], n_p x n_p) to tridiagonal form.
+  void tridiag(const std::vector<int>& n_, double* A,
+               const std::vector<long>& aoff_, hipStream_t s) {
+    hn = n_;
+    const int np = (int)hn.size();
+    hvoff.assign(np, 0);
+    long tot = 0;
+    int nmax = 0;
+    for (int p = 0; p < np; ++p) {
+      hvoff[p] = tot;
+      tot += hn[p];
+      nmax = std::max(nmax, hn[p]);
+    }
+    const size_t vb = sizeof(double) * (tot + 1);
+    for (DevBuf* x : {&d, &e, &e2, &tau, &vb0, &vb1, &wb, &pb}) x->ensure(vb);
+    for (DevBuf* x : {&gl, &gu, &tn, &piv}) x->ensure(sizeof(double) * np);
+    b.nprob = np;
+    b.nmax = nmax;
+    b.n = upload(n, hn, s);
+    b.aoff = upload(aoff, aoff_, s);
+    b.voff = upload(voff, hvoff, s);
+    b.A = A;
+    b.d = d.as<double>();
+    b.e = e.as<double>();
+    b.e2 = e2.as<double>();
+    b.tau = tau.as<double>();
+    b.vbuf0 = vb0.as<double>();
+    b.vbuf1 = vb1.as<double>();
+    b.wbuf = wb.as<double>();
+    b.pbuf = pb.as<double>();
+    b.gl = gl.as<double>();
+    b.gu = gu.as<double>();
+    b.tnrm = tn.as<double>();
+    b.pivmin = piv.as<double>();
+    launch_tridiag(s, b, hn.data());
+    launch_tridiag_bounds(s, b);
+  }
+
+  // Lowest cnt[p] eigenvalues of every problem -> hval (synchronises).
+  void values(const std::vector<int>& cnt, hipStream_t s) {
+    const int np = b.nprob;
+    hcnt = cnt;
+    std::vector<int> hil(np, 0), hk(np + 1, 0);
+    heoff.assign(np, 0);
+    long tot = 0;
+    for (int p = 0; p < np; ++p) {
+      hk[p + 1] = hk[p] + cnt[p];
+      heoff[p] = tot;
+      tot += cnt[p];
+    }
+    eval.ensure(sizeof(double) * (tot + 1));
+    launch_bisect(s, b, upload(il, hil, s), upload(kofs, hk, s), hk[np],
+                  eval.as<double>(), upload(eoff, heoff, s));
+    hval.assign(tot, 0.0);
+    htn.assign(np, 0.0);
+    if (tot)
+      DVR_HIP(hipMemcpyAsync(hval.data(), eval.p, sizeof(double) * tot,
+                             hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipMemcpyAsync(htn.data(), tn.p, sizeof(double) * np,
+                           hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipStreamSynchronize(s));
+  }
+
+  // Number of vectors to compute so that the last wanted one does not split
+  // a cluster of close eigenvalues.
+  int extend(int p, int want) const {
+    if (want <= 0) return 0;
+    const double ctol = 1.0e-6 * std::max(htn[p], 1e-300);
+    const double* v = hval.data() + heoff[p];
+    int k = want;
+    while (k < hcnt[p] && v[k] - v[k - 1] < ctol) ++k;
+    return k;
+  }
+
+  // Eigenvectors 0..m[p]-1 of every problem into Z (n_p x m_p at zoff[p]).
+  void vectors(const std::vector<int>& mv, double* Z,
+               const std::vector<long>& zoff_, hipStream_t s) {
+    const int np = b.nprob;
+    std::vector<Cluster> cl;
+    for (int p = 0; p < np; ++p) {
+      const int mp = mv[p];
+      if (mp <= 0) continue;
+      const double ctol = 1.0e-6 * std::max(htn[p], 1e-300);
+      const double* v = hval.data() + heoff[p];
+      int kb = 0;
+      for (int k = 1; k <= mp; ++k) {
+        if (k == mp || v[k] - v[k - 1] >= ctol) {
+          cl.push_back(Cluster{p, kb, k});
+          kb = k;
+        }
+      }
+    }
+    const int ncl = (int)cl.size();
+    if (ncl == 0) return;
+    const int T = ncl;
+    scr.ensure(sizeof(double) * stein_scratch_doubles(b.nmax, T));
+    DevBuf dcl;
+    dcl.ensure(sizeof(Cluster) * ncl);
+    DVR_HIP(hipMemcpyAsync(dcl.p, cl.data(), sizeof(Cluster) * ncl,
+                           hipMemcpyHostToDevice, s));
+    const long* dz = upload(zoff, zoff_, s);
+    launch_stein(s, b, dcl.as<Cluster>(), ncl, eval.as<double>(),
+                 eoff.as<long>(), Z, dz, scr.as<double>(), T);
+    // WY factors
+    const int NBk = backtransform_nb();
+    std::vector<long> ht(np, 0);
+    long tt = 0;
+    for (int p = 0; p < np; ++p) {
+      ht[p] = tt;
+      const int nr = hn[p] - 2;
+      tt += nr > 0 ? (long)((nr + NBk - 1) / NBk) * NBk * NBk : 0;
+    }
+    tbuf.ensure(sizeof(double) * (tt + 1));
+    launch_backtransform(s, b, Z, dz, upload(m, mv, s), hn.data(), mv.data(),
+                         tbuf.as<double>(), upload(toff, ht, s), tt);
+    DVR_HIP(hipStreamSynchronize(s));
+  }
+};
+
+}  // namespace
+
+struct dvr_ctx {
+  dvr_dims_t dims{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool basis_set = false, pot_set = false, ran = false;
+  double emax1 = 0.0, emax2 = 0.0;
+  DevBuf xl, xkb, xkc, wsum, wterm, rc, rb, xa, v;
+  DevBuf e1full, v1full, cnt, vscr, sweeps;
+  DevBuf srcrow, rowb, v1, e1p, iv1off;
+  DevBuf H2, sdesc2, Z2;
+  DevBuf cint, blk3, e2sel, cblk, sdesc3, H3, Z3, psi;
+  EigWork w2, w3;
+  bool has_wterm = false;
+  std::vector<int> h_iv1;    // [c*npntb + b]
+  std::vector<int> h_nham2, h_iv2;
+  std::vector<double> h_e2all;
+  int nham3 = 0, meval = 0, nvec_done = 0;
+  std::vector<double> h_eval;
+  double stage_ms[DVR_NSTAGE] = {0};
+  hipEvent_t ev[DVR_NSTAGE + 1] = {};
+};
+
+namespace {
+
+int fail(dvr_ctx_t* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+void run_pipeline(dvr_ctx_t* C) {
+  const dvr_dims_t& D = C->dims;
+  hipStream_t s = C->stream;
+  const int na = D.npnta, nb = D.npntb, nc = D.npntc;
+  const int nprob = nb * nc;
+  const double emax1 = C->emax1;
+  for (int i = 0; i < DVR_NSTAGE; ++i) C->stage_ms[i] = 0.0;
+  DVR_HIP(hipEventRecord(C->ev[0], s));
+
+  // ---- 1D: mkham1 + diag + cut1d for every (c, b)
+  C->e1full.ensure(sizeof(double) * (size_t)nprob * na);
+  C->v1full.ensure(sizeof(double) * (size_t)nprob * na * na);
+  C->cnt.ensure(sizeof(int) * nprob);
+  C->sweeps.ensure(sizeof(int) * nprob);
+  const size_t vs = jacobi1d_scratch_doubles(na, nprob);
+  if (vs) C->vscr.ensure(sizeof(double) * vs);
+  launch_jacobi1d(s, C->v.as<double>(), C->xl.as<double>(),
+                  C->wsum.as<double>(),
+                  C->has_wterm ? C->wterm.as<double>() : nullptr, na, nprob,
+                  emax1, C->e1full.as<double>(), C->v1full.as<double>(),
+                  C->cnt.as<int>(), vs ? C->vscr.as<double>() : nullptr,
+                  nullptr, C->sweeps.as<int>());
+  std::vector<int> cnt(nprob);
+  DVR_HIP(hipMemcpyAsync(cnt.data(), C->cnt.p, sizeof(int) * nprob,
+                         hipMemcpyDeviceToHost, s));
+  DVR_HIP(hipEventRecord(C->ev[1], s));
+  DVR_HIP(hipStreamSynchronize(s));
+  if (D.zall)
+    for (int& x : cnt) x = na;
+  C->h_iv1 = cnt;
+
+  // ---- sizes of the 2D blocks (D:2021-2025), capacity check (D:2925)
+  std::vector<int> nham2(nc, 0), rowbase(nc, 0), iv1off(nc * (nb + 1), 0);
+  int rows = 0;
+  for (int c = 0; c < nc; ++c) {
+    rowbase[c] = rows;
+    int s2 = 0;
+    for (int b = 0; b < nb; ++b) {
+      iv1off[c * (nb + 1) + b] = s2;
+      s2 += cnt[c * nb + b];
+    }
+    iv1off[c * (nb + 1) + nb] = s2;
+    nham2[c] = s2;
+    if (s2 > D.max2d && !D.zall)
+      throw std::runtime_error("core2d");
+    rows += s2;
+  }
+  C->h_nham2 = nham2;
+  std::vector<int> srcrow(rows), rowb(rows);
+  for (int c = 0; c < nc; ++c)
+    for (int b = 0; b < nb; ++b)
+      for (int k = 0; k < cnt[c * nb + b]; ++k) {
+        const int r = rowbase[c] + iv1off[c * (nb + 1) + b] + k;
+        srcrow[r] = (c * nb + b) * na + k;
+        rowb[r] = b;
+      }
+  C->v1.ensure(sizeof(double) * ((size_t)rows * na + 1));
+  C->e1p.ensure(sizeof(double) * (rows + 1));
+  launch_pack_1d(s, C->v1full.as<double>(), C->e1full.as<double>(),
+                 upload(C->srcrow, srcrow, s), rows, na, C->v1.as<double>(),
+                 C->e1p.as<double>());
+  const int* d_rowb = upload(C->rowb, rowb, s);
+  const int* d_iv1off = upload(C->iv1off, iv1off, s);
+
+  // ---- mkham2: grouped SYRK over the c blocks
+  std::vector<int> pc;  // c of each nonempty block
+  std::vector<int> pn;
+  std::vector<long> h2off;
+  long h2tot = 0;
+  for (int c = 0; c < nc; ++c)
+    if (nham2[c] > 0) {
+      pc.push_back(c);
+      pn.push_back(nham2[c]);
+      h2off.push_back(h2tot);
+      h2tot += (long)nham2[c] * nham2[c];
+    }
+  const int np2 = (int)pc.size();
+  if (np2 == 0) throw std::runtime_error("empty 2D basis: no 1D level below EMAX1");
+  C->H2.ensure(sizeof(double) * (h2tot + 1));
+  {
+    const int TM = syrk_tile();
+    std::vector<SyrkDesc> desc(np2);
+    int tiles = 0;
+    for (int q = 0; q < np2; ++q) {
+      const int c = pc[q];
+      const int nt = (pn[q] + TM - 1) / TM;
+      SyrkDesc& x = desc[q];
+      x.X = C->v1.as<double>() + (size_t)rowbase[c] * na;
+      x.ldx = na;
+      x.K = na;
+      x.N = pn[q];
+      x.C = C->H2.as<double>() + h2off[q];
+      x.blk = d_rowb + rowbase[c];
+      x.diag = C->e1p.as<double>() + rowbase[c];
+      x.tile0 = tiles;
+      x.ntile = nt * (nt + 1) / 2;
+      tiles += x.ntile;
+    }
+    launch_syrk_scaled(s, upload(C->sdesc2, desc, s), np2, tiles,
+                       C->xkb.as<double>(), nb);
+  }
+  DVR_HIP(hipEventRecord(C->ev[2], s));
+
+  // ---- 2D eigenvalues (diag at D:2030), all of them
+  C->w2.tridiag(pn, C->H2.as<double>(), h2off, s);
+  C->w2.values(pn, s);
+  DVR_HIP(hipEventRecord(C->ev[3], s));
+  {
+    std::vector<double> all;
+    for (int q = 0; q < np2; ++q)
+      all.insert(all.end(), C->w2.hval.begin() + C->w2.heoff[q],
+                 C->w2.hval.begin() + C->w2.heoff[q] + pn[q]);
+    C->h_e2all = all;
+  }
+
+  // ---- selection of the final basis: choose (D:2767-2867) or cut2d (D:2941)
+  std::vector<int> iv2(nc, 0);
+  int n3 = 0;
+  if (D.zcut || D.zall) {
+    const double emax2 = C->emax2;
+    for (int q = 0; q < np2; ++q) {
+      const double* e = C->w2.hval.data() + C->w2.heoff[q];
+      int k = 0;
+      if (D.zall)
+        k = pn[q];
+      else
+        while (k < pn[q] && !(e[k] > emax2)) ++k;
+      iv2[pc[q]] = k;
+      n3 += k;
+    }
+    if (n3 > D.max3d && !D.zall) throw std::runtime_error("core3d");
+  } else {
+    long tot = 0;
+    for (int q = 0; q < np2; ++q) tot += pn[q];
+    const int low3d = (int)std::min<long>(D.max3d, tot);
+    std::vector<std::tuple<double, int, int>> keys;
+    keys.reserve(tot);
+    for (int q = 0; q < np2; ++q) {
+      const double* e = C->w2.hval.data() + C->w2.heoff[q];
+      for (int k = 0; k < pn[q]; ++k) keys.emplace_back(e[k], pc[q], k);
+    }
+    // lowest energies first; on equal energies the lower c block first and
+    // within a block the lower index, as the strict '<' merge of D:2808-2813
+    std::partial_sort(keys.begin(), keys.begin() + low3d, keys.end());
+    for (int i = 0; i < low3d; ++i) iv2[std::get<1>(keys[i])]++;
+    n3 = low3d;
+  }
+  C->h_iv2 = iv2;
+  C->nham3 = n3;
+  if (n3 <= 0) throw std::runtime_error("empty final basis");
+
+  // ---- chosen 2D eigenvectors
+  std::vector<int> m2(np2), m2ext(np2);
+  std::vector<long> z2off(np2);
+  long z2tot = 0;
+  for (int q = 0; q < np2; ++q) {
+    m2[q] = iv2[pc[q]];
+    m2ext[q] = C->w2.extend(q, m2[q]);
+    z2off[q] = z2tot;
+    z2tot += (long)pn[q] * m2ext[q];
+  }
+  C->Z2.ensure(sizeof(double) * (z2tot + 1));
+  C->w2.vectors(m2ext, C->Z2.as<double>(), z2off, s);
+  DVR_HIP(hipEventRecord(C->ev[4], s));
+
+  // ---- 3D: cint expansion + mkham3 SYRK + loadh diagonal
+  std::vector<CBlock> cb(nc);
+  std::vector<int> blk3(n3);
+  std::vector<double> e2sel(n3);
+  {
+    int col = 0, q = 0;
+    for (int c = 0; c < nc; ++c) {
+      cb[c].rowbase = rowbase[c];
+      cb[c].nham2 = nham2[c];
+      cb[c].iv2 = iv2[c];
+      cb[c].col0 = col;
+      cb[c].z2off = 0;
+      if (nham2[c] > 0) {
+        cb[c].z2off = z2off[q];
+        const double* e = C->w2.hval.data() + C->w2.heoff[q];
+        for (int j = 0; j < iv2[c]; ++j) {
+          blk3[col + j] = c;
+          e2sel[col + j] = e[j];
+        }
+        ++q;
+      }
+      col += iv2[c];
+    }
+  }
+  const long G = (long)na * nb;
+  C->cint.ensure(sizeof(double) * (size_t)G * n3);
+  const CBlock* d_cb = upload(C->cblk, cb, s);
+  const int* d_blk3 = upload(C->blk3, blk3, s);
+  launch_cint(s, C->v1.as<double>(), C->Z2.as<double>(), C->cint.as<double>(),
+              d_cb, d_blk3, d_iv1off, na, nb, n3);
+  C->H3.ensure(sizeof(double) * (size_t)n3 * n3);
+  {
+    const int TM = syrk_tile();
+    const int nt = (n3 + TM - 1) / TM;
+    std::vector<SyrkDesc> desc(1);
+    desc[0].X = C->cint.as<double>();
+    desc[0].ldx = G;
+    desc[0].K = (int)G;
+    desc[0].N = n3;
+    desc[0].C = C->H3.as<double>();
+    desc[0].blk = d_blk3;
+    desc[0].diag = upload(C->e2sel, e2sel, s);
+    desc[0].tile0 = 0;
+    desc[0].ntile = nt * (nt + 1) / 2;
+    launch_syrk_scaled(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile,
+                       C->xkc.as<double>(), nc);
+  }
+  DVR_HIP(hipEventRecord(C->ev[5], s));
+
+  // ---- final solve (diag3d, D:2661-2764)
+  const int meval = std::min(D.neval, n3);
+  C->meval = meval;
+  const int want = std::min(n3, meval + 16);
+  C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
+  C->w3.values(std::vector<int>{want}, s);
+  C->h_eval.assign(C->w3.hval.begin(), C->w3.hval.begin() + meval);
+  DVR_HIP(hipEventRecord(C->ev[6], s));
+
+  // ---- eigenvectors and wavefunctions (trans, D:3650-3845)
+  const int nvec = std::min(D.nvec, meval);
+  C->nvec_done = nvec;
+  if (nvec > 0) {
+    const int mext = C->w3.extend(0, nvec);
+    C->Z3.ensure(sizeof(double) * (size_t)n3 * mext);
+    C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);
+    const size_t npsi = (size_t)D.idvr * D.npnt1 * D.npnt2;
+    C->psi.ensure(sizeof(double) * npsi * nvec);
+    DVR_HIP(hipMemsetAsync(C->psi.p, 0, sizeof(double) * npsi * nvec, s));
+    launch_trans(s, C->cint.as<double>(), C->Z3.as<double>(),
+                 C->psi.as<double>(), d_cb, nc, na, nb, n3, nvec, D.zr2r1,
+                 D.idvr, D.npnt1, D.npnt2);
+  }
+  DVR_HIP(hipEventRecord(C->ev[7], s));
+  DVR_HIP(hipStreamSynchronize(s));
+  for (int i = 0; i < DVR_NSTAGE; ++i) {
+    float ms = 0.f;
+    DVR_HIP(hipEventElapsedTime(&ms, C->ev[i], C->ev[i + 1]));
+    C->stage_ms[i] = ms;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dvr_ctx_create(const dvr_dims_t* dims, int32_t device, dvr_ctx_t** out) {
+  if (!dims || !out) return DVR_ERR_ARG;
+  *out = nullptr;
+  const dvr_dims_t& D = *dims;
+  if (D.npnta <= 0 || D.npntb <= 0 || D.npntc <= 0 || D.max2d <= 0 ||
+      D.max3d <= 0 || D.neval <= 0 || D.nvec < 0 || D.idvr < D.npnta ||
+      (D.solver != DVR_SOLVER_DENSE && D.solver != DVR_SOLVER_DAVIDSON))
+    return DVR_ERR_ARG;
+  if (D.nvec > 0 && D.npnt1 * D.npnt2 != D.npntb * D.npntc) return DVR_ERR_ARG;
+  std::unique_ptr<dvr_ctx_t> c(new dvr_ctx_t());
+  c->dims = D;
+  c->device = device;
+  try {
+    DeviceGuard g(device);
+    DVR_HIP(hipSetDevice(device));
+    DVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) DVR_HIP(hipEventCreate(&e));
+  } catch (const std::exception& e) {
+    return DVR_ERR_HIP;
+  }
+  *out = c.release();
+  return DVR_OK;
+}
+
+void dvr_ctx_destroy(dvr_ctx_t* c) {
+  if (!c) return;
+  {
+    DeviceGuard g(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev)
+      if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+    delete c;
+  }
+}
+
+const char* dvr_last_error(const dvr_ctx_t* c) {
+  return c ? c->err.c_str() : "null context";
+}
+
+int dvr_set_basis(dvr_ctx_t* c, const double* xlmatr, const double* xkb,
+                  const double* xkc, const double* wsum, const double* wterm,
+                  double emax1, double emax2) {
+  if (!c || !xlmatr || !xkb || !xkc || !wsum) return fail(c, DVR_ERR_ARG, "null argument");
+  const dvr_dims_t& D = c->dims;
+  try {
+    DeviceGuard g(c->device);
+    auto put = [&](DevBuf& b, const double* h, size_t n) {
+      b.ensure(sizeof(double) * n);
+      DVR_HIP(hipMemcpyAsync(b.p, h, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    };
+    put(c->xl, xlmatr, (size_t)D.npnta * D.npnta);
+    put(c->xkb, xkb, (size_t)D.npntb * D.npntb);
+    put(c->xkc, xkc, (size_t)D.npntc * D.npntc);
+    put(c->wsum, wsum, (size_t)D.npntc * D.npntb);
+    c->has_wterm = wterm != nullptr;
+    if (wterm) put(c->wterm, wterm, (size_t)D.npntc * D.npntb);
+    DVR_HIP(hipStreamSynchronize(c->stream));
+  } catch (const std::exception& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  }
+  c->emax1 = emax1;
+  c->emax2 = emax2;
+  c->basis_set = true;
+  return DVR_OK;
+}
+
+int dvr_set_potential(dvr_ctx_t* c, const double* v) {
+  if (!c || !v) return fail(c, DVR_ERR_ARG, "null argument");
+  const dvr_dims_t& D = c->dims;
+  try {
+    DeviceGuard g(c->device);
+    const size_t n = (size_t)D.npnta * D.npntb * D.npntc;
+    c->v.ensure(sizeof(double) * n);
+    DVR_HIP(hipMemcpyAsync(c->v.p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    DVR_HIP(hipStreamSynchronize(c->stream));
+  } catch (const std::exception& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  }
+  c->pot_set = true;
+  return DVR_OK;
+}
+
+int dvr_eval_potential_ab2(dvr_ctx_t* c, const double* rc, const double* rb,
+                           const double* xa, const double* par) {
+  if (!c || !rc || !rb || !xa || !par) return fail(c, DVR_ERR_ARG, "null argument");
+  const dvr_dims_t& D = c->dims;
+  try {
+    DeviceGuard g(c->device);
+    const size_t n = (size_t)D.npnta * D.npntb * D.npntc;
+    c->v.ensure(sizeof(double) * n);
+    auto put = [&](DevBuf& b, const double* h, size_t k) {
+      b.ensure(sizeof(double) * k);
+      DVR_HIP(hipMemcpyAsync(b.p, h, sizeof(double) * k, hipMemcpyHostToDevice, c->stream));
+    };
+    put(c->rc, rc, D.npntc);
+    put(c->rb, rb, D.npntb);
+    put(c->xa, xa, D.npnta);
+    launch_pes_ab2(c->stream, c->v.as<double>(), c->rc.as<double>(),
+                   c->rb.as<double>(), c->xa.as<double>(), D.npntc, D.npntb,
+                   D.npnta, D.zr2r1, par);
+    DVR_HIP(hipStreamSynchronize(c->stream));
+  } catch (const std::exception& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  }
+  c->pot_set = true;
+  return DVR_OK;
+}
+
+int dvr_run(dvr_ctx_t* c) {
+  if (!c) return DVR_ERR_ARG;
+  if (!c->basis_set || !c->pot_set)
+    return fail(c, DVR_ERR_ARG, "dvr_run: basis and potential must be set first");
+  if (c->dims.solver != DVR_SOLVER_DENSE)
+    return fail(c, DVR_ERR_ARG, "dvr_run: requested solver is not available in this build");
+  c->ran = false;
+  try {
+    DeviceGuard g(c->device);
+    run_pipeline(c);
+  } catch (const HipError& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  } catch (const std::runtime_error& e) {
+    const std::string w = e.what();
+    if (w == "core2d")
+      return fail(c, DVR_ERR_CORE_2D, "**** core exceeded: reduce cut-off emax1 ****");
+    if (w == "core3d")
+      return fail(c, DVR_ERR_CORE_3D, "**** core exceeded: reduce cut-off emax2 ****");
+    return fail(c, DVR_ERR_ARG, w);
+  } catch (const std::exception& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  }
+  c->ran = true;
+  return DVR_OK;
+}
+
+int dvr_get_sizes(const dvr_ctx_t* c, int32_t* iv1, int32_t* iv2, int32_t* nham3) {
+  if (!c || !c->ran) return DVR_ERR_ARG;
+  const dvr_dims_t& D = c->dims;
+  if (iv1)
+    for (int cc = 0; cc < D.npntc; ++cc)
+      for (int b = 0; b < D.npntb; ++b)
+        iv1[cc + D.npntc * b] = c->h_iv1[cc * D.npntb + b];
+  if (iv2)
+    for (int cc = 0; cc < D.npntc; ++cc) iv2[cc] = c->h_iv2[cc];
+  if (nham3) *nham3 = c->nham3;
+  return DVR_OK;
+}
+
+int dvr_get_eigenvalues(const dvr_ctx_t* c, double* eval, int32_t* meval) {
+  if (!c || !c->ran) return DVR_ERR_ARG;
+  if (meval) *meval = c->meval;
+  if (eval) std::memcpy(eval, c->h_eval.data(), sizeof(double) * c->meval);
+  return DVR_OK;
+}
+
+int dvr_get_1d_eigenvalues(const dvr_ctx_t* c, double* e1) {
+  if (!c || !c->ran || !e1) return DVR_ERR_ARG;
+  const dvr_dims_t& D = c->dims;
+  DeviceGuard g(c->device);
+  const size_t n = (size_t)D.npnta * D.npntb * D.npntc;
+  if (hipMemcpy(e1, c->e1full.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return DVR_ERR_HIP;
+  return DVR_OK;
+}
+
+int dvr_get_2d_eigenvalues(const dvr_ctx_t* c, double* e2, int32_t* total) {
+  if (!c || !c->ran) return DVR_ERR_ARG;
+  if (total) *total = (int32_t)c->h_e2all.size();
+  if (e2) std::memcpy(e2, c->h_e2all.data(), sizeof(double) * c->h_e2all.size());
+  return DVR_OK;
+}
+
+int dvr_get_wavefunctions(const dvr_ctx_t* c, double* psi) {
+  if (!c || !c->ran || !psi || c->nvec_done <= 0) return DVR_ERR_ARG;
+  const dvr_dims_t& D = c->dims;
+  DeviceGuard g(c->device);
+  const size_t n = (size_t)D.idvr * D.npnt1 * D.npnt2 * c->nvec_done;
+  if (hipMemcpy(psi, c->psi.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return DVR_ERR_HIP;
+  return DVR_OK;
+}
+
+int dvr_get_vectors(const dvr_ctx_t* c, double* vec) {
+  if (!c || !c->ran || !vec || c->nvec_done <= 0) return DVR_ERR_ARG;
+  DeviceGuard g(c->device);
+  const size_t n = (size_t)c->nham3 * c->nvec_done;
+  if (hipMemcpy(vec, c->Z3.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    return DVR_ERR_HIP;
+  return DVR_OK;
+}
+
+int dvr_get_stage_times(const dvr_ctx_t* c, double* ms) {
+  if (!c || !ms) return DVR_ERR_ARG;
+  for (int i = 0; i < DVR_NSTAGE; ++i) ms[i] = c->stage_ms[i];
+  return DVR_OK;
+}
+
+int dvr_syev_batch(int32_t nprob, const int32_t* n, const double* a,
+                   const int32_t* nvec, double* w, double* z) {
+  if (nprob <= 0 || !n || !a || !nvec || !w) return DVR_ERR_ARG;
+  for (int p = 0; p < nprob; ++p)
+    if (n[p] <= 0 || nvec[p] < 0 || nvec[p] > n[p]) return DVR_ERR_ARG;
+  hipStream_t s = nullptr;
+  try {
+    DVR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<int> hn(n, n + nprob), cntv(nvec, nvec + nprob);
+    std::vector<long> aoff(nprob);
+    long tot = 0;
+    for (int p = 0; p < nprob; ++p) {
+      aoff[p] = tot;
+      tot += (long)hn[p] * hn[p];
+    }
+    DevBuf A;
+    A.ensure(sizeof(double) * tot);
+    // symmetrise from the lower triangle on the host while staging
+    std::vector<double> stage(a, a + tot);
+    for (int p = 0; p < nprob; ++p) {
+      double* M = stage.data() + aoff[p];
+      const int np = hn[p];
+      for (int j = 0; j < np; ++j)
+        for (int i = 0; i < j; ++i) M[i + (long)np * j] = M[j + (long)np * i];
+    }
+    DVR_HIP(hipMemcpyAsync(A.p, stage.data(), sizeof(double) * tot, hipMemcpyHostToDevice, s));
+    EigWork W;
+    W.tridiag(hn, A.as<double>(), aoff, s);
+    std::vector<int> want(nprob);
+    for (int p = 0; p < nprob; ++p) want[p] = std::min(hn[p], cntv[p] + (cntv[p] < hn[p] ? 16 : 0));
+    W.values(want, s);
+    long wo = 0;
+    for (int p = 0; p < nprob; ++p) {
+      std::memcpy(w + wo, W.hval.data() + W.heoff[p], sizeof(double) * cntv[p]);
+      wo += cntv[p];
+    }
+    if (z) {
+      std::vector<int> mext(nprob);
+      std::vector<long> zoff(nprob);
+      long zt = 0;
+      for (int p = 0; p < nprob; ++p) {
+        mext[p] = W.extend(p, cntv[p]);
+        zoff[p] = zt;
+        zt += (long)hn[p] * mext[p];
+      }
+      DevBuf Z;
+      Z.ensure(sizeof(double) * (zt + 1));
+      W.vectors(mext, Z.as<double>(), zoff, s);
+      long zo = 0;
+      for (int p = 0; p < nprob; ++p) {
+        DVR_HIP(hipMemcpyAsync(z + zo, Z.as<double>() + zoff[p],
+                               sizeof(double) * (size_t)hn[p] * cntv[p],
+                               hipMemcpyDeviceToHost, s));
+        zo += (long)hn[p] * cntv[p];
+      }
+      DVR_HIP(hipStreamSynchronize(s));
+    }
+    DVR_HIP(hipStreamSynchronize(s));
+    (void)hipStreamDestroy(s);
+  } catch (const std::exception& e) {
+    fprintf(stderr, "dvr_syev_batch: %s\n", e.what());
+    if (s) (void)hipStreamDestroy(s);
+    return DVR_ERR_HIP;
+  }
+  return DVR_OK;
+}
+
+int dvr_diag(double* ham, int32_t maxham, int32_t nham, double* eig) {
+  if (!ham || !eig || nham <= 0 || maxham < nham) return DVR_ERR_ARG;
+  std::vector<double> a((size_t)nham * nham), z((size_t)nham * nham);
+  for (int j = 0; j < nham; ++j)
+    for (int i = 0; i < nham; ++i) a[i + (size_t)nham * j] = ham[i + (size_t)maxham * j];
+  const int32_t nv = nham;
+  const int rc = dvr_syev_batch(1, &nham, a.data(), &nv, eig, z.data());
+  if (rc != DVR_OK) return rc;
+  for (int j = 0; j < nham; ++j)
+    for (int i = 0; i < nham; ++i) ham[i + (size_t)maxham * j] = z[i + (size_t)nham * j];
+  return DVR_OK;
+}
+
+}  // extern "C"

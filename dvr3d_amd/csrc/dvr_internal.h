@@ -1,0 +1,153 @@
+// Internal declarations shared by the HIP translation units of libdvr3d_amd.
+// Kernels live in k_*.hip; each file exposes host-side launch_* wrappers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <stdexcept>
+
+namespace dvr {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define DVR_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t err_ = (call);                                                \
+    if (err_ != hipSuccess)                                                  \
+      throw ::dvr::HipError(std::string(#call) + ": " +                      \
+                            hipGetErrorString(err_) + " (" __FILE__ ":" +   \
+                            std::to_string(__LINE__) + ")");                 \
+  } while (0)
+
+#define DVR_LAUNCH_CHECK() DVR_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// grid-side kernels (k_grid.hip)
+// ---------------------------------------------------------------------------
+// Analytic AB2 surface on the (c, b, a) grid, a fastest:
+//   V = De[(1-e^{-a(r1-re)})^2 + (1-e^{-a(r2-re)})^2] + kb (x - cte)^2
+//       + c (r1-re)(r2-re)
+// par = {De, a, re, kb, cte, c}.  zr2r1: c indexes r1 and b indexes r2.
+void launch_pes_ab2(hipStream_t s, double* v, const double* rc,
+                    const double* rb, const double* xa, int nc, int nb, int na,
+                    int zr2r1, const double par[6]);
+
+// Gather the kept 1D eigenpairs into packed per-block storage.
+//   src_row[r]  : row of the (nprob*n) x n eigenvector array for packed row r
+//   v1 (rows x n, row-major), e1p[r]
+void launch_pack_1d(hipStream_t s, const double* v1full, const double* e1full,
+                    const int* src_row, int rows, int n, double* v1,
+                    double* e1p);
+
+// Per-c-block bookkeeping of the contracted basis.
+struct CBlock {
+  int rowbase;  // first packed 1D row of this block
+  int nham2;    // 2D dimension of the block
+  int iv2;      // chosen 2D functions (columns of the 3D basis)
+  int col0;     // first 3D basis index of the block
+  long z2off;   // offset of the block's 2D vectors (nham2 x iv2, col-major)
+};
+
+// Expansion of the chosen 2D vectors on the (a, b) grid of each c block:
+//   cint[g + G*(col0+j)] = sum_{k<iv1(c,b)} v1[(rowbase+off+k)*na + a]
+//                                          * c2[z2off + off+k + j*nham2]
+// with g = a*nb + b and off = iv1off[c*(nb+1)+b].
+void launch_cint(hipStream_t s, const double* v1, const double* c2,
+                 double* cint, const CBlock* cb, const int* blk3,
+                 const int* iv1off, int na, int nb, int n3);
+
+// Wavefunction amplitudes of nvec levels on the DVR grid, in the IWAVE
+// record layout psi(idvr, npnt1, npnt2) per level.
+void launch_trans(hipStream_t s, const double* cint, const double* c3,
+                  double* psi, const CBlock* cb, int nc, int na, int nb,
+                  int n3, int nvec, int zr2r1, int idvr, int npnt1,
+                  int npnt2);
+
+// ---------------------------------------------------------------------------
+// 1D angular eigenproblems (k_jacobi.hip)
+// ---------------------------------------------------------------------------
+// One workgroup per (c,b) problem: H1 = diag(V + wterm) + wsum*L, parallel
+// cyclic Jacobi in LDS, eigenpairs sorted ascending, count of E <= emax.
+//   e1[p*n + k], v1[(p*n + k)*n + a] (row k = eigenvector k), cnt[p]
+void launch_jacobi1d(hipStream_t s, const double* vgrid, const double* xl,
+                     const double* wsum, const double* wterm, int n, int nprob,
+                     double emaxau, double* e1, double* v1, int* cnt,
+                     double* vscratch, const int* src_prob, int* sweeps);
+size_t jacobi1d_scratch_doubles(int n, int nprob);
+
+// ---------------------------------------------------------------------------
+// symmetric rank-K products on MFMA (k_syrk.hip)
+// ---------------------------------------------------------------------------
+// For each group g:  C_g = (X_g^T X_g) .* S_g + diag(dg_g), with
+//   X_g : K_g x N_g column-major (ld = ldx_g),
+//   S_g(i,j) = xk[blk_g[i] + ldxk*blk_g[j]]  (a block-constant scale),
+//   C_g : N_g x N_g column-major, ld = N_g, both triangles written.
+struct SyrkDesc {
+  const double* X;
+  long ldx;
+  int K;
+  int N;
+  double* C;
+  const int* blk;
+  const double* diag;
+  int tile0;  // first tile index of this group in the flat launch
+  int ntile;  // number of lower-triangular tiles
+};
+void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
+                        int total_tiles, const double* xk, int ldxk);
+int syrk_tile();
+
+// ---------------------------------------------------------------------------
+// batched dense symmetric eigensolver (k_syev.hip)
+// ---------------------------------------------------------------------------
+struct EigBatch {
+  int nprob;
+  int nmax;
+  const int* n;       // [nprob]
+  const long* aoff;   // [nprob] matrix offset, column-major, ld = n
+  const long* voff;   // [nprob] offset of per-row vectors (sum of n)
+  double* A;
+  double* d;
+  double* e;
+  double* e2;
+  double* tau;
+  double* vbuf0;      // Householder vector ping-pong
+  double* vbuf1;
+  double* wbuf;
+  double* pbuf;
+  double* gl;         // [nprob] Gershgorin lower / upper bounds, norm, pivmin
+  double* gu;
+  double* tnrm;
+  double* pivmin;
+};
+// Householder tridiagonalisation A = Q T Q^T (lower), column by column.
+void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n);
+// Gershgorin bounds + e^2.
+void launch_tridiag_bounds(hipStream_t s, const EigBatch& b);
+// Eigenvalues k in [il_p, il_p + cnt_p) of each problem, by Sturm-count
+// multisection (one wavefront per eigenvalue).  kofs = prefix of cnt.
+void launch_bisect(hipStream_t s, const EigBatch& b, const int* il,
+                   const int* kofs, int total, double* eval,
+                   const long* eoff);
+// Inverse iteration.  Cluster c covers eigenvalues [kb, ke) of problem p
+// (local indices relative to il_p).  Z_p is n_p x m_p column-major.
+struct Cluster {
+  int p;
+  int kb;
+  int ke;
+};
+void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
+                  int ncl, const double* eval, const long* eoff,
+                  double* Z, const long* zoff, double* scratch, int nthreads);
+size_t stein_scratch_doubles(int nmax, int nthreads);
+// Back-transformation Z <- Q Z with the stored reflectors (blocked WY).
+void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
+                          const long* zoff, const int* m, const int* h_n,
+                          const int* h_m, double* tbuf, const long* toff,
+                          long tbuf_total);
+int backtransform_nb();
+
+}  // namespace dvr

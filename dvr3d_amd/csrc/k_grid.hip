@@ -1,0 +1,160 @@
+// Streaming kernels over the DVR grid: potential evaluation, packing of the
+// kept 1D eigenpairs, expansion of 2D vectors on the grid (mkham3's cint,
+// source/dvr3drjz_segmented.f90:2362-2411) and the wavefunction transform
+// (trans, source/dvr3drjz_segmented.f90:3650-3845).
+#include "dvr_internal.h"
+
+namespace dvr {
+
+__global__ void __launch_bounds__(256)
+k_pes_ab2(double* __restrict__ v, const double* __restrict__ rc,
+          const double* __restrict__ rb, const double* __restrict__ xa, int nc,
+          int nb, int na, int zr2r1, double de, double aa, double re,
+          double kb, double cte, double cc) {
+  const long total = (long)nc * nb * na;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int a = (int)(idx % na);
+    const long cb = idx / na;
+    const int b = (int)(cb % nb);
+    const int c = (int)(cb / nb);
+    const double r1 = zr2r1 ? rc[c] : rb[b];
+    const double r2 = zr2r1 ? rb[b] : rc[c];
+    const double x = xa[a];
+    const double t1 = 1.0 - exp(-aa * (r1 - re));
+    const double t2 = 1.0 - exp(-aa * (r2 - re));
+    const double m1 = t1 * t1;
+    const double m2 = t2 * t2;
+    const double dx = x - cte;
+    // (m1+m2) and (r1-re)*(r2-re) are symmetric in r1<->r2 bit for bit.
+    v[idx] = de * (m1 + m2) + kb * (dx * dx) + cc * ((r1 - re) * (r2 - re));
+  }
+}
+
+void launch_pes_ab2(hipStream_t s, double* v, const double* rc,
+                    const double* rb, const double* xa, int nc, int nb, int na,
+                    int zr2r1, const double par[6]) {
+  const long total = (long)nc * nb * na;
+  int grid = (int)((total + 255) / 256);
+  if (grid > 65535 * 8) grid = 65535 * 8;
+  hipLaunchKernelGGL(k_pes_ab2, dim3(grid), dim3(256), 0, s, v, rc, rb, xa, nc,
+                     nb, na, zr2r1, par[0], par[1], par[2], par[3], par[4],
+                     par[5]);
+  DVR_LAUNCH_CHECK();
+}
+
+__global__ void __launch_bounds__(256)
+k_pack_1d(const double* __restrict__ v1full, const double* __restrict__ e1full,
+          const int* __restrict__ src_row, int rows, int n,
+          double* __restrict__ v1, double* __restrict__ e1p) {
+  const long total = (long)rows * n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(idx / n);
+    const int a = (int)(idx % n);
+    const long src = src_row[r];
+    v1[idx] = v1full[src * n + a];
+    if (a == 0) e1p[r] = e1full[src];
+  }
+}
+
+void launch_pack_1d(hipStream_t s, const double* v1full, const double* e1full,
+                    const int* src_row, int rows, int n, double* v1,
+                    double* e1p) {
+  if (rows <= 0) return;
+  const long total = (long)rows * n;
+  int grid = (int)((total + 255) / 256);
+  if (grid > 65535 * 8) grid = 65535 * 8;
+  hipLaunchKernelGGL(k_pack_1d, dim3(grid), dim3(256), 0, s, v1full, e1full,
+                     src_row, rows, n, v1, e1p);
+  DVR_LAUNCH_CHECK();
+}
+
+// One thread per (g, column) of cint.  Consecutive threads walk g with b
+// fastest, matching the column-major cint layout, so the stores coalesce.
+__global__ void __launch_bounds__(256)
+k_cint(const double* __restrict__ v1, const double* __restrict__ c2,
+       double* __restrict__ cint, const CBlock* __restrict__ cb,
+       const int* __restrict__ blk3, const int* __restrict__ iv1off, int na,
+       int nb, int n3) {
+  const long G = (long)na * nb;
+  const long total = G * n3;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int col = (int)(idx / G);
+    const int g = (int)(idx % G);
+    const int a = g / nb;
+    const int b = g % nb;
+    const int c = blk3[col];
+    const CBlock B = cb[c];
+    const int j = col - B.col0;
+    const int off = iv1off[c * (nb + 1) + b];
+    const int cnt = iv1off[c * (nb + 1) + b + 1] - off;
+    const double* vr = v1 + (long)(B.rowbase + off) * na + a;
+    const double* cr = c2 + B.z2off + off + (long)j * B.nham2;
+    double sum = 0.0;
+    for (int k = 0; k < cnt; ++k) sum += cr[k] * vr[(long)k * na];
+    cint[idx] = sum;
+  }
+}
+
+void launch_cint(hipStream_t s, const double* v1, const double* c2,
+                 double* cint, const CBlock* cb, const int* blk3,
+                 const int* iv1off, int na, int nb, int n3) {
+  const long total = (long)na * nb * n3;
+  if (total <= 0) return;
+  int grid = (int)((total + 255) / 256);
+  if (grid > 65535 * 16) grid = 65535 * 16;
+  hipLaunchKernelGGL(k_cint, dim3(grid), dim3(256), 0, s, v1, c2, cint, cb,
+                     blk3, iv1off, na, nb, n3);
+  DVR_LAUNCH_CHECK();
+}
+
+// psi_l(a, r1, r2) = sum_{j in block c} cint(g, col0+j) * c3(col0+j, l),
+// g = a*nb + b.  theta-first: zr2r1 -> (c=r1, b=r2), else (c=r2, b=r1).
+// One thread per (l, c, g); the inner product runs over the block's columns.
+__global__ void __launch_bounds__(256)
+k_trans(const double* __restrict__ cint, const double* __restrict__ c3,
+        double* __restrict__ psi, const CBlock* __restrict__ cb, int nc,
+        int na, int nb, int n3, int nvec, int zr2r1, int idvr, int npnt1,
+        int npnt2) {
+  const long G = (long)na * nb;
+  const long per = G * nc;
+  const long total = per * nvec;
+  const long npsi = (long)idvr * npnt1 * npnt2;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int l = (int)(idx / per);
+    const long rem = idx % per;
+    const int c = (int)(rem / G);
+    const int g = (int)(rem % G);
+    const int a = g / nb;
+    const int b = g % nb;
+    const CBlock B = cb[c];
+    double sum = 0.0;
+    const double* cp = cint + g + G * B.col0;
+    const double* vp = c3 + B.col0 + (long)n3 * l;
+    for (int j = 0; j < B.iv2; ++j) sum += cp[G * j] * vp[j];
+    long out;
+    if (zr2r1)
+      out = a + (long)idvr * (c + (long)npnt1 * b);
+    else
+      out = a + (long)idvr * (b + (long)npnt1 * c);
+    psi[npsi * l + out] = sum;
+  }
+}
+
+void launch_trans(hipStream_t s, const double* cint, const double* c3,
+                  double* psi, const CBlock* cb, int nc, int na, int nb,
+                  int n3, int nvec, int zr2r1, int idvr, int npnt1,
+                  int npnt2) {
+  const long total = (long)na * nb * nc * nvec;
+  if (total <= 0) return;
+  int grid = (int)((total + 255) / 256);
+  if (grid > 65535 * 16) grid = 65535 * 16;
+  hipLaunchKernelGGL(k_trans, dim3(grid), dim3(256), 0, s, cint, c3, psi, cb,
+                     nc, na, nb, n3, nvec, zr2r1, idvr, npnt1, npnt2);
+  DVR_LAUNCH_CHECK();
+}
+
+}  // namespace dvr

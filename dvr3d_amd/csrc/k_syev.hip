@@ -1,0 +1,611 @@
+// Batched dense symmetric eigensolver in FP64: the replacement for the
+// reference's  diag -> dsyev('V','L', ...)  calls
+// (source/dvr3drjz_segmented.f90:2638-2659; call sites D:1987, 2030, 2699).
+//
+// Stages (each batched over independent problems of unequal size):
+//   1. Householder reduction to tridiagonal form, A = Q T Q^T, one column at
+//      a time.  Per column j two launches:
+//        k_tri_col : finishes w_{j-1} = p - (tau/2)(p.v) v, applies the
+//                    pending rank-2 update to column j only, and forms the
+//                    reflector (v_j, tau_j) that annihilates A(j+2:n, j);
+//        k_tri_upd : applies the pending rank-2 update (v_{j-1}, w_{j-1}) to
+//                    the trailing columns and, in the same pass over the
+//                    matrix, forms p_j = tau_j A v_j.
+//      The reflectors stay in the lower part of A (v(j+1) = 1 implied).
+//   2. Gershgorin bounds; eigenvalues by Sturm-count multisection: one
+//      wavefront per wanted eigenvalue, 64 shifts per step.
+//   3. Eigenvectors of T by inverse iteration (tridiagonal LU with partial
+//      pivoting), with modified Gram-Schmidt inside clusters of close
+//      eigenvalues; one thread per cluster.
+//   4. Back-transformation Z <- Q Z in blocks of NB reflectors in WY form,
+//      Q_blk = I - V T V^T, one work-group per (problem, 8-column tile).
+#include "dvr_internal.h"
+#include <cfloat>
+
+namespace dvr {
+
+namespace {
+constexpr int TB = 256;
+constexpr int CB = 16;   // trailing columns per work-group in k_tri_upd
+constexpr int NB = 32;   // reflectors per WY block
+constexpr int CT = 8;    // columns per back-transform tile
+constexpr int RC = 64;   // rows per staged chunk in the back-transform
+constexpr double kEps = 2.220446049250313e-16;
+
+__device__ inline double block_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < TB / 64; ++i) s += red[i];
+  return s;
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+}  // namespace
+
+int backtransform_nb() { return NB; }
+
+// ---------------------------------------------------------------------------
+// 1. tridiagonalisation
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TB) k_tri_col(EigBatch b, int j) {
+  __shared__ double red[TB / 64];
+  const int p = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  double* A = b.A + b.aoff[p];
+  if (n == 1) {
+    if (j == 0 && tid == 0) b.d[vo] = A[0];
+    return;
+  }
+  if (j > n - 2) return;
+  double* vprev = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  double* vcur = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  double* w = b.wbuf + vo;
+  const double* pv = b.pbuf + vo;
+  double* col = A + (long)j * n;
+  if (j > 0) {
+    const double taup = b.tau[vo + j - 1];
+    double part = 0.0;
+    for (int i = j + tid; i < n; i += TB) part += pv[i] * vprev[i];
+    const double K = block_sum(part, red);
+    const double alpha = -0.5 * taup * K;
+    for (int i = j + tid; i < n; i += TB) w[i] = pv[i] + alpha * vprev[i];
+    __syncthreads();
+    const double wj = w[j], vj = vprev[j];
+    for (int i = j + tid; i < n; i += TB) col[i] -= (vprev[i] * wj + w[i] * vj);
+    __syncthreads();
+  }
+  if (j == n - 2) {
+    if (tid == 0) {
+      b.d[vo + j] = col[j];
+      b.e[vo + j] = col[n - 1];
+      b.tau[vo + j] = 0.0;
+      double dl = A[(n - 1) + (long)(n - 1) * n];
+      if (j > 0) dl -= (vprev[n - 1] * w[n - 1] + w[n - 1] * vprev[n - 1]);
+      b.d[vo + n - 1] = dl;
+    }
+    return;
+  }
+  const double alpha = col[j + 1];
+  double part = 0.0;
+  for (int i = j + 2 + tid; i < n; i += TB) part += col[i] * col[i];
+  const double sigma = block_sum(part, red);
+  double tau, beta, scal;
+  if (sigma == 0.0) {
+    tau = 0.0;
+    beta = alpha;
+    scal = 0.0;
+  } else {
+    const double mu = sqrt(alpha * alpha + sigma);
+    beta = (alpha <= 0.0) ? mu : -mu;
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  for (int i = j + 1 + tid; i < n; i += TB) {
+    const double vi = (i == j + 1) ? 1.0 : col[i] * scal;
+    vcur[i] = vi;
+    if (i > j + 1) col[i] = vi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    b.d[vo + j] = col[j];
+    b.e[vo + j] = beta;
+    b.tau[vo + j] = tau;
+    col[j + 1] = beta;
+  }
+}
+
+__global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
+  const int bid = blockIdx.x;
+  int p = 0, base = 0;
+  for (; p < b.nprob; ++p) {
+    const int n = b.n[p];
+    const int nb = (j <= n - 3) ? (n - j - 1 + CB - 1) / CB : 0;
+    if (bid < base + nb) break;
+    base += nb;
+  }
+  if (p >= b.nprob) return;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  double* A = b.A + b.aoff[p];
+  const double* vprev = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  const double* vcur = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  const double* w = b.wbuf + vo;
+  double* pv = b.pbuf + vo;
+  const double tau = b.tau[vo + j];
+  const int k0 = j + 1 + (bid - base) * CB;
+  const int k1 = min(k0 + CB, n);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int k = k0 + wave; k < k1; k += TB / 64) {
+    double* colk = A + (long)k * n;
+    double acc = 0.0;
+    if (j > 0) {
+      const double wk = w[k], vk = vprev[k];
+      for (int i = j + 1 + lane; i < n; i += 64) {
+        const double a = colk[i] - (vprev[i] * wk + w[i] * vk);
+        colk[i] = a;
+        acc += a * vcur[i];
+      }
+    } else {
+      for (int i = j + 1 + lane; i < n; i += 64) acc += colk[i] * vcur[i];
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) pv[k] = tau * acc;
+  }
+}
+
+void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
+  int nmax = 0;
+  for (int p = 0; p < b.nprob; ++p) nmax = h_n[p] > nmax ? h_n[p] : nmax;
+  if (nmax <= 0) return;
+  const int jmax = nmax >= 2 ? nmax - 2 : 0;
+  for (int j = 0; j <= jmax; ++j) {
+    hipLaunchKernelGGL(k_tri_col, dim3(b.nprob), dim3(TB), 0, s, b, j);
+    int grid = 0;
+    for (int p = 0; p < b.nprob; ++p)
+      if (j <= h_n[p] - 3) grid += (h_n[p] - j - 1 + CB - 1) / CB;
+    if (grid > 0) hipLaunchKernelGGL(k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
+  }
+  DVR_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// 2. bounds and multisection
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TB) k_tri_bounds(EigBatch b) {
+  __shared__ double rmin[TB], rmax[TB];
+  const int p = blockIdx.x;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  const double* d = b.d + vo;
+  const double* e = b.e + vo;
+  double* e2 = b.e2 + vo;
+  double lo = DBL_MAX, hi = -DBL_MAX, m2 = 0.0;
+  for (int i = threadIdx.x; i < n; i += TB) {
+    const double em = i > 0 ? fabs(e[i - 1]) : 0.0;
+    const double ep = i < n - 1 ? fabs(e[i]) : 0.0;
+    lo = fmin(lo, d[i] - em - ep);
+    hi = fmax(hi, d[i] + em + ep);
+    if (i < n - 1) {
+      const double q = e[i] * e[i];
+      e2[i] = q;
+      m2 = fmax(m2, q);
+    }
+  }
+  rmin[threadIdx.x] = lo;
+  rmax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int st = TB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      rmin[threadIdx.x] = fmin(rmin[threadIdx.x], rmin[threadIdx.x + st]);
+      rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  const double glo = rmin[0], ghi = rmax[0];
+  __syncthreads();
+  rmax[threadIdx.x] = m2;
+  __syncthreads();
+  for (int st = TB / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) rmax[threadIdx.x] = fmax(rmax[threadIdx.x], rmax[threadIdx.x + st]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double tn = fmax(fabs(glo), fabs(ghi));
+    const double piv = DBL_MIN * fmax(1.0, rmax[0]);
+    const double pad = 2.1 * kEps * tn * n + 4.0 * piv;
+    b.gl[p] = glo - pad;
+    b.gu[p] = ghi + pad;
+    b.tnrm[p] = tn;
+    b.pivmin[p] = piv;
+  }
+}
+
+void launch_tridiag_bounds(hipStream_t s, const EigBatch& b) {
+  hipLaunchKernelGGL(k_tri_bounds, dim3(b.nprob), dim3(TB), 0, s, b);
+  DVR_LAUNCH_CHECK();
+}
+
+__device__ inline int sturm_count(const double* __restrict__ d,
+                                  const double* __restrict__ e2, int n,
+                                  double x, double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q <= 0.0;
+  for (int i = 1; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q <= 0.0;
+  }
+  return c;
+}
+
+__global__ void __launch_bounds__(TB)
+k_bisect(EigBatch b, const int* __restrict__ il, const int* __restrict__ kofs,
+         int total, double* __restrict__ eval, const long* __restrict__ eoff) {
+  const int lane = threadIdx.x & 63;
+  const int gid = __builtin_amdgcn_readfirstlane(blockIdx.x * (TB / 64) + (threadIdx.x >> 6));
+  if (gid >= total) return;
+  int lo_p = 0, hi_p = b.nprob - 1;
+  while (lo_p < hi_p) {  // largest p with kofs[p] <= gid
+    const int mid = (lo_p + hi_p + 1) >> 1;
+    if (kofs[mid] <= gid) lo_p = mid; else hi_p = mid - 1;
+  }
+  const int p = __builtin_amdgcn_readfirstlane(lo_p);
+  const int kloc = gid - kofs[p];
+  const int k = il[p] + kloc;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  const double* d = b.d + vo;
+  const double* e2 = b.e2 + vo;
+  const double piv = b.pivmin[p];
+  double lo = b.gl[p], hi = b.gu[p];
+  for (int it = 0; it < 48; ++it) {
+    const double width = hi - lo;
+    if (width <= 2.0 * kEps * fmax(fabs(lo), fabs(hi)) || width <= 4.0 * piv) break;
+    const double step = width / 65.0;
+    const double x = lo + (double)(lane + 1) * step;
+    const int c = sturm_count(d, e2, n, x, piv);
+    const unsigned long long mask = __ballot(c <= k);
+    double nlo, nhi;
+    if (mask == 0ull) {
+      nlo = lo;
+      nhi = lo + 1.0 * step;
+    } else {
+      const int L = 63 - __clzll((long long)mask);
+      nlo = lo + (double)(L + 1) * step;
+      nhi = (L == 63) ? hi : lo + (double)(L + 2) * step;
+    }
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (lane == 0) eval[eoff[p] + kloc] = 0.5 * (lo + hi);
+}
+
+void launch_bisect(hipStream_t s, const EigBatch& b, const int* il,
+                   const int* kofs, int total, double* eval,
+                   const long* eoff) {
+  if (total <= 0) return;
+  const int grid = (total + TB / 64 - 1) / (TB / 64);
+  hipLaunchKernelGGL(k_bisect, dim3(grid), dim3(TB), 0, s, b, il, kofs, total,
+                     eval, eoff);
+  DVR_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// 3. inverse iteration
+// ---------------------------------------------------------------------------
+__device__ inline double unit_hash(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x = x ^ (x >> 31);
+  return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+size_t stein_scratch_doubles(int nmax, int nthreads) {
+  return (size_t)6 * (size_t)nmax * (size_t)nthreads;
+}
+
+__global__ void __launch_bounds__(TB)
+k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
+        const double* __restrict__ eval, const long* __restrict__ eoff,
+        double* __restrict__ Z, const long* __restrict__ zoff,
+        double* __restrict__ scratch, int T) {
+  const int t = blockIdx.x * TB + threadIdx.x;
+  if (t >= ncl) return;
+  const Cluster C = cl[t];
+  const int p = C.p;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  const double* d = b.d + vo;
+  const double* e = b.e + vo;
+  const double* ev = eval + eoff[p];
+  double* Zp = Z + zoff[p];
+  const long nm = b.nmax;
+  double* DL = scratch + t;
+  double* DD = DL + nm * T;
+  double* DU = DD + nm * T;
+  double* DU2 = DU + nm * T;
+  double* PV = DU2 + nm * T;
+  double* X = PV + nm * T;
+#define AT(arr, i) arr[(long)(i) * T]
+  const double tn = b.tnrm[p] > 0.0 ? b.tnrm[p] : 1.0;
+  const double pertol = 10.0 * kEps * tn;
+  const double tiny = kEps * tn;
+  double lprev = 0.0;
+  for (int k = C.kb; k < C.ke; ++k) {
+    double lam = ev[k];
+    if (k > C.kb && lam - lprev < pertol) lam = lprev + pertol;
+    lprev = lam;
+    // T - lam I = P L U (dgttrf-style partial pivoting)
+    for (int i = 0; i < n; ++i) {
+      AT(DD, i) = d[i] - lam;
+      if (i < n - 1) {
+        AT(DL, i) = e[i];
+        AT(DU, i) = e[i];
+      }
+      AT(DU2, i) = 0.0;
+      AT(PV, i) = 0.0;
+    }
+    for (int i = 0; i < n - 1; ++i) {
+      const double di = AT(DD, i), li = AT(DL, i);
+      if (fabs(di) >= fabs(li)) {
+        if (di != 0.0) {
+          const double f = li / di;
+          AT(DL, i) = f;
+          AT(DD, i + 1) -= f * AT(DU, i);
+        } else {
+          AT(DD, i) = tiny;
+          AT(DL, i) = 0.0;
+        }
+      } else {
+        const double f = di / li;
+        AT(DD, i) = li;
+        AT(DL, i) = f;
+        const double tmp = AT(DU, i);
+        AT(DU, i) = AT(DD, i + 1);
+        AT(DD, i + 1) = tmp - f * AT(DD, i + 1);
+        if (i < n - 2) {
+          AT(DU2, i) = AT(DU, i + 1);
+          AT(DU, i + 1) = -f * AT(DU, i + 1);
+        }
+        AT(PV, i) = 1.0;
+      }
+    }
+    for (int i = 0; i < n; ++i)
+      if (AT(DD, i) == 0.0) AT(DD, i) = tiny;
+    for (int i = 0; i < n; ++i)
+      AT(X, i) = unit_hash(((unsigned long long)p << 40) ^ ((unsigned long long)k << 20) ^ (unsigned long long)i);
+    for (int it = 0; it < 3; ++it) {
+      for (int i = 0; i < n - 1; ++i) {
+        if (AT(PV, i) == 0.0) {
+          AT(X, i + 1) -= AT(DL, i) * AT(X, i);
+        } else {
+          const double tmp = AT(X, i);
+          AT(X, i) = AT(X, i + 1);
+          AT(X, i + 1) = tmp - AT(DL, i) * AT(X, i);
+        }
+      }
+      AT(X, n - 1) /= AT(DD, n - 1);
+      if (n > 1) AT(X, n - 2) = (AT(X, n - 2) - AT(DU, n - 2) * AT(X, n - 1)) / AT(DD, n - 2);
+      for (int i = n - 3; i >= 0; --i)
+        AT(X, i) = (AT(X, i) - AT(DU, i) * AT(X, i + 1) - AT(DU2, i) * AT(X, i + 2)) / AT(DD, i);
+      // scale to avoid growth across iterations, then orthogonalise
+      double mx = 0.0;
+      for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(AT(X, i)));
+      const double sc = mx > 0.0 ? 1.0 / mx : 1.0;
+      for (int i = 0; i < n; ++i) AT(X, i) *= sc;
+      for (int q = C.kb; q < k; ++q) {
+        const double* zq = Zp + (long)q * n;
+        double dot = 0.0;
+        for (int i = 0; i < n; ++i) dot += AT(X, i) * zq[i];
+        for (int i = 0; i < n; ++i) AT(X, i) -= dot * zq[i];
+      }
+      double nrm = 0.0;
+      for (int i = 0; i < n; ++i) nrm += AT(X, i) * AT(X, i);
+      nrm = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+      for (int i = 0; i < n; ++i) AT(X, i) *= nrm;
+    }
+    double* zk = Zp + (long)k * n;
+    for (int i = 0; i < n; ++i) zk[i] = AT(X, i);
+  }
+#undef AT
+}
+
+void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
+                  int ncl, const double* eval, const long* eoff, double* Z,
+                  const long* zoff, double* scratch, int nthreads) {
+  if (ncl <= 0) return;
+  const int grid = (ncl + TB - 1) / TB;
+  hipLaunchKernelGGL(k_stein, dim3(grid), dim3(TB), 0, s, b, cl, ncl, eval,
+                     eoff, Z, zoff, scratch, nthreads);
+  DVR_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------------------
+// 4. back-transformation
+// ---------------------------------------------------------------------------
+// Reflector r of block blk: index j = blk*NB + r, vector v(i) for i > j with
+// v(j+1) = 1 and v(i) = A(i, j) below.  T is the upper-triangular factor of
+// the forward column-wise WY form (H_j0 ... H_j1 = I - V T V^T).
+__device__ inline double refl(const double* A, int n, int j, int i) {
+  if (i <= j) return 0.0;
+  if (i == j + 1) return 1.0;
+  return A[i + (long)j * n];
+}
+
+__global__ void __launch_bounds__(TB)
+k_larft(EigBatch b, const int* __restrict__ nblk_ofs, double* __restrict__ tbuf,
+        const long* __restrict__ toff) {
+  __shared__ double G[NB * NB];
+  __shared__ double Ts[NB * NB];
+  const int bid = blockIdx.x;
+  int lo_p = 0, hi_p = b.nprob - 1;
+  while (lo_p < hi_p) {
+    const int mid = (lo_p + hi_p + 1) >> 1;
+    if (nblk_ofs[mid] <= bid) lo_p = mid; else hi_p = mid - 1;
+  }
+  const int p = lo_p;
+  const int blk = bid - nblk_ofs[p];
+  const int n = b.n[p];
+  const int nr = n - 2;
+  const int j0 = blk * NB;
+  if (nr <= 0 || j0 >= nr) return;
+  const int nbk = min(NB, nr - j0);
+  const double* A = b.A + b.aoff[p];
+  const double* tau = b.tau + b.voff[p];
+  for (int idx = threadIdx.x; idx < NB * NB; idx += TB) {
+    G[idx] = 0.0;
+    Ts[idx] = 0.0;
+  }
+  __syncthreads();
+  // G(r,c) = v_r . v_c for r < c
+  for (int idx = threadIdx.x; idx < nbk * nbk; idx += TB) {
+    const int r = idx % nbk, c = idx / nbk;
+    if (r >= c) continue;
+    const int jr = j0 + r, jc = j0 + c;
+    double s = refl(A, n, jr, jc + 1);
+    for (int i = jc + 2; i < n; ++i) s += A[i + (long)jr * n] * A[i + (long)jc * n];
+    G[r + NB * c] = s;
+  }
+  __syncthreads();
+  for (int c = 0; c < nbk; ++c) {
+    const double tc = tau[j0 + c];
+    if (threadIdx.x < c) {
+      const int r = threadIdx.x;
+      double s = 0.0;
+      for (int l = r; l < c; ++l) s += Ts[r + NB * l] * G[l + NB * c];
+      Ts[r + NB * c] = -tc * s;
+    }
+    if (threadIdx.x == 0) Ts[c + NB * c] = tc;
+    __syncthreads();
+  }
+  double* Tout = tbuf + toff[p] + (long)blk * NB * NB;
+  for (int idx = threadIdx.x; idx < NB * NB; idx += TB) Tout[idx] = Ts[idx];
+}
+
+__global__ void __launch_bounds__(TB)
+k_ormtr(EigBatch b, double* __restrict__ Z, const long* __restrict__ zoff,
+        const int* __restrict__ m, const int* __restrict__ tile_ofs,
+        const double* __restrict__ tbuf, const long* __restrict__ toff) {
+  __shared__ double Vs[RC * (NB + 1)];
+  __shared__ double Xs[RC * CT];
+  __shared__ double W[NB * CT];
+  __shared__ double W2[NB * CT];
+  const int bid = blockIdx.x;
+  int lo_p = 0, hi_p = b.nprob - 1;
+  while (lo_p < hi_p) {
+    const int mid = (lo_p + hi_p + 1) >> 1;
+    if (tile_ofs[mid] <= bid) lo_p = mid; else hi_p = mid - 1;
+  }
+  const int p = lo_p;
+  const int tile = bid - tile_ofs[p];
+  const int n = b.n[p];
+  const int mp = m[p];
+  const int c0 = tile * CT;
+  if (c0 >= mp) return;
+  const int ct = min(CT, mp - c0);
+  const int nr = n - 2;
+  if (nr <= 0) return;
+  const double* A = b.A + b.aoff[p];
+  double* Zp = Z + zoff[p] + (long)c0 * n;
+  const double* Tp = tbuf + toff[p];
+  const int nblk = (nr + NB - 1) / NB;
+  const int tid = threadIdx.x;
+  const int tr = tid / CT, tc = tid % CT;  // (r, c) owner for W
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int j0 = blk * NB;
+    const int nbk = min(NB, nr - j0);
+    const int i0 = j0 + 1;
+    // W = V^T X
+    double acc = 0.0;
+    for (int ib = i0; ib < n; ib += RC) {
+      __syncthreads();
+      for (int idx = tid; idx < RC * NB; idx += TB) {
+        const int rr = idx % RC, r = idx / RC;
+        const int i = ib + rr;
+        Vs[rr * (NB + 1) + r] = (i < n && r < nbk) ? refl(A, n, j0 + r, i) : 0.0;
+      }
+      for (int idx = tid; idx < RC * CT; idx += TB) {
+        const int rr = idx % RC, c = idx / RC;
+        const int i = ib + rr;
+        Xs[rr * CT + c] = (i < n && c < ct) ? Zp[i + (long)c * n] : 0.0;
+      }
+      __syncthreads();
+      for (int rr = 0; rr < RC; ++rr) acc += Vs[rr * (NB + 1) + tr] * Xs[rr * CT + tc];
+    }
+    W[tr * CT + tc] = acc;
+    __syncthreads();
+    // W2 = T W  (T upper triangular)
+    {
+      double s = 0.0;
+      const double* Tb = Tp + (long)blk * NB * NB;
+      for (int l = tr; l < nbk; ++l) s += Tb[tr + NB * l] * W[l * CT + tc];
+      W2[tr * CT + tc] = (tr < nbk) ? s : 0.0;
+    }
+    __syncthreads();
+    // X -= V W2
+    for (int ib = i0; ib < n; ib += RC) {
+      __syncthreads();
+      for (int idx = tid; idx < RC * NB; idx += TB) {
+        const int rr = idx % RC, r = idx / RC;
+        const int i = ib + rr;
+        Vs[rr * (NB + 1) + r] = (i < n && r < nbk) ? refl(A, n, j0 + r, i) : 0.0;
+      }
+      __syncthreads();
+      for (int idx = tid; idx < RC * CT; idx += TB) {
+        const int rr = idx % RC, c = idx / RC;
+        const int i = ib + rr;
+        if (i >= n || c >= ct) continue;
+        double s = 0.0;
+        for (int r = 0; r < nbk; ++r) s += Vs[rr * (NB + 1) + r] * W2[r * CT + c];
+        Zp[i + (long)c * n] -= s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
+                          const long* zoff, const int* m, const int* h_n,
+                          const int* h_m, double* tbuf, const long* toff,
+                          long tbuf_total) {
+  (void)tbuf_total;
+  // host-side offsets of blocks and tiles, uploaded through a small buffer
+  const int np = b.nprob;
+  int* h = (int*)malloc(sizeof(int) * 2 * (np + 1));
+  int nbt = 0, ntt = 0;
+  for (int p = 0; p < np; ++p) {
+    h[p] = nbt;
+    h[np + 1 + p] = ntt;
+    const int nr = h_n[p] - 2;
+    nbt += nr > 0 ? (nr + NB - 1) / NB : 0;
+    ntt += (nr > 0 && h_m[p] > 0) ? (h_m[p] + CT - 1) / CT : 0;
+  }
+  h[np] = nbt;
+  h[np + 1 + np] = ntt;
+  int* d = nullptr;
+  DVR_HIP(hipMallocAsync((void**)&d, sizeof(int) * 2 * (np + 1), s));
+  DVR_HIP(hipMemcpyAsync(d, h, sizeof(int) * 2 * (np + 1), hipMemcpyHostToDevice, s));
+  if (nbt > 0)
+    hipLaunchKernelGGL(k_larft, dim3(nbt), dim3(TB), 0, s, b, d, tbuf, toff);
+  if (ntt > 0)
+    hipLaunchKernelGGL(k_ormtr, dim3(ntt), dim3(TB), 0, s, b, Z, zoff, m,
+                       d + np + 1, tbuf, toff);
+  DVR_LAUNCH_CHECK();
+  DVR_HIP(hipFreeAsync(d, s));
+  DVR_HIP(hipStreamSynchronize(s));
+  free(h);
+}
+
+}  // namespace dvr

@@ -1,0 +1,107 @@
+// Grouped symmetric rank-K products with a block-constant scale, on FP64 MFMA.
+//
+//   C = (X^T X) .* S + diag(dg),   S(i,j) = xk(blk(i), blk(j))
+//
+// This is the shape of both contraction stages of the theta-first path:
+//  * mkham2 (source/dvr3drjz_segmented.f90:2228-2298): X = the kept 1D
+//    eigenvectors of one r1 block (K = npnta angles, N = nham2 functions),
+//    blk = the r2 point of each function, xk = xk2, dg = 1D eigenvalues;
+//  * mkham3 loop 81 + loadh (D:2459-2463, D:2554-2609): X = cint (K = npnta *
+//    npntb grid points, N = nham3), blk = the r1 block, xk = xk1,
+//    dg = the chosen 2D eigenvalues.
+// Only lower-triangular 64x64 tiles are computed; each writes its block and
+// the mirrored block.  Work-group = 4 wavefronts, each a 32x32 sub-tile of
+// 2x2 v_mfma_f64_16x16x4f64 accumulators, operands staged through LDS.
+#include "dvr_internal.h"
+
+namespace dvr {
+
+namespace {
+constexpr int TM = 64;   // tile edge
+constexpr int KT = 16;   // k depth per LDS stage
+constexpr int LDT = TM + 1;
+typedef double v4d __attribute__((ext_vector_type(4)));
+}  // namespace
+
+int syrk_tile() { return TM; }
+
+__global__ void __launch_bounds__(256)
+k_syrk_scaled(const SyrkDesc* __restrict__ desc, int ngroup,
+              const double* __restrict__ xk, int ldxk) {
+  __shared__ double As[KT * LDT];
+  __shared__ double Bs[KT * LDT];
+  const int bid = blockIdx.x;
+  int g = 0;
+  while (g + 1 < ngroup && desc[g + 1].tile0 <= bid) ++g;
+  const SyrkDesc D = desc[g];
+  const int t = bid - D.tile0;
+  if (t >= D.ntile) return;
+  // t -> (ti, tj), ti >= tj
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  const int row0 = ti * TM, col0 = tj * TM;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wy = wave >> 1, wx = wave & 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  v4d acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const double* X = D.X;
+  const long ldx = D.ldx;
+  const int K = D.K, N = D.N;
+  for (int k0 = 0; k0 < K; k0 += KT) {
+    for (int idx = tid; idx < KT * TM; idx += 256) {
+      const int kk = idx % KT;
+      const int cc = idx / KT;
+      const int k = k0 + kk;
+      const int ri = row0 + cc, cj = col0 + cc;
+      As[kk * LDT + cc] = (k < K && ri < N) ? X[k + (long)ri * ldx] : 0.0;
+      Bs[kk * LDT + cc] = (k < K && cj < N) ? X[k + (long)cj * ldx] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < KT; k4 += 4) {
+      const int kr = (k4 + lk) * LDT;
+      double a0 = As[kr + wy * 32 + l16];
+      double a1 = As[kr + wy * 32 + 16 + l16];
+      double b0 = Bs[kr + wx * 32 + l16];
+      double b1 = Bs[kr + wx * 32 + 16 + l16];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  double* C = D.C;
+  const int* blk = D.blk;
+  const double* dg = D.diag;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + wy * 32 + bi * 16 + 4 * lk + r;
+        const int j = col0 + wx * 32 + bj * 16 + l16;
+        if (i >= N || j >= N) continue;
+        double v = __dmul_rn(acc[bi][bj][r], xk[blk[i] + (long)ldxk * blk[j]]);
+        if (i == j && dg) v = __dadd_rn(v, dg[i]);
+        C[i + (long)N * j] = v;
+        if (ti != tj) C[j + (long)N * i] = v;
+      }
+}
+
+void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
+                        int total_tiles, const double* xk, int ldxk) {
+  if (total_tiles <= 0) return;
+  hipLaunchKernelGGL(k_syrk_scaled, dim3(total_tiles), dim3(256), 0, s, d_desc,
+                     ngroup, xk, ldxk);
+  DVR_LAUNCH_CHECK();
+}
+
+}  // namespace dvr

@@ -1,0 +1,98 @@
+! ISO_C_BINDING interface of libdvr3d_amd (include/dvr3d_amd.h).
+module dvr3d_amd_api
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: DVR_OK = 0, DVR_ERR_ARG = 1, DVR_ERR_HIP = 2
+  integer(c_int), parameter :: DVR_ERR_CORE_2D = 3, DVR_ERR_CORE_3D = 4, DVR_ERR_NOCONV = 5
+  integer(c_int), parameter :: DVR_SOLVER_DENSE = 0, DVR_SOLVER_DAVIDSON = 1
+  integer(c_int), parameter :: DVR_NSTAGE = 7
+
+  type, bind(C) :: dvr_dims_t
+    integer(c_int) :: npnta, npntb, npntc
+    integer(c_int) :: max2d, max3d, neval
+    integer(c_int) :: zcut, zall, zr2r1
+    integer(c_int) :: idvr, npnt1, npnt2
+    integer(c_int) :: nvec, solver
+  end type
+
+  interface
+    integer(c_int) function dvr_ctx_create(dims, device, ctx) bind(C, name="dvr_ctx_create")
+      import :: c_int, c_ptr, dvr_dims_t
+      type(dvr_dims_t), intent(in) :: dims
+      integer(c_int), value :: device
+      type(c_ptr), intent(out) :: ctx
+    end function
+    subroutine dvr_ctx_destroy(ctx) bind(C, name="dvr_ctx_destroy")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+    type(c_ptr) function dvr_last_error(ctx) bind(C, name="dvr_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function dvr_set_basis(ctx, xlmatr, xkb, xkc, wsum, wterm, emax1, emax2) &
+        bind(C, name="dvr_set_basis")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: xlmatr(*), xkb(*), xkc(*), wsum(*), wterm(*)
+      real(c_double), value :: emax1, emax2
+    end function
+    integer(c_int) function dvr_set_potential(ctx, v) bind(C, name="dvr_set_potential")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: v(*)
+    end function
+    integer(c_int) function dvr_run(ctx) bind(C, name="dvr_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function dvr_get_sizes(ctx, iv1, iv2, nham3) bind(C, name="dvr_get_sizes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: iv1(*), iv2(*), nham3
+    end function
+    integer(c_int) function dvr_get_eigenvalues(ctx, eval, meval) bind(C, name="dvr_get_eigenvalues")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: eval(*)
+      integer(c_int), intent(out) :: meval
+    end function
+    integer(c_int) function dvr_get_wavefunctions(ctx, psi) bind(C, name="dvr_get_wavefunctions")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: psi(*)
+    end function
+    integer(c_int) function dvr_get_stage_times(ctx, ms) bind(C, name="dvr_get_stage_times")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ms(*)
+    end function
+    integer(c_int) function dvr_diag(ham, maxham, nham, eig) bind(C, name="dvr_diag")
+      import :: c_int, c_double
+      real(c_double), intent(inout) :: ham(*)
+      integer(c_int), value :: maxham, nham
+      real(c_double), intent(out) :: eig(*)
+    end function
+  end interface
+
+contains
+
+  ! Message of the last failed call on ctx, as a Fortran string.
+  function dvr_error_text(ctx) result(s)
+    type(c_ptr), intent(in) :: ctx
+    character(len=256) :: s
+    character(kind=c_char), pointer :: p(:)
+    type(c_ptr) :: cp
+    integer :: i
+    s = ' '
+    cp = dvr_last_error(ctx)
+    if (.not. c_associated(cp)) return
+    call c_f_pointer(cp, p, [256])
+    do i = 1, 256
+      if (p(i) == c_null_char) exit
+      s(i:i) = p(i)
+    end do
+  end function
+
+end module dvr3d_amd_api

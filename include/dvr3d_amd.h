@@ -1,0 +1,155 @@
+/*
+ * dvr3d_amd.h -- C ABI of the MI355X (gfx950) implementation of the DVR3DRJZ
+ * theta-first vibrational Hamiltonian path.
+ *
+ * The reference program (ExoMol DVR3D, source/dvr3drjz_segmented.f90, cited
+ * below as D:<line>) has no foreign-function interface; its replaceable units
+ * are Fortran subroutines that share COMMON blocks and scratch files.  This
+ * ABI exposes the same units with plain pointers and sizes so that a Fortran
+ * host binds them with ISO_C_BINDING:
+ *
+ *   dvr_diag            <- SUBROUTINE diag(ham,maxham,nham,eig)     D:2638-2659
+ *   dvr_ctx_* + dvr_run <- the body of SUBROUTINE jhmain            D:1915-2118
+ *                          (mkham1 D:2121, diag/cut1d D:1987/2870, mkham2
+ *                          D:2228, choose/cut2d D:2767/2941, mkham3 D:2301,
+ *                          loadh D:2554, diag3d D:2661, trans D:3650)
+ *
+ * Conventions: 32-bit ints, IEEE double; matrices column-major like the
+ * Fortran caller; errors are returned as an int (0 = success) and the message
+ * is available from dvr_last_error(); no C++ exception crosses the ABI.  One
+ * context owns its device memory and one HIP stream; calls on one context must
+ * not overlap.
+ */
+#ifndef DVR3D_AMD_H
+#define DVR3D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DVR_OK 0
+#define DVR_ERR_ARG 1       /* invalid argument or call order */
+#define DVR_ERR_HIP 2       /* HIP runtime error (message has details) */
+#define DVR_ERR_CORE_2D 3   /* "core exceeded: reduce cut-off emax1" (D:2923-2929) */
+#define DVR_ERR_CORE_3D 4   /* "core exceeded: reduce cut-off emax2" (D:2981-2985) */
+#define DVR_ERR_NOCONV 5    /* iterative solver did not converge */
+
+/* final-solve methods */
+#define DVR_SOLVER_DENSE 0     /* dense FP64 eigensolver (plain form) */
+#define DVR_SOLVER_DAVIDSON 1  /* block-Davidson, dense fallback (chosen form) */
+
+/* stage timers reported by dvr_get_stage_times (milliseconds) */
+#define DVR_STAGE_1D 0       /* 1D build + Jacobi + cut            */
+#define DVR_STAGE_2D_BUILD 1 /* packing + mkham2 SYRK             */
+#define DVR_STAGE_2D_EIG 2   /* 2D eigenvalues (tridiag+bisect)   */
+#define DVR_STAGE_2D_VEC 3   /* chosen 2D eigenvectors            */
+#define DVR_STAGE_3D_BUILD 4 /* cint + mkham3 SYRK + loadh        */
+#define DVR_STAGE_3D_EIG 5   /* final eigenvalues                 */
+#define DVR_STAGE_3D_VEC 6   /* final eigenvectors + trans        */
+#define DVR_NSTAGE 7
+
+typedef struct dvr_ctx dvr_ctx_t;
+
+/*
+ * Problem dimensions after the reference's insize clamps (D:232-355).
+ * theta-first order (ztheta=T): a = angle (npnta = nalf/max(1,idia)),
+ * b = the second coordinate and c = the last one; zr2r1=1 means b = r2 and
+ * c = r1 (D:310-318).
+ */
+typedef struct dvr_dims {
+  int32_t npnta, npntb, npntc;
+  int32_t max2d, max3d, neval;
+  int32_t zcut;   /* 1: final basis by energy cut emax2 (cut2d), 0: lowest max3d (choose) */
+  int32_t zall;   /* 1: no truncation of intermediate solutions */
+  int32_t zr2r1;
+  int32_t idvr, npnt1, npnt2; /* wavefunction record layout psi(idvr,npnt1,npnt2) */
+  int32_t nvec;   /* number of lowest levels whose wavefunctions are wanted (0..neval) */
+  int32_t solver; /* DVR_SOLVER_* */
+} dvr_dims_t;
+
+/* Create / destroy a context on HIP device `device`. */
+int dvr_ctx_create(const dvr_dims_t* dims, int32_t device, dvr_ctx_t** out);
+void dvr_ctx_destroy(dvr_ctx_t* ctx);
+const char* dvr_last_error(const dvr_ctx_t* ctx);
+
+/*
+ * Host-computed basis arrays for one k block (copied; caller keeps ownership).
+ *   xlmatr [npnta*npnta]  angular L matrix (lmatrx, D:1846-1911)
+ *   xkb    [npntb*npntb]  DVR kinetic matrix of coordinate b (k1k2, D:1603)
+ *   xkc    [npntc*npntc]  DVR kinetic matrix of coordinate c
+ *   wsum   [npntc*npntb]  (w1+w2) at (c,b), index c*npntb + b (mkham1 D:2155-2157)
+ *   wterm  [npntc*npntb]  rotational diagonal shift, or NULL for J=0 (D:2159-2173)
+ *   emax1, emax2          cut-off energies in hartree (EMAX1/EMAX2 card / autocm)
+ */
+int dvr_set_basis(dvr_ctx_t* ctx, const double* xlmatr, const double* xkb,
+                  const double* xkc, const double* wsum, const double* wterm,
+                  double emax1, double emax2);
+
+/* Potential on the grid, V[(c*npntb + b)*npnta + a] in hartree, as returned by
+ * the user's POTV(V,R1,R2,XCOS) at the grid points (mkham1, D:2178). */
+int dvr_set_potential(dvr_ctx_t* ctx, const double* v);
+
+/*
+ * Evaluate the analytic AB2 surface on the device instead:
+ *   V = De[(1-exp(-a(r1-re)))^2 + (1-exp(-a(r2-re)))^2]
+ *       + kb (cos(theta) - cte)^2 + c (r1-re)(r2-re)
+ * rc[npntc], rb[npntb] are the radial points of coordinates c and b, xa[npnta]
+ * the angular points; par = {De, a, re, kb, cte, c}.
+ */
+int dvr_eval_potential_ab2(dvr_ctx_t* ctx, const double* rc, const double* rb,
+                           const double* xa, const double* par);
+
+/* Run 1D -> 2D -> selection -> 3D -> final solve (-> wavefunctions). */
+int dvr_run(dvr_ctx_t* ctx);
+
+/* Truncation results: iv1 as the reference's iv1(ndimc,ndimb) (column-major,
+ * iv1[c + npntc*b]), iv2[npntc], nham3 (D:1997, D:2053, D:2086). */
+int dvr_get_sizes(const dvr_ctx_t* ctx, int32_t* iv1, int32_t* iv2,
+                  int32_t* nham3);
+
+/* Lowest meval = min(neval, nham3) eigenvalues in hartree, ascending. */
+int dvr_get_eigenvalues(const dvr_ctx_t* ctx, double* eval, int32_t* meval);
+
+/* All 1D eigenvalues e1[(c*npntb + b)*npnta + k] (ascending in k). */
+int dvr_get_1d_eigenvalues(const dvr_ctx_t* ctx, double* e1);
+
+/* All 2D eigenvalues of every c block, concatenated in c order (sum of nham2). */
+int dvr_get_2d_eigenvalues(const dvr_ctx_t* ctx, double* e2, int32_t* total);
+
+/* Wavefunction amplitudes of the nvec lowest levels, each level a record
+ * psi(idvr,npnt1,npnt2) as written to IWAVE by trans (D:3841). */
+int dvr_get_wavefunctions(const dvr_ctx_t* ctx, double* psi);
+
+/* Final eigenvectors in the contracted basis, nham3 x nvec column-major. */
+int dvr_get_vectors(const dvr_ctx_t* ctx, double* vec);
+
+/* Per-stage device times of the last dvr_run, DVR_NSTAGE doubles (ms). */
+int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
+
+/*
+ * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)
+ * (D:2638-2659: dsyev('V','L',nham,ham,maxham,eig,...)).  On entry the lower
+ * triangle of the column-major ham(maxham,nham) holds the symmetric matrix; on
+ * return columns 1..nham hold orthonormal eigenvectors and eig(nham) the
+ * eigenvalues in ascending order.  Runs on the current HIP device.
+ */
+int dvr_diag(double* ham, int32_t maxham, int32_t nham, double* eig);
+
+/*
+ * Batched form used by the pipeline, exposed for callers with several
+ * independent problems: nprob symmetric matrices of orders n[p], stored one
+ * after another (each n[p]*n[p], column-major, lower triangle read).
+ * Returns the lowest nvec[p] eigenvalues of each (w, concatenated) and, if z
+ * is not NULL, their eigenvectors (n[p] x nvec[p] each, concatenated).
+ */
+int dvr_syev_batch(int32_t nprob, const int32_t* n, const double* a,
+                   const int32_t* nvec, double* w, double* z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DVR3D_AMD_H */

@@ -1,0 +1,101 @@
+"""Write the golden fixtures of the parity tests from reference runs.
+
+Runs the reference DVR3DRJZ executable built by oracle/Makefile
+(oracle/_ref/dvr3drjz_ref, compiled from the reference sources in this
+container) on the .job decks in oracle/jobs/ and stores, per config, the
+numbers the reference prints: the 2D block sizes (nham2 per r1 block), the
+selection outcome iv2, the quadrature points and the lowest eigenvalues in
+cm^-1 (D:2727-2730, format 5d24.12).  The output is plain JSON data.
+
+usage: python tests/golden/make_goldens.py c0 [c1 ...] [--rundir DIR]
+       python tests/golden/make_goldens.py --parse c1 OUTFILE   (existing run)
+"""
+from __future__ import annotations
+
+import json
+import os
+import re
+import shutil
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF = os.path.join(REPO, "oracle", "_ref", "dvr3drjz_ref")
+JOBS = os.path.join(REPO, "oracle", "jobs")
+PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par"}
+
+
+def _floats(line: str):
+    return [float(x.replace("D", "E")) for x in line.split()]
+
+
+def parse_output(text: str) -> dict:
+    out = {}
+    out["nham2"] = [int(m) for m in re.findall(r"nham2 =\s*(\d+)", text)]
+    out["iv2"] = [int(m) for m in re.findall(r"npntc =\s*\d+, no\. of eigenvectors =\s*(\d+)", text)]
+    m = re.search(r"(\d+) eigenvalues selected from", text)
+    out["nham3"] = int(m.group(1)) if m else None
+    i = text.index("eigenvalues in wavenumbers")
+    vals = []
+    for line in text[i:].split("\n")[1:]:
+        if "Band origins" in line or "secs" in line:
+            break
+        if line.strip():
+            vals += _floats(line)
+    out["eval_cm"] = vals
+    i = text.index("eigenvalues in hartrees")
+    vals = []
+    for line in text[i:].split("\n")[1:]:
+        if "wavenumbers" in line:
+            break
+        if line.strip():
+            vals += _floats(line)
+    out["eval_au"] = vals
+    # radial quadrature tables (points, weights, r)
+    tabs = re.findall(r"point Gauss-Laguerre integration\s*\n.*\n\s*\n((?:\s+\S+\s+\S+\s+\S+\n)+)", text)
+    out["r1"] = [_floats(l)[2] for l in tabs[0].strip().split("\n")] if tabs else []
+    out["r2"] = [_floats(l)[2] for l in tabs[1].strip().split("\n")] if len(tabs) > 1 else []
+    tab = re.search(r"Gauss-associated Legendre integration with k =\s*\d+\s*\n\s*\n.*\n\s*\n((?:\s+\S+\s+\S+\n)+)", text)
+    if tab:
+        rows = [_floats(l) for l in tab.group(1).strip().split("\n")]
+        out["xalf_half"] = [r[0] for r in rows]
+        out["walf_half"] = [r[1] for r in rows]
+    m = re.search(r"Computed sum of weights\s*(\S+)", text)
+    out["cswalf"] = float(m.group(1).replace("D", "E")) if m else None
+    m = re.search(r"(\d+) secs CPU time used", text)
+    out["ref_seconds"] = int(m.group(1)) if m else None
+    return out
+
+
+def run_config(cfg: str, rundir: str) -> dict:
+    os.makedirs(rundir, exist_ok=True)
+    shutil.copy(os.path.join(JOBS, cfg + ".job"), os.path.join(rundir, cfg + ".job"))
+    shutil.copy(os.path.join(JOBS, PES_OF[cfg]), os.path.join(rundir, "pes.par"))
+    t0 = time.time()
+    with open(os.path.join(rundir, cfg + ".job")) as fi, open(os.path.join(rundir, cfg + ".out"), "w") as fo:
+        subprocess.run([REF], stdin=fi, stdout=fo, cwd=rundir, check=True)
+    wall = time.time() - t0
+    res = parse_output(open(os.path.join(rundir, cfg + ".out")).read())
+    res["ref_wall_s"] = wall
+    return res
+
+
+def main(argv):
+    if argv and argv[0] == "--parse":
+        cfg, path = argv[1], argv[2]
+        res = parse_output(open(path).read())
+        res["source"] = "oracle/_ref/dvr3drjz_ref on oracle/jobs/%s.job" % cfg
+        json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"), indent=1)
+        return
+    rundir = os.path.join(REPO, "_work", "runs")
+    cfgs = [a for a in argv if not a.startswith("--")]
+    for cfg in cfgs:
+        res = run_config(cfg, os.path.join(rundir, cfg))
+        res["source"] = "oracle/_ref/dvr3drjz_ref on oracle/jobs/%s.job" % cfg
+        json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"), indent=1)
+        print(cfg, "levels", len(res["eval_cm"]), "wall", res["ref_wall_s"])
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

@@ -1,0 +1,65 @@
+"""Shared helpers of the parity tests (test infrastructure)."""
+import json
+import os
+
+import numpy as np
+
+from oracle import dvr3d_oracle as O
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+JOBS = os.path.join(REPO, "oracle", "jobs")
+GOLD = os.path.join(REPO, "tests", "golden")
+PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par"}
+
+
+def job_and_pes(cfg):
+    job = O.load_job(os.path.join(JOBS, cfg + ".job"))
+    pes = O.PES.from_par(open(os.path.join(JOBS, PES_OF[cfg])).read())
+    return job, pes
+
+
+def golden(cfg):
+    with open(os.path.join(GOLD, cfg + "_ref.json")) as f:
+        return json.load(f)
+
+
+def device_inputs(job, pes, nvec=0, solver=0, basis=None):
+    """Inputs of the C ABI computed from the oracle's basis arrays."""
+    b = basis or O.setup_basis(job)
+    s, c = b.sizes, b.consts
+    if s.zr2r1:
+        rc, rb = b.r1, b.r2
+        w1 = 0.5 / (rc * rc * c.ur1)
+        w2 = 0.5 / (rb * rb * c.ur2)
+        wsum = (w1[:, None] + w2[None, :]).ravel()
+        xkb, xkc = b.xk2, b.xk1
+    else:
+        rc, rb = b.r2, b.r1
+        w1 = 0.5 / (rb * rb * c.ur1)
+        w2 = 0.5 / (rc * rc * c.ur2)
+        wsum = (w1[None, :] + w2[:, None]).ravel()
+        xkb, xkc = b.xk1, b.xk2
+    dims = dict(npnta=s.npnta, npntb=s.npntb, npntc=s.npntc, max2d=s.max2d,
+                max3d=s.max3d, neval=s.neval, zcut=int(s.zcut), zall=int(s.zall),
+                zr2r1=int(s.zr2r1), idvr=s.idvr, npnt1=s.npnt1, npnt2=s.npnt2,
+                nvec=nvec, solver=solver)
+    na = s.npnta
+    xl = np.ascontiguousarray(b.xlmatr[:na, :na])
+    par = np.array([pes.de, pes.a, pes.re, pes.kb, pes.cte, pes.c])
+    return dict(dims=dims, xlmatr=xl, xkb=np.asfortranarray(xkb).ravel(order="F"),
+                xkc=np.asfortranarray(xkc).ravel(order="F"), wsum=wsum, rc=rc, rb=rb,
+                xa=b.xalf[:na], par=par, emax1=job["emax1"] / O.AUTOCM,
+                emax2=job["emax2"] / O.AUTOCM, basis=b)
+
+
+def run_device(inp, analytic=True, vgrid=None):
+    from dvr3d_amd import _lib
+    ctx = _lib.Context(inp["dims"])
+    ctx.set_basis(inp["xlmatr"], inp["xkb"], inp["xkc"], inp["wsum"], None,
+                  inp["emax1"], inp["emax2"])
+    if analytic:
+        ctx.eval_potential_ab2(inp["rc"], inp["rb"], inp["xa"], inp["par"])
+    else:
+        ctx.set_potential(vgrid)
+    ctx.run()
+    return ctx

@@ -1,0 +1,69 @@
+"""GPU parity of the batched dense eigensolver (dvr_syev_batch / dvr_diag).
+
+The replaced routine is dsyev('V','L') behind SUBROUTINE diag
+(source/dvr3drjz_segmented.f90:2638-2659).  LAPACK (numpy.linalg.eigh) is the
+check: eigenvalues to a few ulps of ||A||, eigenvectors by residual and
+orthogonality (their signs are arbitrary in both).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _sym(n, rng, spread=1.0):
+    a = rng.standard_normal((n, n)) * spread
+    return 0.5 * (a + a.T)
+
+
+def _check(a, w, z, nv, tol=1e-12):
+    ref = np.linalg.eigvalsh(a)[:nv]
+    nrm = max(np.abs(ref).max(), 1.0)
+    assert np.abs(w - ref).max() <= tol * nrm * max(1, a.shape[0]) ** 0.5
+    if z is not None and nv:
+        r = a @ z - z * w[None, :]
+        assert np.abs(r).max() <= 1e-11 * nrm * a.shape[0] ** 0.5
+        o = z.T @ z - np.eye(nv)
+        assert np.abs(o).max() <= 1e-10
+
+
+def test_dvr_diag_closed_form():
+    from dvr3d_amd import _lib
+    # tridiagonal (2,-1): eigenvalues 2 - 2 cos(k pi/(n+1))
+    n = 3
+    a = np.diag([2.0] * n) + np.diag([-1.0] * (n - 1), -1)
+    e, v = _lib.diag(a)
+    assert np.allclose(e, [2 - np.sqrt(2), 2, 2 + np.sqrt(2)], atol=1e-14)
+    n = 50
+    a = np.diag([2.0] * n) + np.diag([-1.0] * (n - 1), -1)
+    e, v = _lib.diag(a)
+    k = np.arange(1, n + 1)
+    assert np.abs(e - (2 - 2 * np.cos(k * np.pi / (n + 1)))).max() < 1e-13
+    full = a + np.tril(a, -1).T
+    assert np.abs(full @ v - v * e).max() < 1e-12
+
+
+@pytest.mark.parametrize("sizes", [[1], [2], [3, 5], [17, 64, 1, 130], [257, 31]])
+def test_syev_batch_random(sizes):
+    from dvr3d_amd import _lib
+    rng = np.random.default_rng(sum(sizes))
+    mats = [_sym(n, rng) for n in sizes]
+    ws, zs = _lib.syev_batch(mats)
+    for a, w, z in zip(mats, ws, zs):
+        _check(a, w, z, a.shape[0])
+
+
+def test_syev_batch_partial_and_clusters():
+    from dvr3d_amd import _lib
+    rng = np.random.default_rng(7)
+    n = 300
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.sort(rng.uniform(0.0, 1.0, n))
+    lam[10] = lam[9] + 1e-13          # near-degenerate pair
+    lam[20:23] = lam[19]              # exact triple degeneracy
+    lam = np.sort(lam)
+    a = (q * lam) @ q.T
+    a = 0.5 * (a + a.T)
+    ws, zs = _lib.syev_batch([a, a[:120, :120]], nvec=[40, 7])
+    _check(a, ws[0], zs[0], 40, tol=2e-12)
+    _check(a[:120, :120], ws[1], zs[1], 7, tol=2e-12)
