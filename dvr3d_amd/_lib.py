@@ -29,7 +29,7 @@ EXPORTS = (
     "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_get_sizes",
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
-    "dvr_diag", "dvr_syev_batch",
+    "dvr_diag", "dvr_syev_batch", "dvr_set_profiling", "dvr_get_profile",
 )
 
 
@@ -78,6 +78,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_get_stage_times.argtypes = [P, D]
     lib.dvr_diag.argtypes = [D, C.c_int32, C.c_int32, D]
     lib.dvr_syev_batch.argtypes = [C.c_int32, I, D, I, D, D]
+    lib.dvr_set_profiling.argtypes = [P, C.c_int32]
+    lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     for name in EXPORTS:
         if name not in ("dvr_ctx_destroy", "dvr_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -185,6 +187,26 @@ class Context:
         out = np.zeros(n3 * nvec)
         self._check(self.lib.dvr_get_vectors(self.h, _dp(out)))
         return out.reshape(nvec, n3).T
+
+    def set_profiling(self, on: bool):
+        self._check(self.lib.dvr_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_profile(self):
+        """Per-kernel stats of the last profiled run: name -> dict."""
+        out = {}
+        i = 0
+        buf = C.create_string_buffer(128)
+        while True:
+            n = C.c_int32()
+            sec, by, fl = C.c_double(), C.c_double(), C.c_double()
+            rc = self.lib.dvr_get_profile(self.h, i, buf, 128, C.byref(n), C.byref(sec),
+                                          C.byref(by), C.byref(fl))
+            if rc != DVR_OK:
+                break
+            out[buf.value.decode()] = dict(launches=n.value, seconds=sec.value,
+                                           bytes=by.value, flops=fl.value)
+            i += 1
+        return out
 
     def stage_times(self):
         out = np.zeros(DVR_NSTAGE)

@@ -19,6 +19,10 @@
 
 using namespace dvr;
 
+namespace dvr {
+thread_local Profiler* g_prof = nullptr;
+}
+
 namespace {
 
 struct DevBuf {
@@ -207,6 +211,14 @@ struct dvr_ctx {
   std::vector<double> h_eval;
   double stage_ms[DVR_NSTAGE] = {0};
   hipEvent_t ev[DVR_NSTAGE + 1] = {};
+  bool profiling = false;
+  std::unique_ptr<Profiler> prof;
+  struct KStat {
+    std::string name;
+    int launches;
+    double seconds, bytes, flops;
+  };
+  std::vector<KStat> kstats;
 };
 
 namespace {
@@ -237,6 +249,7 @@ void run_pipeline(dvr_ctx_t* C) {
   DVR_HIP(hipEventRecord(C->ev[0], s));
 
   // ---- 1D: mkham1 + diag + cut1d for every (c, b)
+  prof_phase("1d");
   C->e1full.ensure(sizeof(double) * (size_t)nprob * na);
   C->v1full.ensure(sizeof(double) * (size_t)nprob * na * na);
   C->cnt.ensure(sizeof(int) * nprob);
@@ -292,6 +305,7 @@ void run_pipeline(dvr_ctx_t* C) {
   const int* d_iv1off = upload(C->iv1off, iv1off, s);
 
   // ---- mkham2: grouped SYRK over the c blocks
+  prof_phase("2d");
   std::vector<int> pc;  // c of each nonempty block
   std::vector<int> pn;
   std::vector<long> h2off;
@@ -325,8 +339,13 @@ void run_pipeline(dvr_ctx_t* C) {
       x.ntile = nt * (nt + 1) / 2;
       tiles += x.ntile;
     }
+    double by = 0.0, fl = 0.0;
+    for (int q = 0; q < np2; ++q) {
+      by += 8.0 * ((double)na * pn[q] + (double)pn[q] * pn[q]);
+      fl += (double)na * pn[q] * (pn[q] + 1.0);
+    }
     launch_syrk_scaled(s, upload(C->sdesc2, desc, s), np2, tiles,
-                       C->xkb.as<double>(), nb);
+                       C->xkb.as<double>(), nb, by, fl);
   }
   DVR_HIP(hipEventRecord(C->ev[2], s));
 
@@ -393,6 +412,7 @@ void run_pipeline(dvr_ctx_t* C) {
   DVR_HIP(hipEventRecord(C->ev[4], s));
 
   // ---- 3D: cint expansion + mkham3 SYRK + loadh diagonal
+  prof_phase("3d");
   std::vector<CBlock> cb(nc);
   std::vector<int> blk3(n3);
   std::vector<double> e2sel(n3);
@@ -437,7 +457,9 @@ void run_pipeline(dvr_ctx_t* C) {
     desc[0].tile0 = 0;
     desc[0].ntile = nt * (nt + 1) / 2;
     launch_syrk_scaled(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile,
-                       C->xkc.as<double>(), nc);
+                       C->xkc.as<double>(), nc,
+                       8.0 * ((double)G * n3 + (double)n3 * n3),
+                       (double)G * n3 * (n3 + 1.0));
   }
   DVR_HIP(hipEventRecord(C->ev[5], s));
 
@@ -596,7 +618,34 @@ int dvr_run(dvr_ctx_t* c) {
   c->ran = false;
   try {
     DeviceGuard g(c->device);
+    if (c->profiling) {
+      if (!c->prof) c->prof.reset(new Profiler());
+      c->prof->reset();
+      c->prof->phase.clear();
+      g_prof = c->prof.get();
+    }
+    struct ProfOff {
+      ~ProfOff() { g_prof = nullptr; }
+    } prof_off;
     run_pipeline(c);
+    if (c->profiling) {
+      DVR_HIP(hipStreamSynchronize(c->stream));
+      c->kstats.clear();
+      for (const ProfRec& r : c->prof->recs) {
+        float ms = 0.f;
+        DVR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        auto it = std::find_if(c->kstats.begin(), c->kstats.end(),
+                               [&](const dvr_ctx::KStat& k) { return k.name == r.name; });
+        if (it == c->kstats.end()) {
+          c->kstats.push_back(dvr_ctx::KStat{r.name, 0, 0.0, 0.0, 0.0});
+          it = c->kstats.end() - 1;
+        }
+        it->launches += 1;
+        it->seconds += 1e-3 * ms;
+        it->bytes += r.bytes;
+        it->flops += r.flops;
+      }
+    }
   } catch (const HipError& e) {
     return fail(c, DVR_ERR_HIP, e.what());
   } catch (const std::runtime_error& e) {
@@ -666,6 +715,27 @@ int dvr_get_vectors(const dvr_ctx_t* c, double* vec) {
   const size_t n = (size_t)c->nham3 * c->nvec_done;
   if (hipMemcpy(vec, c->Z3.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
     return DVR_ERR_HIP;
+  return DVR_OK;
+}
+
+int dvr_set_profiling(dvr_ctx_t* c, int32_t on) {
+  if (!c) return DVR_ERR_ARG;
+  c->profiling = on != 0;
+  return DVR_OK;
+}
+
+int dvr_get_profile(const dvr_ctx_t* c, int32_t idx, char* name, int32_t name_len,
+                    int32_t* launches, double* seconds, double* bytes, double* flops) {
+  if (!c || idx < 0 || idx >= (int)c->kstats.size()) return DVR_ERR_ARG;
+  const auto& k = c->kstats[idx];
+  if (name && name_len > 0) {
+    std::strncpy(name, k.name.c_str(), name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (launches) *launches = k.launches;
+  if (seconds) *seconds = k.seconds;
+  if (bytes) *bytes = k.bytes;
+  if (flops) *flops = k.flops;
   return DVR_OK;
 }
 

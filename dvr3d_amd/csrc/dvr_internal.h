@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <string>
 #include <stdexcept>
+#include <vector>
 
 namespace dvr {
 
@@ -23,6 +24,57 @@ struct HipError : std::runtime_error {
   } while (0)
 
 #define DVR_LAUNCH_CHECK() DVR_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------------------
+// Optional per-launch timing: while a Profiler is installed (one profiled
+// dvr_run), every launch wrapper brackets its kernel with two HIP events on
+// the launch stream and records the algorithmic bytes and flops of the launch.
+// ---------------------------------------------------------------------------
+struct ProfRec {
+  std::string name;
+  hipEvent_t a, b;
+  double bytes, flops;
+};
+struct Profiler {
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<ProfRec> recs;
+  std::string phase;
+  hipEvent_t event() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      DVR_HIP(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  void reset() {
+    used = 0;
+    recs.clear();
+  }
+  ~Profiler() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+extern thread_local Profiler* g_prof;
+inline void prof_begin(hipStream_t s, const char* kernel, double bytes, double flops) {
+  if (!g_prof) return;
+  ProfRec r;
+  r.name = g_prof->phase.empty() ? std::string(kernel) : g_prof->phase + ":" + kernel;
+  r.a = g_prof->event();
+  r.b = g_prof->event();
+  r.bytes = bytes;
+  r.flops = flops;
+  DVR_HIP(hipEventRecord(r.a, s));
+  g_prof->recs.push_back(r);
+}
+inline void prof_end(hipStream_t s) {
+  if (!g_prof) return;
+  DVR_HIP(hipEventRecord(g_prof->recs.back().b, s));
+}
+inline void prof_phase(const char* ph) {
+  if (g_prof) g_prof->phase = ph;
+}
 
 // ---------------------------------------------------------------------------
 // grid-side kernels (k_grid.hip)
@@ -96,8 +148,10 @@ struct SyrkDesc {
   int tile0;  // first tile index of this group in the flat launch
   int ntile;  // number of lower-triangular tiles
 };
+// bytes/flops: algorithmic traffic and work of the launch (profiling only).
 void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
-                        int total_tiles, const double* xk, int ldxk);
+                        int total_tiles, const double* xk, int ldxk,
+                        double bytes, double flops);
 int syrk_tile();
 
 // ---------------------------------------------------------------------------

@@ -238,9 +238,11 @@ void launch_jacobi1d(hipStream_t s, const double* vgrid, const double* xl,
   DVR_HIP(hipFuncSetAttribute((const void*)k_jacobi1d,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
+  prof_begin(s, "jacobi1d", 8.0 * nprob * ((double)n * n + 3.0 * n), 0.0);
   hipLaunchKernelGGL(k_jacobi1d, dim3(nprob), dim3(kThreads), lds, s, vgrid, xl,
                      wsum, wterm, n, emaxau, e1, v1, cnt, vscratch, vlds ? 1 : 0,
                      src_prob, sweeps);
+  prof_end(s);
   DVR_LAUNCH_CHECK();
 }
 

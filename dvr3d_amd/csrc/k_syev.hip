@@ -168,11 +168,26 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
   if (nmax <= 0) return;
   const int jmax = nmax >= 2 ? nmax - 2 : 0;
   for (int j = 0; j <= jmax; ++j) {
-    hipLaunchKernelGGL(k_tri_col, dim3(b.nprob), dim3(TB), 0, s, b, j);
+    double cbytes = 0.0, ubytes = 0.0, uflops = 0.0;
     int grid = 0;
-    for (int p = 0; p < b.nprob; ++p)
-      if (j <= h_n[p] - 3) grid += (h_n[p] - j - 1 + CB - 1) / CB;
-    if (grid > 0) hipLaunchKernelGGL(k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
+    for (int p = 0; p < b.nprob; ++p) {
+      const double m = (double)(h_n[p] - j);
+      if (m > 0) cbytes += 8.0 * 4.0 * m;
+      if (j <= h_n[p] - 3) {
+        grid += (h_n[p] - j - 1 + CB - 1) / CB;
+        const double t = m - 1.0;
+        ubytes += (j > 0 ? 16.0 : 8.0) * t * t + 8.0 * 4.0 * t;
+        uflops += (j > 0 ? 6.0 : 2.0) * t * t;
+      }
+    }
+    prof_begin(s, "tri_col", cbytes, 0.0);
+    hipLaunchKernelGGL(k_tri_col, dim3(b.nprob), dim3(TB), 0, s, b, j);
+    prof_end(s);
+    if (grid > 0) {
+      prof_begin(s, "tri_upd", ubytes, uflops);
+      hipLaunchKernelGGL(k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
+      prof_end(s);
+    }
   }
   DVR_LAUNCH_CHECK();
 }
@@ -296,8 +311,10 @@ void launch_bisect(hipStream_t s, const EigBatch& b, const int* il,
                    const long* eoff) {
   if (total <= 0) return;
   const int grid = (total + TB / 64 - 1) / (TB / 64);
+  prof_begin(s, "bisect", 0.0, 0.0);
   hipLaunchKernelGGL(k_bisect, dim3(grid), dim3(TB), 0, s, b, il, kofs, total,
                      eval, eoff);
+  prof_end(s);
   DVR_LAUNCH_CHECK();
 }
 
@@ -427,8 +444,10 @@ void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
                   const long* zoff, double* scratch, int nthreads) {
   if (ncl <= 0) return;
   const int grid = (ncl + TB - 1) / TB;
+  prof_begin(s, "stein", 0.0, 0.0);
   hipLaunchKernelGGL(k_stein, dim3(grid), dim3(TB), 0, s, b, cl, ncl, eval,
                      eoff, Z, zoff, scratch, nthreads);
+  prof_end(s);
   DVR_LAUNCH_CHECK();
 }
 
@@ -597,11 +616,23 @@ void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
   int* d = nullptr;
   DVR_HIP(hipMallocAsync((void**)&d, sizeof(int) * 2 * (np + 1), s));
   DVR_HIP(hipMemcpyAsync(d, h, sizeof(int) * 2 * (np + 1), hipMemcpyHostToDevice, s));
-  if (nbt > 0)
+  double obytes = 0.0, oflops = 0.0;
+  for (int p = 0; p < np; ++p) {
+    const double n = h_n[p], mm = h_m[p];
+    obytes += 8.0 * (n * n * 0.5 + 2.0 * n * mm * ((n - 2.0) / NB > 1.0 ? (n - 2.0) / NB : 1.0));
+    oflops += 4.0 * n * n * mm;
+  }
+  if (nbt > 0) {
+    prof_begin(s, "larft", 0.0, 0.0);
     hipLaunchKernelGGL(k_larft, dim3(nbt), dim3(TB), 0, s, b, d, tbuf, toff);
-  if (ntt > 0)
+    prof_end(s);
+  }
+  if (ntt > 0) {
+    prof_begin(s, "ormtr", obytes, oflops);
     hipLaunchKernelGGL(k_ormtr, dim3(ntt), dim3(TB), 0, s, b, Z, zoff, m,
                        d + np + 1, tbuf, toff);
+    prof_end(s);
+  }
   DVR_LAUNCH_CHECK();
   DVR_HIP(hipFreeAsync(d, s));
   DVR_HIP(hipStreamSynchronize(s));

@@ -12,6 +12,9 @@
 // Only lower-triangular 64x64 tiles are computed; each writes its block and
 // the mirrored block.  Work-group = 4 wavefronts, each a 32x32 sub-tile of
 // 2x2 v_mfma_f64_16x16x4f64 accumulators, operands staged through LDS.
+// Lane layout of v_mfma_f64_16x16x4f64 (checked on gfx950): A(i,k) comes from
+// lane i + 16k, B(k,j) from lane j + 16k, and result register r of lane l is
+// D(4r + l/16, l%16).
 #include "dvr_internal.h"
 
 namespace dvr {
@@ -86,7 +89,7 @@ k_syrk_scaled(const SyrkDesc* __restrict__ desc, int ngroup,
   for (int bi = 0; bi < 2; ++bi)
     for (int bj = 0; bj < 2; ++bj)
       for (int r = 0; r < 4; ++r) {
-        const int i = row0 + wy * 32 + bi * 16 + 4 * lk + r;
+        const int i = row0 + wy * 32 + bi * 16 + 4 * r + lk;
         const int j = col0 + wx * 32 + bj * 16 + l16;
         if (i >= N || j >= N) continue;
         double v = __dmul_rn(acc[bi][bj][r], xk[blk[i] + (long)ldxk * blk[j]]);
@@ -97,10 +100,13 @@ k_syrk_scaled(const SyrkDesc* __restrict__ desc, int ngroup,
 }
 
 void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
-                        int total_tiles, const double* xk, int ldxk) {
+                        int total_tiles, const double* xk, int ldxk,
+                        double bytes, double flops) {
   if (total_tiles <= 0) return;
+  prof_begin(s, "syrk", bytes, flops);
   hipLaunchKernelGGL(k_syrk_scaled, dim3(total_tiles), dim3(256), 0, s, d_desc,
                      ngroup, xk, ldxk);
+  prof_end(s);
   DVR_LAUNCH_CHECK();
 }
 

@@ -130,6 +130,18 @@ int dvr_get_vectors(const dvr_ctx_t* ctx, double* vec);
 int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
 
 /*
+ * Kernel-level timing (no counterpart in the reference, which only prints
+ * the total run time, D:67/D:75-78).  With profiling on, the next dvr_run
+ * brackets every kernel launch with HIP events on the context's stream and
+ * accumulates, per "<stage>:<kernel>" name, the launch count, device
+ * seconds and the launches' algorithmic bytes and flops.  Entry idx =
+ * 0, 1, ... returns DVR_ERR_ARG past the last kernel name.
+ */
+int dvr_set_profiling(dvr_ctx_t* ctx, int32_t on);
+int dvr_get_profile(const dvr_ctx_t* ctx, int32_t idx, char* name, int32_t name_len,
+                    int32_t* launches, double* seconds, double* bytes, double* flops);
+
+/*
  * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)
  * (D:2638-2659: dsyev('V','L',nham,ham,maxham,eig,...)).  On entry the lower
  * triangle of the column-major ham(maxham,nham) holds the symmetric matrix; on
