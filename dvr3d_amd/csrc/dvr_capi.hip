@@ -198,7 +198,8 @@ struct dvr_ctx {
   bool basis_set = false, pot_set = false, ran = false;
   double emax1 = 0.0, emax2 = 0.0;
   DevBuf xl, xkb, xkc, wsum, wterm, rc, rb, xa, v;
-  DevBuf e1full, v1full, cnt, vscr, sweeps;
+  DevBuf e1full, v1full, cnt, vscr, sweeps, mflag, uniq;
+  bool mirror_used = false;
   DevBuf srcrow, rowb, v1, e1p, iv1off;
   DevBuf H2, sdesc2, Z2;
   DevBuf cint, blk3, e2sel, cblk, sdesc3, H3, Z3, psi;
@@ -254,14 +255,36 @@ void run_pipeline(dvr_ctx_t* C) {
   C->v1full.ensure(sizeof(double) * (size_t)nprob * na * na);
   C->cnt.ensure(sizeof(int) * nprob);
   C->sweeps.ensure(sizeof(int) * nprob);
-  const size_t vs = jacobi1d_scratch_doubles(na, nprob);
+  // problems (c,b) and (b,c) with identical inputs are solved once
+  bool mirror = false;
+  if (nb == nc && nb > 1) {
+    C->mflag.ensure(sizeof(int));
+    launch_mirror_check(s, C->v.as<double>(), C->wsum.as<double>(),
+                        C->has_wterm ? C->wterm.as<double>() : nullptr, nb, na,
+                        C->mflag.as<int>());
+    int flag = 1;
+    DVR_HIP(hipMemcpyAsync(&flag, C->mflag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipStreamSynchronize(s));
+    mirror = flag == 0;
+  }
+  C->mirror_used = mirror;
+  std::vector<int> uniq;
+  if (mirror) {
+    for (int c = 0; c < nc; ++c)
+      for (int b = 0; b <= c; ++b) uniq.push_back(c * nb + b);
+  }
+  const int nsolve = mirror ? (int)uniq.size() : nprob;
+  const size_t vs = jacobi1d_scratch_doubles(na, nsolve);
   if (vs) C->vscr.ensure(sizeof(double) * vs);
   launch_jacobi1d(s, C->v.as<double>(), C->xl.as<double>(),
                   C->wsum.as<double>(),
-                  C->has_wterm ? C->wterm.as<double>() : nullptr, na, nprob,
+                  C->has_wterm ? C->wterm.as<double>() : nullptr, na, nsolve,
                   emax1, C->e1full.as<double>(), C->v1full.as<double>(),
                   C->cnt.as<int>(), vs ? C->vscr.as<double>() : nullptr,
-                  nullptr, C->sweeps.as<int>());
+                  mirror ? upload(C->uniq, uniq, s) : nullptr, C->sweeps.as<int>());
+  if (mirror)
+    launch_mirror_copy(s, nb, na, C->e1full.as<double>(), C->v1full.as<double>(),
+                       C->cnt.as<int>());
   std::vector<int> cnt(nprob);
   DVR_HIP(hipMemcpyAsync(cnt.data(), C->cnt.p, sizeof(int) * nprob,
                          hipMemcpyDeviceToHost, s));

@@ -129,6 +129,13 @@ void launch_jacobi1d(hipStream_t s, const double* vgrid, const double* xl,
                      double emaxau, double* e1, double* v1, int* cnt,
                      double* vscratch, const int* src_prob, int* sweeps);
 size_t jacobi1d_scratch_doubles(int n, int nprob);
+// Mirror symmetry of the 1D problems (square (c,b) grid): flag = 0 when
+// every problem (c,b) has the same inputs as (b,c); then only c >= b need
+// solving and k_mirror_copy fills the rest.
+void launch_mirror_check(hipStream_t s, const double* v, const double* wsum,
+                         const double* wterm, int nb, int na, int* flag);
+void launch_mirror_copy(hipStream_t s, int nb, int n, double* e1, double* v1,
+                        int* cnt);
 
 // ---------------------------------------------------------------------------
 // symmetric rank-K products on MFMA (k_syrk.hip)

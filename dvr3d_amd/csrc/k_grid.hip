@@ -23,11 +23,12 @@ k_pes_ab2(double* __restrict__ v, const double* __restrict__ rc,
     const double x = xa[a];
     const double t1 = 1.0 - exp(-aa * (r1 - re));
     const double t2 = 1.0 - exp(-aa * (r2 - re));
-    const double m1 = t1 * t1;
-    const double m2 = t2 * t2;
+    // explicitly rounded products and sum (no fused multiply-add), so that
+    // m1 + m2 -- and hence V -- is symmetric in r1 <-> r2 bit for bit, as
+    // in the reference's unfused evaluation of the same formula
+    const double msum = __dadd_rn(__dmul_rn(t1, t1), __dmul_rn(t2, t2));
     const double dx = x - cte;
-    // (m1+m2) and (r1-re)*(r2-re) are symmetric in r1<->r2 bit for bit.
-    v[idx] = de * (m1 + m2) + kb * (dx * dx) + cc * ((r1 - re) * (r2 - re));
+    v[idx] = de * msum + kb * (dx * dx) + cc * ((r1 - re) * (r2 - re));
   }
 }
 

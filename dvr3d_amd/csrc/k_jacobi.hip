@@ -18,18 +18,23 @@
 namespace dvr {
 
 namespace {
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kMaxSweeps = 40;
 constexpr double kPad = 1.0e300;  // diagonal of the padding row (odd n)
 
+// Round-robin ("chess tournament") pairing of np players in step s < np-1:
+// player np-1 meets s, and (s+k, s-k) mod (np-1) meet for k = 1..np/2-1.
 __device__ inline void pair_of(int s, int k, int np, int& p, int& q) {
+  const int m1 = np - 1;
   int pa, pb;
   if (k == 0) {
-    pa = np - 1;
+    pa = m1;
     pb = s;
   } else {
-    pa = (s + k) % (np - 1);
-    pb = (s - k + (np - 1)) % (np - 1);
+    pa = s + k;
+    if (pa >= m1) pa -= m1;
+    pb = s - k;
+    if (pb < 0) pb += m1;
   }
   p = pa < pb ? pa : pb;
   q = pa < pb ? pb : pa;
@@ -60,6 +65,12 @@ k_jacobi1d(const double* __restrict__ vgrid, const double* __restrict__ xl,
   int* Mode = Qq + m;   // 0 none, 1 rotate, 2 zero a negligible apq
   int* flag = Mode + m;
   int* rank = flag + 1;
+  unsigned* tri = reinterpret_cast<unsigned*>(rank + n);  // (u<<16)|v, u <= v
+  const int ntri = m * (m + 1) / 2;
+  for (int u = tid; u < m; u += kThreads) {
+    const int base = u * m - u * (u - 1) / 2;  // first slot of row u
+    for (int v = u; v < m; ++v) tri[base + (v - u)] = ((unsigned)u << 16) | (unsigned)v;
+  }
 
   // ---- H1 in LDS (mkham1 with ztheta=T, D:2180-2193)
   const double ws = wsum[p];
@@ -120,10 +131,10 @@ k_jacobi1d(const double* __restrict__ vgrid, const double* __restrict__ xl,
       }
       __syncthreads();
       // A <- J^T A J, one 2x2 block per work item (u <= v)
-      for (int w = tid; w < m * m; w += kThreads) {
-        const int u = w / m;
-        const int v = w - u * m;
-        if (u > v) continue;
+      for (int w = tid; w < ntri; w += kThreads) {
+        const unsigned uv = tri[w];
+        const int u = (int)(uv >> 16);
+        const int v = (int)(uv & 0xffffu);
         const int p1 = Pp[u], q1 = Qq[u];
         if (u == v) {
           const int mode = Mode[u];
@@ -159,16 +170,17 @@ k_jacobi1d(const double* __restrict__ vgrid, const double* __restrict__ xl,
         A[q2 * lda + q1] = W;
       }
       // V <- V J  (rows of V^T)
-      for (int w = tid; w < m * np; w += kThreads) {
-        const int v = w / np;
-        const int i = w - v * np;
+      const int lane = tid & 63, wv = tid >> 6;
+      for (int v = wv; v < m; v += kThreads / 64) {
         if (Mode[v] != 1) continue;
+        for (int i = lane; i < np; i += 64) {
         const double c = Cs[v], sn = Sn[v];
         double* vp = Vt + (long)Pp[v] * np + i;
         double* vq = Vt + (long)Qq[v] * np + i;
         const double a = *vp, b = *vq;
         *vp = c * a - sn * b;
         *vq = sn * a + c * b;
+        }
       }
       __syncthreads();
     }
@@ -214,10 +226,78 @@ static size_t jacobi_lds_bytes(int n, bool vlds) {
   size_t b = (size_t)np * (np + 1) * 8;
   if (vlds) b += (size_t)np * np * 8;
   b += (size_t)m * 3 * 8 + (size_t)m * 3 * 4 + 4 + (size_t)n * 4;
+  b += (size_t)m * (m + 1) / 2 * 4;
   return b;
 }
 
 static bool jacobi_vlds(int n) { return jacobi_lds_bytes(n, true) <= 160 * 1024; }
+
+// Mirror test of the 1D inputs: problem (c,b) equals problem (b,c) bit for
+// bit when V(c,b,:) == V(b,c,:), wsum and wterm are symmetric (square grid).
+__global__ void __launch_bounds__(256)
+k_mirror_check(const double* __restrict__ v, const double* __restrict__ wsum,
+               const double* __restrict__ wterm, int nb, int na,
+               int* __restrict__ flag) {
+  const long total = (long)nb * nb * na;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int a = (int)(idx % na);
+    const long cb = idx / na;
+    const int b = (int)(cb % nb);
+    const int c = (int)(cb / nb);
+    if (c <= b) continue;
+    bool diff = v[idx] != v[((long)b * nb + c) * na + a];
+    if (a == 0) {
+      diff |= wsum[c * nb + b] != wsum[b * nb + c];
+      if (wterm) diff |= wterm[c * nb + b] != wterm[b * nb + c];
+    }
+    if (diff) *flag = 1;
+  }
+}
+
+// Copy the solution of problem (b,c), b < c, to its mirror (c,b).
+__global__ void __launch_bounds__(256)
+k_mirror_copy(int nb, int n, double* __restrict__ e1, double* __restrict__ v1,
+              int* __restrict__ cnt) {
+  const long per = (long)n * n + n + 1;
+  const long total = (long)nb * nb * per;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long pp = idx / per;
+    const long r = idx % per;
+    const int c = (int)(pp / nb), b = (int)(pp % nb);
+    if (c >= b) continue;          // destination (c,b) with c < b
+    const long dst = (long)c * nb + b, src = (long)b * nb + c;
+    if (r < (long)n * n)
+      v1[dst * n * n + r] = v1[src * n * n + r];
+    else if (r < (long)n * n + n)
+      e1[dst * n + (r - (long)n * n)] = e1[src * n + (r - (long)n * n)];
+    else
+      cnt[dst] = cnt[src];
+  }
+}
+
+void launch_mirror_check(hipStream_t s, const double* v, const double* wsum,
+                         const double* wterm, int nb, int na, int* flag) {
+  const long total = (long)nb * nb * na;
+  int grid = (int)((total + 255) / 256);
+  if (grid > 65535 * 8) grid = 65535 * 8;
+  DVR_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_mirror_check, dim3(grid), dim3(256), 0, s, v, wsum, wterm,
+                     nb, na, flag);
+  DVR_LAUNCH_CHECK();
+}
+
+void launch_mirror_copy(hipStream_t s, int nb, int n, double* e1, double* v1,
+                        int* cnt) {
+  const long total = (long)nb * nb * ((long)n * n + n + 1);
+  int grid = (int)((total + 255) / 256);
+  if (grid > 65535 * 16) grid = 65535 * 16;
+  prof_begin(s, "mirror_copy", 16.0 * (double)nb * (nb - 1) / 2 * ((double)n * n + n), 0.0);
+  hipLaunchKernelGGL(k_mirror_copy, dim3(grid), dim3(256), 0, s, nb, n, e1, v1, cnt);
+  prof_end(s);
+  DVR_LAUNCH_CHECK();
+}
 
 size_t jacobi1d_scratch_doubles(int n, int nprob) {
   if (jacobi_vlds(n)) return 0;
@@ -238,6 +318,7 @@ void launch_jacobi1d(hipStream_t s, const double* vgrid, const double* xl,
   DVR_HIP(hipFuncSetAttribute((const void*)k_jacobi1d,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
+  (void)0;
   prof_begin(s, "jacobi1d", 8.0 * nprob * ((double)n * n + 3.0 * n), 0.0);
   hipLaunchKernelGGL(k_jacobi1d, dim3(nprob), dim3(kThreads), lds, s, vgrid, xl,
                      wsum, wterm, n, emaxau, e1, v1, cnt, vscratch, vlds ? 1 : 0,

@@ -26,7 +26,6 @@ namespace dvr {
 
 namespace {
 constexpr int TB = 256;
-constexpr int CB = 16;   // trailing columns per work-group in k_tri_upd
 constexpr int NB = 32;   // reflectors per WY block
 constexpr int CT = 8;    // columns per back-transform tile
 constexpr int RC = 64;   // rows per staged chunk in the back-transform
@@ -54,8 +53,24 @@ int backtransform_nb() { return NB; }
 // ---------------------------------------------------------------------------
 // 1. tridiagonalisation
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(TB) k_tri_col(EigBatch b, int j) {
-  __shared__ double red[TB / 64];
+// k_tri_col uses a larger work-group: it is latency bound (three dependent
+// reductions over one column per step).
+constexpr int TC = 1024;
+
+__device__ inline double block_sum_n(double v, double* red, int nthreads) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nthreads / 64; ++i) s += red[i];
+  return s;
+}
+
+// pbuf holds s = A v (unscaled); p = tau s.
+__global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
+  __shared__ double red[TC / 64];
   const int p = blockIdx.x;
   const int tid = threadIdx.x;
   const int n = b.n[p];
@@ -69,19 +84,24 @@ __global__ void __launch_bounds__(TB) k_tri_col(EigBatch b, int j) {
   double* vprev = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
   double* vcur = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
   double* w = b.wbuf + vo;
-  const double* pv = b.pbuf + vo;
+  double* sv = b.pbuf + vo;
   double* col = A + (long)j * n;
   if (j > 0) {
     const double taup = b.tau[vo + j - 1];
     double part = 0.0;
-    for (int i = j + tid; i < n; i += TB) part += pv[i] * vprev[i];
-    const double K = block_sum(part, red);
+    for (int i = j + tid; i < n; i += TC) part += sv[i] * vprev[i];
+    const double K = taup * block_sum_n(part, red, TC);
     const double alpha = -0.5 * taup * K;
-    for (int i = j + tid; i < n; i += TB) w[i] = pv[i] + alpha * vprev[i];
+    for (int i = j + tid; i < n; i += TC) w[i] = taup * sv[i] + alpha * vprev[i];
     __syncthreads();
     const double wj = w[j], vj = vprev[j];
-    for (int i = j + tid; i < n; i += TB) col[i] -= (vprev[i] * wj + w[i] * vj);
+    for (int i = j + tid; i < n; i += TC) {
+      col[i] -= (vprev[i] * wj + w[i] * vj);
+      sv[i] = 0.0;  // accumulator of the next product (read above, now free)
+    }
     __syncthreads();
+  } else {
+    for (int i = tid; i < n; i += TC) sv[i] = 0.0;
   }
   if (j == n - 2) {
     if (tid == 0) {
@@ -96,8 +116,8 @@ __global__ void __launch_bounds__(TB) k_tri_col(EigBatch b, int j) {
   }
   const double alpha = col[j + 1];
   double part = 0.0;
-  for (int i = j + 2 + tid; i < n; i += TB) part += col[i] * col[i];
-  const double sigma = block_sum(part, red);
+  for (int i = j + 2 + tid; i < n; i += TC) part += col[i] * col[i];
+  const double sigma = block_sum_n(part, red, TC);
   double tau, beta, scal;
   if (sigma == 0.0) {
     tau = 0.0;
@@ -109,7 +129,7 @@ __global__ void __launch_bounds__(TB) k_tri_col(EigBatch b, int j) {
     tau = (beta - alpha) / beta;
     scal = 1.0 / (alpha - beta);
   }
-  for (int i = j + 1 + tid; i < n; i += TB) {
+  for (int i = j + 1 + tid; i < n; i += TC) {
     const double vi = (i == j + 1) ? 1.0 : col[i] * scal;
     vcur[i] = vi;
     if (i > j + 1) col[i] = vi;
@@ -123,14 +143,23 @@ __global__ void __launch_bounds__(TB) k_tri_col(EigBatch b, int j) {
   }
 }
 
+// Trailing update + matrix-vector product over the full trailing square
+// A(j+1:n, j+1:n), both triangles kept current so every column is contiguous:
+//   A(i,k) -= v(i) w(k) + w(i) v(k)      (pending update of step j-1)
+//   s(k)    = sum_i A(i,k) v_j(i)
+// One wavefront per column, four columns per work-group; each lane keeps
+// four independent row loads in flight.  s(k) is complete within the wave,
+// so no atomics are needed.
+constexpr int UC = 4;
+
 __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   const int bid = blockIdx.x;
   int p = 0, base = 0;
   for (; p < b.nprob; ++p) {
     const int n = b.n[p];
-    const int nb = (j <= n - 3) ? (n - j - 1 + CB - 1) / CB : 0;
-    if (bid < base + nb) break;
-    base += nb;
+    const int cnt = (j <= n - 3) ? (n - j - 1 + UC - 1) / UC : 0;
+    if (bid < base + cnt) break;
+    base += cnt;
   }
   if (p >= b.nprob) return;
   const int n = b.n[p];
@@ -139,27 +168,48 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   const double* vprev = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
   const double* vcur = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
   const double* w = b.wbuf + vo;
-  double* pv = b.pbuf + vo;
-  const double tau = b.tau[vo + j];
-  const int k0 = j + 1 + (bid - base) * CB;
-  const int k1 = min(k0 + CB, n);
+  double* sv = b.pbuf + vo;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = k0 + wave; k < k1; k += TB / 64) {
-    double* colk = A + (long)k * n;
-    double acc = 0.0;
-    if (j > 0) {
-      const double wk = w[k], vk = vprev[k];
-      for (int i = j + 1 + lane; i < n; i += 64) {
-        const double a = colk[i] - (vprev[i] * wk + w[i] * vk);
-        colk[i] = a;
-        acc += a * vcur[i];
-      }
-    } else {
-      for (int i = j + 1 + lane; i < n; i += 64) acc += colk[i] * vcur[i];
+  const int k = j + 1 + (bid - base) * UC + wave;
+  if (k >= n) return;
+  double* colk = A + (long)k * n;
+  const int i0 = j + 1;
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+  if (j > 0) {
+    const double wk = w[k], vk = vprev[k];
+    int i = i0 + lane;
+    for (; i + 192 < n; i += 256) {
+      const double a0 = colk[i], a1 = colk[i + 64], a2 = colk[i + 128], a3 = colk[i + 192];
+      const double u0 = a0 - (vprev[i] * wk + w[i] * vk);
+      const double u1 = a1 - (vprev[i + 64] * wk + w[i + 64] * vk);
+      const double u2 = a2 - (vprev[i + 128] * wk + w[i + 128] * vk);
+      const double u3 = a3 - (vprev[i + 192] * wk + w[i + 192] * vk);
+      colk[i] = u0;
+      colk[i + 64] = u1;
+      colk[i + 128] = u2;
+      colk[i + 192] = u3;
+      acc0 += u0 * vcur[i];
+      acc1 += u1 * vcur[i + 64];
+      acc2 += u2 * vcur[i + 128];
+      acc3 += u3 * vcur[i + 192];
     }
-    acc = wave_sum(acc);
-    if (lane == 0) pv[k] = tau * acc;
+    for (; i < n; i += 64) {
+      const double u = colk[i] - (vprev[i] * wk + w[i] * vk);
+      colk[i] = u;
+      acc0 += u * vcur[i];
+    }
+  } else {
+    int i = i0 + lane;
+    for (; i + 192 < n; i += 256) {
+      acc0 += colk[i] * vcur[i];
+      acc1 += colk[i + 64] * vcur[i + 64];
+      acc2 += colk[i + 128] * vcur[i + 128];
+      acc3 += colk[i + 192] * vcur[i + 192];
+    }
+    for (; i < n; i += 64) acc0 += colk[i] * vcur[i];
   }
+  const double acc = wave_sum((acc0 + acc1) + (acc2 + acc3));
+  if (lane == 0) sv[k] = acc;
 }
 
 void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
@@ -172,16 +222,17 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
     int grid = 0;
     for (int p = 0; p < b.nprob; ++p) {
       const double m = (double)(h_n[p] - j);
-      if (m > 0) cbytes += 8.0 * 4.0 * m;
+      if (m > 0) cbytes += 8.0 * 6.0 * m;
       if (j <= h_n[p] - 3) {
-        grid += (h_n[p] - j - 1 + CB - 1) / CB;
+        grid += (h_n[p] - j - 1 + UC - 1) / UC;
         const double t = m - 1.0;
+        // trailing square read (+ written back when an update is pending)
         ubytes += (j > 0 ? 16.0 : 8.0) * t * t + 8.0 * 4.0 * t;
         uflops += (j > 0 ? 6.0 : 2.0) * t * t;
       }
     }
     prof_begin(s, "tri_col", cbytes, 0.0);
-    hipLaunchKernelGGL(k_tri_col, dim3(b.nprob), dim3(TB), 0, s, b, j);
+    hipLaunchKernelGGL(k_tri_col, dim3(b.nprob), dim3(TC), 0, s, b, j);
     prof_end(s);
     if (grid > 0) {
       prof_begin(s, "tri_upd", ubytes, uflops);

@@ -7,8 +7,12 @@ numbers the reference prints: the 2D block sizes (nham2 per r1 block), the
 selection outcome iv2, the quadrature points and the lowest eigenvalues in
 cm^-1 (D:2727-2730, format 5d24.12).  The output is plain JSON data.
 
-usage: python tests/golden/make_goldens.py c0 [c1 ...] [--rundir DIR]
+The IWAVE fixture (c0_iwave_ref.npz) holds the header records 1-11 and the
+first two wavefunction records of the reference's fort.26 for c0.
+
+usage: python tests/golden/make_goldens.py c0 [c1 ...]
        python tests/golden/make_goldens.py --parse c1 OUTFILE   (existing run)
+       python tests/golden/make_goldens.py --iwave c0 FORT26    (IWAVE fixture)
 """
 from __future__ import annotations
 
@@ -26,8 +30,20 @@ JOBS = os.path.join(REPO, "oracle", "jobs")
 PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par"}
 
 
+def _fortran_float(tok: str) -> float:
+    t = tok.replace("D", "E").replace("d", "e")
+    try:
+        return float(t)
+    except ValueError:
+        # three-digit exponents are printed without the letter: 0.32-100
+        m = re.match(r"^([+-]?\d*\.\d*)([+-]\d+)$", t)
+        if not m:
+            raise
+        return float(m.group(1) + "e" + m.group(2))
+
+
 def _floats(line: str):
-    return [float(x.replace("D", "E")) for x in line.split()]
+    return [_fortran_float(x) for x in line.split()]
 
 
 def parse_output(text: str) -> dict:
@@ -81,7 +97,31 @@ def run_config(cfg: str, rundir: str) -> dict:
     return res
 
 
+def write_iwave_fixture(cfg: str, path: str):
+    import numpy as np
+    sys.path.insert(0, REPO)
+    from tests import fortran_io as F
+    recs = F.read_records(path)
+    r2 = recs[1]
+    out = dict(
+        rec1=F.ints(recs[0]),
+        rec2_logicals=np.frombuffer(r2[0:12], dtype="<i4"),
+        rec2_reals=np.frombuffer(r2[12:52], dtype="<f8"),
+        rec2_logicals_tail=np.frombuffer(r2[52:60], dtype="<i4"),
+        rec3=F.doubles(recs[2]), rec4=F.ints(recs[3]),
+        rec5_r1=F.doubles(recs[4]), rec6_r2=F.doubles(recs[5]),
+        rec7_xalf=F.doubles(recs[6]), rec8=F.ints(recs[7]),
+        rec9_plegw=F.doubles(recs[8]), rec10_meval=F.ints(recs[9]),
+        rec11_eval=F.doubles(recs[10]), psi_level1=F.doubles(recs[11]),
+        psi_level2=F.doubles(recs[12]), nrec=np.array([len(recs)]),
+        reclens=np.array([len(r) for r in recs[:12]]))
+    np.savez_compressed(os.path.join(REPO, "tests", "golden", cfg + "_iwave_ref.npz"), **out)
+
+
 def main(argv):
+    if argv and argv[0] == "--iwave":
+        write_iwave_fixture(argv[1], argv[2])
+        return
     if argv and argv[0] == "--parse":
         cfg, path = argv[1], argv[2]
         res = parse_output(open(path).read())

@@ -1,0 +1,103 @@
+"""The drop-in executable dvr3drjz_amd against the reference executable's
+outputs on the same .job deck: the stdout eigenvalue report, the ILEV file
+and the IWAVE records that ROTLEV3/DIPOLE3/XPECT3 read."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import dvr3d_oracle as O
+from tests import fortran_io as F
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(H.REPO, "dvr3d_amd", "host", "dvr3drjz_amd")
+
+
+@pytest.fixture(scope="module")
+def c0_run(tmp_path_factory):
+    d = tmp_path_factory.mktemp("c0")
+    shutil.copy(os.path.join(H.JOBS, "c0.job"), d / "c0.job")
+    shutil.copy(os.path.join(H.JOBS, "ab2_c0.par"), d / "pes.par")
+    with open(d / "c0.job") as fi, open(d / "c0.out", "w") as fo:
+        r = subprocess.run([EXE], stdin=fi, stdout=fo, stderr=subprocess.STDOUT, cwd=d, timeout=120)
+    out = open(d / "c0.out").read()
+    assert r.returncode == 0, out[-2000:]
+    return d, out
+
+
+def _block(text, head, stop):
+    i = text.index(head)
+    vals = []
+    for line in text[i:].split("\n")[1:]:
+        if stop in line:
+            break
+        vals += [float(x.replace("D", "E")) for x in line.split()]
+    return np.array(vals)
+
+
+def test_stdout_levels_match_reference(c0_run):
+    d, out = c0_run
+    g = H.golden("c0")
+    cm = _block(out, "eigenvalues in wavenumbers", "Band origins")
+    au = _block(out, "eigenvalues in hartrees", "wavenumbers")
+    assert len(cm) == len(au) == 100
+    assert np.abs(cm - np.array(g["eval_cm"])).max() < 1e-4
+    assert np.abs(au - np.array(g["eval_au"])).max() < 1e-9
+
+
+def test_ilev_file(c0_run):
+    d, out = c0_run
+    lines = open(d / "fort.14").read().split("\n")
+    assert lines[0].startswith(" c0: AB2")                      # title (9a8)
+    assert lines[1] == "   0   0   0   0   1 100"                # 6i4: J ip jdia jpar isym meval
+    vals = [float(x.replace("D", "E")) for l in lines[2:] for x in re.findall(r"\S+", l)]
+    g = H.golden("c0")
+    assert len(vals) == 100
+    assert np.abs(np.array(vals) - np.array(g["eval_au"])).max() < 1e-9
+
+
+def test_iwave_records_match_reference(c0_run):
+    d, out = c0_run
+    ref = np.load(os.path.join(H.GOLD, "c0_iwave_ref.npz"))
+    recs = F.read_records(d / "fort.26")
+    assert len(recs) == int(ref["nrec"][0])
+    assert [len(r) for r in recs[:12]] == ref["reclens"].tolist()
+    assert np.array_equal(F.ints(recs[0]), ref["rec1"])
+    r2 = recs[1]
+    assert np.array_equal(np.frombuffer(r2[0:12], "<i4") != 0, ref["rec2_logicals"] != 0)
+    assert np.array_equal(np.frombuffer(r2[12:52], "<f8"), ref["rec2_reals"])
+    assert np.array_equal(np.frombuffer(r2[52:60], "<i4") != 0, ref["rec2_logicals_tail"] != 0)
+    assert np.array_equal(F.doubles(recs[2]), ref["rec3"])
+    assert np.array_equal(F.ints(recs[3]), ref["rec4"])
+    assert np.array_equal(F.doubles(recs[4]), ref["rec5_r1"])
+    assert np.array_equal(F.doubles(recs[5]), ref["rec6_r2"])
+    assert np.array_equal(F.doubles(recs[6]), ref["rec7_xalf"])
+    assert np.array_equal(F.ints(recs[7]), ref["rec8"])
+    assert np.abs(F.doubles(recs[8]) - ref["rec9_plegw"]).max() < 1e-14
+    assert np.array_equal(F.ints(recs[9]), ref["rec10_meval"])
+    assert np.abs(F.doubles(recs[10]) - ref["rec11_eval"]).max() < 1e-9
+    for k, name in ((11, "psi_level1"), (12, "psi_level2")):
+        a, b = F.doubles(recs[k]), ref[name]
+        s = np.sign(a @ b)                                   # eigenvector sign is arbitrary
+        assert np.abs(a - s * b).max() < 1e-7
+
+
+def test_c1_pipeline_matches_reference():
+    """Full c1 workload on the device: 500 levels within 1e-4 cm-1 of the
+    reference executable, identical truncation."""
+    job, pes = H.job_and_pes("c1")
+    inp = H.device_inputs(job, pes)
+    ctx = H.run_device(inp)
+    g = H.golden("c1")
+    iv1, iv2, n3 = ctx.sizes()
+    assert iv1.sum(axis=1).tolist() == g["nham2"]
+    assert iv2.tolist() == g["iv2"] and n3 == g["nham3"]
+    ev = ctx.eigenvalues() * O.AUTOCM
+    err = np.abs(ev - np.array(g["eval_cm"]))
+    print("c1 max |dE| (cm-1):", err.max(), "stage ms:", ctx.stage_times())
+    assert len(ev) == 500 and err.max() < 1e-4

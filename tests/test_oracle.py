@@ -140,3 +140,16 @@ def test_choose_prefers_lower_block_on_ties():
     assert iv2.tolist() == [1, 2, 1]
     iv2, low3d = O.choose(e, 10)
     assert low3d == 5 and iv2.tolist() == [2, 2, 1]
+
+
+def test_c1_oracle_matches_reference_golden():
+    """The full c1 workload (1600 angular problems of size 80, nham3 = 4000):
+    the oracle's 500 levels against the reference executable's."""
+    job, pes = H.job_and_pes("c1")
+    r = O.run(job, pes)
+    g = H.golden("c1")
+    assert r["ndim2d"].tolist() == g["nham2"]
+    assert r["iv2"].tolist() == g["iv2"]
+    ev = r["eval"] * O.AUTOCM
+    assert len(ev) == len(g["eval_cm"]) == 500
+    assert np.abs(ev - np.array(g["eval_cm"])).max() < 1e-6
