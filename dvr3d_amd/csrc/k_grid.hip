@@ -11,6 +11,10 @@ k_pes_ab2(double* __restrict__ v, const double* __restrict__ rc,
           const double* __restrict__ rb, const double* __restrict__ xa, int nc,
           int nb, int na, int zr2r1, double de, double aa, double re,
           double kb, double cte, double cc) {
+  // no fused multiply-add in this kernel: products and sums are rounded one
+  // by one, as in the reference's Fortran evaluation, and V comes out
+  // symmetric in r1 <-> r2 bit for bit
+#pragma clang fp contract(off)
   const long total = (long)nc * nb * na;
   for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
        idx += (long)gridDim.x * blockDim.x) {
@@ -23,10 +27,7 @@ k_pes_ab2(double* __restrict__ v, const double* __restrict__ rc,
     const double x = xa[a];
     const double t1 = 1.0 - exp(-aa * (r1 - re));
     const double t2 = 1.0 - exp(-aa * (r2 - re));
-    // explicitly rounded products and sum (no fused multiply-add), so that
-    // m1 + m2 -- and hence V -- is symmetric in r1 <-> r2 bit for bit, as
-    // in the reference's unfused evaluation of the same formula
-    const double msum = __dadd_rn(__dmul_rn(t1, t1), __dmul_rn(t2, t2));
+    const double msum = t1 * t1 + t2 * t2;
     const double dx = x - cte;
     v[idx] = de * msum + kb * (dx * dx) + cc * ((r1 - re) * (r2 - re));
   }

@@ -72,7 +72,8 @@ k_jacobi1d(const double* __restrict__ vgrid, const double* __restrict__ xl,
     for (int v = u; v < m; ++v) tri[base + (v - u)] = ((unsigned)u << 16) | (unsigned)v;
   }
 
-  // ---- H1 in LDS (mkham1 with ztheta=T, D:2180-2193)
+  // ---- H1 in LDS (mkham1 with ztheta=T, D:2180-2193), rounded like the
+  // reference: (V + wterm) + L*wsum with an unfused product
   const double ws = wsum[p];
   const double wt = wterm ? wterm[p] : 0.0;
   for (int idx = tid; idx < np * np; idx += kThreads) {
@@ -80,8 +81,9 @@ k_jacobi1d(const double* __restrict__ vgrid, const double* __restrict__ xl,
     const int j = idx - i * np;
     double a;
     if (i < n && j < n) {
-      a = __dmul_rn(xl[i * n + j], ws);
-      if (i == j) a = __dadd_rn(__dadd_rn(vgrid[(long)p * n + i], wt), a);
+#pragma clang fp contract(off)
+      a = xl[i * n + j] * ws;
+      if (i == j) a = (vgrid[(long)p * n + i] + wt) + a;
     } else {
       a = (i == j) ? kPad : 0.0;
     }

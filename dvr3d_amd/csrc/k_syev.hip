@@ -86,22 +86,27 @@ __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
   double* w = b.wbuf + vo;
   double* sv = b.pbuf + vo;
   double* col = A + (long)j * n;
+  // pending update of step j-1 on column j, fused with the norm of the part
+  // of the column the new reflector annihilates
+  double part = 0.0;
   if (j > 0) {
     const double taup = b.tau[vo + j - 1];
-    double part = 0.0;
-    for (int i = j + tid; i < n; i += TC) part += sv[i] * vprev[i];
-    const double K = taup * block_sum_n(part, red, TC);
+    double kp = 0.0;
+    for (int i = j + tid; i < n; i += TC) kp += sv[i] * vprev[i];
+    const double K = taup * block_sum_n(kp, red, TC);
     const double alpha = -0.5 * taup * K;
-    for (int i = j + tid; i < n; i += TC) w[i] = taup * sv[i] + alpha * vprev[i];
-    __syncthreads();
-    const double wj = w[j], vj = vprev[j];
+    const double vj = vprev[j];
+    const double wj = taup * sv[j] + alpha * vj;
     for (int i = j + tid; i < n; i += TC) {
-      col[i] -= (vprev[i] * wj + w[i] * vj);
-      sv[i] = 0.0;  // accumulator of the next product (read above, now free)
+      const double wi = taup * sv[i] + alpha * vprev[i];
+      w[i] = wi;
+      const double ci = col[i] - (vprev[i] * wj + wi * vj);
+      col[i] = ci;
+      if (i >= j + 2) part += ci * ci;
     }
-    __syncthreads();
+    if (j == n - 2) __syncthreads();  // w(n-1) is read below by thread 0
   } else {
-    for (int i = tid; i < n; i += TC) sv[i] = 0.0;
+    for (int i = j + 2 + tid; i < n; i += TC) part += col[i] * col[i];
   }
   if (j == n - 2) {
     if (tid == 0) {
@@ -114,10 +119,8 @@ __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
     }
     return;
   }
-  const double alpha = col[j + 1];
-  double part = 0.0;
-  for (int i = j + 2 + tid; i < n; i += TC) part += col[i] * col[i];
   const double sigma = block_sum_n(part, red, TC);
+  const double alpha = col[j + 1];
   double tau, beta, scal;
   if (sigma == 0.0) {
     tau = 0.0;

@@ -92,10 +92,14 @@ k_syrk_scaled(const SyrkDesc* __restrict__ desc, int ngroup,
         const int i = row0 + wy * 32 + bi * 16 + 4 * r + lk;
         const int j = col0 + wx * 32 + bj * 16 + l16;
         if (i >= N || j >= N) continue;
-        double v = __dmul_rn(acc[bi][bj][r], xk[blk[i] + (long)ldxk * blk[j]]);
-        if (i == j && dg) v = __dadd_rn(v, dg[i]);
-        C[i + (long)N * j] = v;
-        if (ti != tj) C[j + (long)N * i] = v;
+        {
+#pragma clang fp contract(off)
+          // (sum * xk) + diag, rounded one by one as in mkham2/loadh
+          double v = acc[bi][bj][r] * xk[blk[i] + (long)ldxk * blk[j]];
+          if (i == j && dg) v = v + dg[i];
+          C[i + (long)N * j] = v;
+          if (ti != tj) C[j + (long)N * i] = v;
+        }
       }
 }
 
