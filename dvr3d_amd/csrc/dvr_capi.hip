@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -69,9 +70,15 @@ struct EigWork {
   std::vector<double> htn;
   std::vector<double> hval;  // concatenated eigenvalues (heoff offsets)
 
-  // Reduce A_p (at A + aoff[p], n_p x n_p) to tridiagonal form.
+  bool wg = false;  // single-work-group solver (values and vectors split)
+  DevBuf soffb;
+
+  // Reduce A_p (at A + aoff[p], n_p x n_p) to tridiagonal form.  With the
+  // single-work-group solver this only sets the batch up; values() does the
+  // work.
   void tridiag(const std::vector<int>& n_, double* A,
-               const std::vector<long>& aoff_, hipStream_t s) {
+               const std::vector<long>& aoff_, hipStream_t s, bool use_wg = false) {
+    wg = use_wg;
     hn = n_;
     const int np = (int)hn.size();
     hvoff.assign(np, 0);
@@ -103,8 +110,10 @@ struct EigWork {
     b.gu = gu.as<double>();
     b.tnrm = tn.as<double>();
     b.pivmin = piv.as<double>();
-    launch_tridiag(s, b, hn.data());
-    launch_tridiag_bounds(s, b);
+    if (!wg) {
+      launch_tridiag(s, b, hn.data());
+      launch_tridiag_bounds(s, b);
+    }
   }
 
   // Lowest cnt[p] eigenvalues of every problem -> hval (synchronises).
@@ -120,8 +129,13 @@ struct EigWork {
       tot += cnt[p];
     }
     eval.ensure(sizeof(double) * (tot + 1));
-    launch_bisect(s, b, upload(il, hil, s), upload(kofs, hk, s), hk[np],
-                  eval.as<double>(), upload(eoff, heoff, s));
+    if (wg) {
+      launch_syev_wg(s, b, upload(kofs, cnt, s), upload(eoff, heoff, s),
+                     eval.as<double>(), nullptr, nullptr, nullptr, nullptr, nullptr, 1);
+    } else {
+      launch_bisect(s, b, upload(il, hil, s), upload(kofs, hk, s), hk[np],
+                    eval.as<double>(), upload(eoff, heoff, s));
+    }
     hval.assign(tot, 0.0);
     htn.assign(np, 0.0);
     if (tot)
@@ -147,6 +161,20 @@ struct EigWork {
   void vectors(const std::vector<int>& mv, double* Z,
                const std::vector<long>& zoff_, hipStream_t s) {
     const int np = b.nprob;
+    if (wg) {
+      std::vector<long> so(np, 0);
+      long st = 0;
+      for (int p = 0; p < np; ++p) {
+        so[p] = st;
+        st += (long)syev_wg_scratch_doubles(hn[p], mv[p]);
+      }
+      scr.ensure(sizeof(double) * (st + 1));
+      launch_syev_wg(s, b, kofs.as<int>(), eoff.as<long>(), eval.as<double>(),
+                     upload(m, mv, s), upload(zoff, zoff_, s), Z, scr.as<double>(),
+                     upload(soffb, so, s), 2);
+      DVR_HIP(hipStreamSynchronize(s));
+      return;
+    }
     std::vector<Cluster> cl;
     for (int p = 0; p < np; ++p) {
       const int mp = mv[p];
@@ -198,8 +226,8 @@ struct dvr_ctx {
   bool basis_set = false, pot_set = false, ran = false;
   double emax1 = 0.0, emax2 = 0.0;
   DevBuf xl, xkb, xkc, wsum, wterm, rc, rb, xa, v;
-  DevBuf e1full, v1full, cnt, vscr, sweeps, mflag, uniq;
-  bool mirror_used = false;
+  DevBuf e1full, v1full, cnt, vscr, sweeps, mflag, uniq, h1, lscr;
+  bool mirror_used = false, eig1d_householder = false;
   DevBuf srcrow, rowb, v1, e1p, iv1off;
   DevBuf H2, sdesc2, Z2;
   DevBuf cint, blk3, e2sel, cblk, sdesc3, H3, Z3, psi;
@@ -274,14 +302,31 @@ void run_pipeline(dvr_ctx_t* C) {
       for (int b = 0; b <= c; ++b) uniq.push_back(c * nb + b);
   }
   const int nsolve = mirror ? (int)uniq.size() : nprob;
-  const size_t vs = jacobi1d_scratch_doubles(na, nsolve);
-  if (vs) C->vscr.ensure(sizeof(double) * vs);
-  launch_jacobi1d(s, C->v.as<double>(), C->xl.as<double>(),
-                  C->wsum.as<double>(),
-                  C->has_wterm ? C->wterm.as<double>() : nullptr, na, nsolve,
-                  emax1, C->e1full.as<double>(), C->v1full.as<double>(),
-                  C->cnt.as<int>(), vs ? C->vscr.as<double>() : nullptr,
-                  mirror ? upload(C->uniq, uniq, s) : nullptr, C->sweeps.as<int>());
+  const int* d_src = mirror ? upload(C->uniq, uniq, s) : nullptr;
+  // 1D eigensolver: LDS Householder (default for npnta <= 128) or the
+  // LDS Jacobi sweeps; DVR_EIG1D=jacobi|householder overrides
+  const char* e1env = getenv("DVR_EIG1D");
+  const bool hh = e1env ? std::string(e1env) == "householder" : na <= 128;
+  C->eig1d_householder = hh;
+  if (hh) {
+    C->h1.ensure(sizeof(double) * (size_t)nprob * na * na);
+    C->lscr.ensure(sizeof(double) * syev_lds_scratch_doubles(na, nsolve));
+    launch_build_h1(s, C->v.as<double>(), C->xl.as<double>(), C->wsum.as<double>(),
+                    C->has_wterm ? C->wterm.as<double>() : nullptr, na, nsolve, d_src,
+                    C->h1.as<double>());
+    launch_syev_lds(s, C->h1.as<double>(), (long)na * na, na, na, nsolve, d_src,
+                    C->e1full.as<double>(), na, C->v1full.as<double>(), (long)na * na,
+                    C->cnt.as<int>(), D.zall ? na : -1, emax1, C->lscr.as<double>());
+  } else {
+    const size_t vs = jacobi1d_scratch_doubles(na, nsolve);
+    if (vs) C->vscr.ensure(sizeof(double) * vs);
+    launch_jacobi1d(s, C->v.as<double>(), C->xl.as<double>(),
+                    C->wsum.as<double>(),
+                    C->has_wterm ? C->wterm.as<double>() : nullptr, na, nsolve,
+                    emax1, C->e1full.as<double>(), C->v1full.as<double>(),
+                    C->cnt.as<int>(), vs ? C->vscr.as<double>() : nullptr,
+                    d_src, C->sweeps.as<int>());
+  }
   if (mirror)
     launch_mirror_copy(s, nb, na, C->e1full.as<double>(), C->v1full.as<double>(),
                        C->cnt.as<int>());
@@ -373,7 +418,13 @@ void run_pipeline(dvr_ctx_t* C) {
   DVR_HIP(hipEventRecord(C->ev[2], s));
 
   // ---- 2D eigenvalues (diag at D:2030), all of them
-  C->w2.tridiag(pn, C->H2.as<double>(), h2off, s);
+  int nmax2 = 0;
+  for (int x : pn) nmax2 = std::max(nmax2, x);
+  const char* e2env = getenv("DVR_EIG2D");
+  // multi-launch by default: measured faster for the 2D blocks (the single
+  // work-group solver is latency bound at one work-group per CU)
+  const bool wg2 = e2env ? (std::string(e2env) == "wg") : false;
+  C->w2.tridiag(pn, C->H2.as<double>(), h2off, s, wg2);
   C->w2.values(pn, s);
   DVR_HIP(hipEventRecord(C->ev[3], s));
   {
@@ -795,7 +846,13 @@ int dvr_syev_batch(int32_t nprob, const int32_t* n, const double* a,
     }
     DVR_HIP(hipMemcpyAsync(A.p, stage.data(), sizeof(double) * tot, hipMemcpyHostToDevice, s));
     EigWork W;
-    W.tridiag(hn, A.as<double>(), aoff, s);
+    // DVR_SYEV=wg selects the single-work-group solver, =multi the
+    // multi-launch one; by default problems up to kWgMaxN use the former
+    int nmx = 0;
+    for (int x : hn) nmx = std::max(nmx, x);
+    const char* senv = getenv("DVR_SYEV");
+    const bool use_wg = senv ? std::string(senv) == "wg" : false;
+    W.tridiag(hn, A.as<double>(), aoff, s, use_wg);
     std::vector<int> want(nprob);
     for (int p = 0; p < nprob; ++p) want[p] = std::min(hn[p], cntv[p] + (cntv[p] < hn[p] ? 16 : 0));
     W.values(want, s);

@@ -2,6 +2,7 @@
 // Kernels live in k_*.hip; each file exposes host-side launch_* wrappers.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -75,6 +76,26 @@ inline void prof_end(hipStream_t s) {
 inline void prof_phase(const char* ph) {
   if (g_prof) g_prof->phase = ph;
 }
+// Launch a kernel; while profiling, through hipExtLaunchKernelGGL so the two
+// events timestamp the dispatch itself (no extra queue packets around it).
+#define DVR_LAUNCH(pname, pbytes, pflops, kern, grid, block, shm, strm, ...)      \
+  do {                                                                           \
+    if (::dvr::g_prof) {                                                         \
+      ::dvr::ProfRec r_;                                                         \
+      r_.name = ::dvr::g_prof->phase.empty()                                     \
+                    ? std::string(pname)                                         \
+                    : ::dvr::g_prof->phase + ":" + (pname);                      \
+      r_.a = ::dvr::g_prof->event();                                             \
+      r_.b = ::dvr::g_prof->event();                                             \
+      r_.bytes = (pbytes);                                                       \
+      r_.flops = (pflops);                                                       \
+      ::dvr::g_prof->recs.push_back(r_);                                         \
+      hipExtLaunchKernelGGL(kern, grid, block, shm, strm, r_.a, r_.b, 0,         \
+                            __VA_ARGS__);                                        \
+    } else {                                                                     \
+      hipLaunchKernelGGL(kern, grid, block, shm, strm, __VA_ARGS__);             \
+    }                                                                            \
+  } while (0)
 
 // ---------------------------------------------------------------------------
 // grid-side kernels (k_grid.hip)
@@ -210,5 +231,26 @@ void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
                           const int* h_m, double* tbuf, const long* toff,
                           long tbuf_total);
 int backtransform_nb();
+// Single-work-group solver (k_syev_wg.hip): phase bit 0 = tridiagonalise and
+// find the lowest cnt[p] eigenvalues; bit 1 = the lowest mvec[p] vectors into
+// Z (from the reflectors, d, e and eigenvalues left by phase 0).
+void launch_syev_wg(hipStream_t s, const EigBatch& b, const int* cnt,
+                    const long* eoff, double* eval, const int* mvec,
+                    const long* zoff, double* Z, double* scratch,
+                    const long* soff, int phase);
+size_t syev_wg_scratch_doubles(int n, int m);
+// LDS-resident single-work-group solver for n <= 128 (k_syev_lds.hip).
+// Problem p = src[slot] (or slot): A at A + p*astride (n x n, both
+// triangles); lowest kcnt eigenvalues -> eval[p*estride + k]; lowest m
+// eigenvectors -> Z[p*zstride + k*n + i], m = mfixed or, when mfixed < 0,
+// the count of eigenvalues <= emax (also written to cnt[p]).
+void launch_syev_lds(hipStream_t s, const double* A, long astride, int n, int kcnt,
+                     int nsolve, const int* src, double* eval, long estride, double* Z,
+                     long zstride, int* cnt, int mfixed, double emax, double* scratch);
+size_t syev_lds_scratch_doubles(int n, int nsolve);
+// Dense H1 of every solved (c,b) problem into H[p*n*n] (column-major).
+void launch_build_h1(hipStream_t s, const double* vgrid, const double* xl,
+                     const double* wsum, const double* wterm, int n, int nsolve,
+                     const int* src, double* H);
 
 }  // namespace dvr

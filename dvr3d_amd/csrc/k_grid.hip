@@ -39,11 +39,9 @@ void launch_pes_ab2(hipStream_t s, double* v, const double* rc,
   const long total = (long)nc * nb * na;
   int grid = (int)((total + 255) / 256);
   if (grid > 65535 * 8) grid = 65535 * 8;
-  prof_begin(s, "pes", 8.0 * total, 0.0);
-  hipLaunchKernelGGL(k_pes_ab2, dim3(grid), dim3(256), 0, s, v, rc, rb, xa, nc,
-                     nb, na, zr2r1, par[0], par[1], par[2], par[3], par[4],
-                     par[5]);
-  prof_end(s);
+  DVR_LAUNCH("pes", 8.0 * total, 0.0, k_pes_ab2, dim3(grid), dim3(256), 0, s, v, rc, rb, xa, nc,
+                   nb, na, zr2r1, par[0], par[1], par[2], par[3], par[4],
+                   par[5]);
   DVR_LAUNCH_CHECK();
 }
 
@@ -69,10 +67,8 @@ void launch_pack_1d(hipStream_t s, const double* v1full, const double* e1full,
   const long total = (long)rows * n;
   int grid = (int)((total + 255) / 256);
   if (grid > 65535 * 8) grid = 65535 * 8;
-  prof_begin(s, "pack1d", 16.0 * total, 0.0);
-  hipLaunchKernelGGL(k_pack_1d, dim3(grid), dim3(256), 0, s, v1full, e1full,
-                     src_row, rows, n, v1, e1p);
-  prof_end(s);
+  DVR_LAUNCH("pack1d", 16.0 * total, 0.0, k_pack_1d, dim3(grid), dim3(256), 0, s, v1full, e1full,
+                   src_row, rows, n, v1, e1p);
   DVR_LAUNCH_CHECK();
 }
 
@@ -111,10 +107,8 @@ void launch_cint(hipStream_t s, const double* v1, const double* c2,
   if (total <= 0) return;
   int grid = (int)((total + 255) / 256);
   if (grid > 65535 * 16) grid = 65535 * 16;
-  prof_begin(s, "cint", 8.0 * total, 0.0);
-  hipLaunchKernelGGL(k_cint, dim3(grid), dim3(256), 0, s, v1, c2, cint, cb,
-                     blk3, iv1off, na, nb, n3);
-  prof_end(s);
+  DVR_LAUNCH("cint", 8.0 * total, 0.0, k_cint, dim3(grid), dim3(256), 0, s, v1, c2, cint, cb,
+                   blk3, iv1off, na, nb, n3);
   DVR_LAUNCH_CHECK();
 }
 
@@ -160,10 +154,8 @@ void launch_trans(hipStream_t s, const double* cint, const double* c3,
   if (total <= 0) return;
   int grid = (int)((total + 255) / 256);
   if (grid > 65535 * 16) grid = 65535 * 16;
-  prof_begin(s, "trans", 8.0 * total, 2.0 * (double)na * nb * n3 * nvec);
-  hipLaunchKernelGGL(k_trans, dim3(grid), dim3(256), 0, s, cint, c3, psi, cb,
-                     nc, na, nb, n3, nvec, zr2r1, idvr, npnt1, npnt2);
-  prof_end(s);
+  DVR_LAUNCH("trans", 8.0 * total, 2.0 * (double)na * nb * n3 * nvec, k_trans, dim3(grid), dim3(256), 0, s, cint, c3, psi, cb,
+                   nc, na, nb, n3, nvec, zr2r1, idvr, npnt1, npnt2);
   DVR_LAUNCH_CHECK();
 }
 

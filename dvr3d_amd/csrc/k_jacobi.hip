@@ -295,9 +295,7 @@ void launch_mirror_copy(hipStream_t s, int nb, int n, double* e1, double* v1,
   const long total = (long)nb * nb * ((long)n * n + n + 1);
   int grid = (int)((total + 255) / 256);
   if (grid > 65535 * 16) grid = 65535 * 16;
-  prof_begin(s, "mirror_copy", 16.0 * (double)nb * (nb - 1) / 2 * ((double)n * n + n), 0.0);
-  hipLaunchKernelGGL(k_mirror_copy, dim3(grid), dim3(256), 0, s, nb, n, e1, v1, cnt);
-  prof_end(s);
+  DVR_LAUNCH("mirror_copy", 16.0 * (double)nb * (nb - 1) / 2 * ((double)n * n + n), 0.0, k_mirror_copy, dim3(grid), dim3(256), 0, s, nb, n, e1, v1, cnt);
   DVR_LAUNCH_CHECK();
 }
 
@@ -321,11 +319,9 @@ void launch_jacobi1d(hipStream_t s, const double* vgrid, const double* xl,
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds));
   (void)0;
-  prof_begin(s, "jacobi1d", 8.0 * nprob * ((double)n * n + 3.0 * n), 0.0);
-  hipLaunchKernelGGL(k_jacobi1d, dim3(nprob), dim3(kThreads), lds, s, vgrid, xl,
-                     wsum, wterm, n, emaxau, e1, v1, cnt, vscratch, vlds ? 1 : 0,
-                     src_prob, sweeps);
-  prof_end(s);
+  DVR_LAUNCH("jacobi1d", 8.0 * nprob * ((double)n * n + 3.0 * n), 0.0, k_jacobi1d, dim3(nprob), dim3(kThreads), lds, s, vgrid, xl,
+                   wsum, wterm, n, emaxau, e1, v1, cnt, vscratch, vlds ? 1 : 0,
+                   src_prob, sweeps);
   DVR_LAUNCH_CHECK();
 }
 

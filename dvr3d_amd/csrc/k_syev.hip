@@ -150,10 +150,11 @@ __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
 // A(j+1:n, j+1:n), both triangles kept current so every column is contiguous:
 //   A(i,k) -= v(i) w(k) + w(i) v(k)      (pending update of step j-1)
 //   s(k)    = sum_i A(i,k) v_j(i)
-// One wavefront per column, four columns per work-group; each lane keeps
-// four independent row loads in flight.  s(k) is complete within the wave,
-// so no atomics are needed.
-constexpr int UC = 4;
+// Each wavefront owns two columns and walks their rows together, so the three
+// row vectors (v_{j-1}, w, v_j) are loaded once per row for both columns; each
+// lane keeps four rows of both columns in flight.  s(k) is complete within the
+// wave, so no atomics are needed.
+constexpr int UC = 8;   // columns per work-group (2 per wave)
 
 __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   const int bid = blockIdx.x;
@@ -173,46 +174,78 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   const double* w = b.wbuf + vo;
   double* sv = b.pbuf + vo;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int k = j + 1 + (bid - base) * UC + wave;
-  if (k >= n) return;
-  double* colk = A + (long)k * n;
+  const int k0 = j + 1 + (bid - base) * UC + 2 * wave;
+  if (k0 >= n) return;
+  const bool two = k0 + 1 < n;
+  double* ca = A + (long)k0 * n;
+  double* cb = two ? A + (long)(k0 + 1) * n : ca;
   const int i0 = j + 1;
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+  double sa0 = 0.0, sa1 = 0.0, sb0 = 0.0, sb1 = 0.0;
   if (j > 0) {
-    const double wk = w[k], vk = vprev[k];
+    const double wa = w[k0], pa = vprev[k0];
+    const double wb = two ? w[k0 + 1] : 0.0, pb = two ? vprev[k0 + 1] : 0.0;
     int i = i0 + lane;
     for (; i + 192 < n; i += 256) {
-      const double a0 = colk[i], a1 = colk[i + 64], a2 = colk[i + 128], a3 = colk[i + 192];
-      const double u0 = a0 - (vprev[i] * wk + w[i] * vk);
-      const double u1 = a1 - (vprev[i + 64] * wk + w[i + 64] * vk);
-      const double u2 = a2 - (vprev[i + 128] * wk + w[i + 128] * vk);
-      const double u3 = a3 - (vprev[i + 192] * wk + w[i + 192] * vk);
-      colk[i] = u0;
-      colk[i + 64] = u1;
-      colk[i + 128] = u2;
-      colk[i + 192] = u3;
-      acc0 += u0 * vcur[i];
-      acc1 += u1 * vcur[i + 64];
-      acc2 += u2 * vcur[i + 128];
-      acc3 += u3 * vcur[i + 192];
+      double x[4], y[4], vp[4], ww[4], vc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        x[q] = ca[i + 64 * q];
+        y[q] = cb[i + 64 * q];
+        vp[q] = vprev[i + 64 * q];
+        ww[q] = w[i + 64 * q];
+        vc[q] = vcur[i + 64 * q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double ua = x[q] - (vp[q] * wa + ww[q] * pa);
+        const double ub = y[q] - (vp[q] * wb + ww[q] * pb);
+        ca[i + 64 * q] = ua;
+        if (two) cb[i + 64 * q] = ub;
+        if (q & 1) {
+          sa1 += ua * vc[q];
+          sb1 += ub * vc[q];
+        } else {
+          sa0 += ua * vc[q];
+          sb0 += ub * vc[q];
+        }
+      }
     }
     for (; i < n; i += 64) {
-      const double u = colk[i] - (vprev[i] * wk + w[i] * vk);
-      colk[i] = u;
-      acc0 += u * vcur[i];
+      const double vpi = vprev[i], wi = w[i], vci = vcur[i];
+      const double ua = ca[i] - (vpi * wa + wi * pa);
+      const double ub = cb[i] - (vpi * wb + wi * pb);
+      ca[i] = ua;
+      if (two) cb[i] = ub;
+      sa0 += ua * vci;
+      sb0 += ub * vci;
     }
   } else {
     int i = i0 + lane;
     for (; i + 192 < n; i += 256) {
-      acc0 += colk[i] * vcur[i];
-      acc1 += colk[i + 64] * vcur[i + 64];
-      acc2 += colk[i + 128] * vcur[i + 128];
-      acc3 += colk[i + 192] * vcur[i + 192];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double vc = vcur[i + 64 * q];
+        if (q & 1) {
+          sa1 += ca[i + 64 * q] * vc;
+          sb1 += cb[i + 64 * q] * vc;
+        } else {
+          sa0 += ca[i + 64 * q] * vc;
+          sb0 += cb[i + 64 * q] * vc;
+        }
+      }
     }
-    for (; i < n; i += 64) acc0 += colk[i] * vcur[i];
+    for (; i < n; i += 64) {
+      const double vc = vcur[i];
+      sa0 += ca[i] * vc;
+      sb0 += cb[i] * vc;
+    }
   }
-  const double acc = wave_sum((acc0 + acc1) + (acc2 + acc3));
-  if (lane == 0) sv[k] = acc;
+  const double ra = wave_sum(sa0 + sa1);
+  const double rb = wave_sum(sb0 + sb1);
+  if (lane == 0) {
+    sv[k0] = ra;
+    if (two) sv[k0 + 1] = rb;
+  }
 }
 
 void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
@@ -234,13 +267,9 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
         uflops += (j > 0 ? 6.0 : 2.0) * t * t;
       }
     }
-    prof_begin(s, "tri_col", cbytes, 0.0);
-    hipLaunchKernelGGL(k_tri_col, dim3(b.nprob), dim3(TC), 0, s, b, j);
-    prof_end(s);
+    DVR_LAUNCH("tri_col", cbytes, 0.0, k_tri_col, dim3(b.nprob), dim3(TC), 0, s, b, j);
     if (grid > 0) {
-      prof_begin(s, "tri_upd", ubytes, uflops);
-      hipLaunchKernelGGL(k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
-      prof_end(s);
+      DVR_LAUNCH("tri_upd", ubytes, uflops, k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
     }
   }
   DVR_LAUNCH_CHECK();
@@ -365,10 +394,8 @@ void launch_bisect(hipStream_t s, const EigBatch& b, const int* il,
                    const long* eoff) {
   if (total <= 0) return;
   const int grid = (total + TB / 64 - 1) / (TB / 64);
-  prof_begin(s, "bisect", 0.0, 0.0);
-  hipLaunchKernelGGL(k_bisect, dim3(grid), dim3(TB), 0, s, b, il, kofs, total,
-                     eval, eoff);
-  prof_end(s);
+  DVR_LAUNCH("bisect", 0.0, 0.0, k_bisect, dim3(grid), dim3(TB), 0, s, b, il, kofs, total,
+                   eval, eoff);
   DVR_LAUNCH_CHECK();
 }
 
@@ -498,10 +525,8 @@ void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
                   const long* zoff, double* scratch, int nthreads) {
   if (ncl <= 0) return;
   const int grid = (ncl + TB - 1) / TB;
-  prof_begin(s, "stein", 0.0, 0.0);
-  hipLaunchKernelGGL(k_stein, dim3(grid), dim3(TB), 0, s, b, cl, ncl, eval,
-                     eoff, Z, zoff, scratch, nthreads);
-  prof_end(s);
+  DVR_LAUNCH("stein", 0.0, 0.0, k_stein, dim3(grid), dim3(TB), 0, s, b, cl, ncl, eval,
+                   eoff, Z, zoff, scratch, nthreads);
   DVR_LAUNCH_CHECK();
 }
 
@@ -677,15 +702,11 @@ void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
     oflops += 4.0 * n * n * mm;
   }
   if (nbt > 0) {
-    prof_begin(s, "larft", 0.0, 0.0);
-    hipLaunchKernelGGL(k_larft, dim3(nbt), dim3(TB), 0, s, b, d, tbuf, toff);
-    prof_end(s);
+    DVR_LAUNCH("larft", 0.0, 0.0, k_larft, dim3(nbt), dim3(TB), 0, s, b, d, tbuf, toff);
   }
   if (ntt > 0) {
-    prof_begin(s, "ormtr", obytes, oflops);
-    hipLaunchKernelGGL(k_ormtr, dim3(ntt), dim3(TB), 0, s, b, Z, zoff, m,
-                       d + np + 1, tbuf, toff);
-    prof_end(s);
+    DVR_LAUNCH("ormtr", obytes, oflops, k_ormtr, dim3(ntt), dim3(TB), 0, s, b, Z, zoff, m,
+                   d + np + 1, tbuf, toff);
   }
   DVR_LAUNCH_CHECK();
   DVR_HIP(hipFreeAsync(d, s));

@@ -107,10 +107,8 @@ void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
                         int total_tiles, const double* xk, int ldxk,
                         double bytes, double flops) {
   if (total_tiles <= 0) return;
-  prof_begin(s, "syrk", bytes, flops);
-  hipLaunchKernelGGL(k_syrk_scaled, dim3(total_tiles), dim3(256), 0, s, d_desc,
-                     ngroup, xk, ldxk);
-  prof_end(s);
+  DVR_LAUNCH("syrk", bytes, flops, k_syrk_scaled, dim3(total_tiles), dim3(256), 0, s, d_desc,
+                   ngroup, xk, ldxk);
   DVR_LAUNCH_CHECK();
 }
 

@@ -5,10 +5,20 @@ The replaced routine is dsyev('V','L') behind SUBROUTINE diag
 check: eigenvalues to a few ulps of ||A||, eigenvectors by residual and
 orthogonality (their signs are arbitrary in both).
 """
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["multi", "wg"], autouse=True)
+def solver_kind(request, monkeypatch):
+    """Run every case through both the multi-launch and the single-work-group
+    solver."""
+    monkeypatch.setenv("DVR_SYEV", request.param)
+    return request.param
 
 
 def _sym(n, rng, spread=1.0):

@@ -21,7 +21,10 @@ def c0_setup():
     return job, pes, inp, ref
 
 
-def test_c0_pipeline_matches_reference(c0_setup):
+@pytest.mark.parametrize("eig1d", ["householder", "jacobi"])
+def test_c0_pipeline_matches_reference(c0_setup, eig1d, monkeypatch):
+    """Both 1D solvers (LDS Householder and LDS Jacobi sweeps)."""
+    monkeypatch.setenv("DVR_EIG1D", eig1d)
     job, pes, inp, ref = c0_setup
     ctx = H.run_device(inp)
     gold = H.golden("c0")
