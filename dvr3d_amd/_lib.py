@@ -30,7 +30,10 @@ EXPORTS = (
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
     "dvr_diag", "dvr_syev_batch", "dvr_set_profiling", "dvr_get_profile",
+    "dvr_get_solver_stats", "dvr_ldl_solve",
 )
+SOLVER_STATS = ("status", "iterations", "matvecs", "solves", "factorizations", "slices",
+                "certified", "missing", "dense_fallback")
 
 
 class Dims(C.Structure):
@@ -80,6 +83,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_syev_batch.argtypes = [C.c_int32, I, D, I, D, D]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
+    lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
+    lib.dvr_ldl_solve.argtypes = [D, C.c_int32, C.c_double, D, C.c_int32, D, I]
     for name in EXPORTS:
         if name not in ("dvr_ctx_destroy", "dvr_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -213,6 +218,12 @@ class Context:
         self._check(self.lib.dvr_get_stage_times(self.h, _dp(out)))
         return dict(zip(STAGE_NAMES, out.tolist()))
 
+    def solver_stats(self):
+        out = np.zeros(len(SOLVER_STATS), dtype=np.int32)
+        self._check(self.lib.dvr_get_solver_stats(
+            self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), len(SOLVER_STATS)))
+        return dict(zip(SOLVER_STATS, out.tolist()))
+
 
 def syev_batch(mats, nvec=None, vectors=True):
     """Lowest eigenpairs of several symmetric matrices (lower triangle read)."""
@@ -233,6 +244,26 @@ def syev_batch(mats, nvec=None, vectors=True):
             zs.append(z[zo:zo + n[p] * nv[p]].reshape(nv[p], n[p]).T)
             zo += n[p] * nv[p]
     return (ws, zs) if vectors else ws
+
+
+def ldl_solve(a: np.ndarray, sigma: float, r: np.ndarray | None = None):
+    """dvr_ldl_solve: inertia of a - sigma I and (a - sigma I)^-1 r."""
+    lib = load()
+    h = np.asfortranarray(a, dtype=np.float64)
+    n = h.shape[0]
+    nu = np.zeros(1, dtype=np.int32)
+    if r is None:
+        rc = lib.dvr_ldl_solve(_dp(h), n, float(sigma), None, 0, None,
+                               nu.ctypes.data_as(C.POINTER(C.c_int32)))
+        t = None
+    else:
+        rr = np.asfortranarray(r.reshape(n, -1), dtype=np.float64)
+        t = np.zeros_like(rr, order="F")
+        rc = lib.dvr_ldl_solve(_dp(h), n, float(sigma), _dp(rr), rr.shape[1], _dp(t),
+                               nu.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_ldl_solve failed")
+    return int(nu[0]), t
 
 
 def diag(ham: np.ndarray):

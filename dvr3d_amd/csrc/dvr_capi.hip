@@ -26,195 +26,6 @@ thread_local Profiler* g_prof = nullptr;
 
 namespace {
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  DevBuf() = default;
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
-  void ensure(size_t b) {
-    if (b <= bytes && p) return;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    if (b == 0) b = 8;
-    DVR_HIP(hipMalloc(&p, b));
-    bytes = b;
-  }
-  template <class T>
-  T* as() const { return static_cast<T*>(p); }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-};
-
-template <class T>
-T* upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
-  d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1));
-  if (!h.empty())
-    DVR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(),
-                           hipMemcpyHostToDevice, s));
-  return d.as<T>();
-}
-
-// Working storage of one batched eigensolve.
-struct EigWork {
-  DevBuf n, aoff, voff, d, e, e2, tau, vb0, vb1, wb, pb, gl, gu, tn, piv;
-  DevBuf il, kofs, eoff, eval;
-  DevBuf zoff, m, clus, scr, tbuf, toff;
-  EigBatch b{};
-  std::vector<int> hn;
-  std::vector<long> hvoff;
-  std::vector<int> hcnt;
-  std::vector<long> heoff;
-  std::vector<double> htn;
-  std::vector<double> hval;  // concatenated eigenvalues (heoff offsets)
-
-  bool wg = false;  // single-work-group solver (values and vectors split)
-  DevBuf soffb;
-
-  // Reduce A_p (at A + aoff[p], n_p x n_p) to tridiagonal form.  With the
-  // single-work-group solver this only sets the batch up; values() does the
-  // work.
-  void tridiag(const std::vector<int>& n_, double* A,
-               const std::vector<long>& aoff_, hipStream_t s, bool use_wg = false) {
-    wg = use_wg;
-    hn = n_;
-    const int np = (int)hn.size();
-    hvoff.assign(np, 0);
-    long tot = 0;
-    int nmax = 0;
-    for (int p = 0; p < np; ++p) {
-      hvoff[p] = tot;
-      tot += hn[p];
-      nmax = std::max(nmax, hn[p]);
-    }
-    const size_t vb = sizeof(double) * (tot + 1);
-    for (DevBuf* x : {&d, &e, &e2, &tau, &vb0, &vb1, &wb, &pb}) x->ensure(vb);
-    for (DevBuf* x : {&gl, &gu, &tn, &piv}) x->ensure(sizeof(double) * np);
-    b.nprob = np;
-    b.nmax = nmax;
-    b.n = upload(n, hn, s);
-    b.aoff = upload(aoff, aoff_, s);
-    b.voff = upload(voff, hvoff, s);
-    b.A = A;
-    b.d = d.as<double>();
-    b.e = e.as<double>();
-    b.e2 = e2.as<double>();
-    b.tau = tau.as<double>();
-    b.vbuf0 = vb0.as<double>();
-    b.vbuf1 = vb1.as<double>();
-    b.wbuf = wb.as<double>();
-    b.pbuf = pb.as<double>();
-    b.gl = gl.as<double>();
-    b.gu = gu.as<double>();
-    b.tnrm = tn.as<double>();
-    b.pivmin = piv.as<double>();
-    if (!wg) {
-      launch_tridiag(s, b, hn.data());
-      launch_tridiag_bounds(s, b);
-    }
-  }
-
-  // Lowest cnt[p] eigenvalues of every problem -> hval (synchronises).
-  void values(const std::vector<int>& cnt, hipStream_t s) {
-    const int np = b.nprob;
-    hcnt = cnt;
-    std::vector<int> hil(np, 0), hk(np + 1, 0);
-    heoff.assign(np, 0);
-    long tot = 0;
-    for (int p = 0; p < np; ++p) {
-      hk[p + 1] = hk[p] + cnt[p];
-      heoff[p] = tot;
-      tot += cnt[p];
-    }
-    eval.ensure(sizeof(double) * (tot + 1));
-    if (wg) {
-      launch_syev_wg(s, b, upload(kofs, cnt, s), upload(eoff, heoff, s),
-                     eval.as<double>(), nullptr, nullptr, nullptr, nullptr, nullptr, 1);
-    } else {
-      launch_bisect(s, b, upload(il, hil, s), upload(kofs, hk, s), hk[np],
-                    eval.as<double>(), upload(eoff, heoff, s));
-    }
-    hval.assign(tot, 0.0);
-    htn.assign(np, 0.0);
-    if (tot)
-      DVR_HIP(hipMemcpyAsync(hval.data(), eval.p, sizeof(double) * tot,
-                             hipMemcpyDeviceToHost, s));
-    DVR_HIP(hipMemcpyAsync(htn.data(), tn.p, sizeof(double) * np,
-                           hipMemcpyDeviceToHost, s));
-    DVR_HIP(hipStreamSynchronize(s));
-  }
-
-  // Number of vectors to compute so that the last wanted one does not split
-  // a cluster of close eigenvalues.
-  int extend(int p, int want) const {
-    if (want <= 0) return 0;
-    const double ctol = 1.0e-6 * std::max(htn[p], 1e-300);
-    const double* v = hval.data() + heoff[p];
-    int k = want;
-    while (k < hcnt[p] && v[k] - v[k - 1] < ctol) ++k;
-    return k;
-  }
-
-  // Eigenvectors 0..m[p]-1 of every problem into Z (n_p x m_p at zoff[p]).
-  void vectors(const std::vector<int>& mv, double* Z,
-               const std::vector<long>& zoff_, hipStream_t s) {
-    const int np = b.nprob;
-    if (wg) {
-      std::vector<long> so(np, 0);
-      long st = 0;
-      for (int p = 0; p < np; ++p) {
-        so[p] = st;
-        st += (long)syev_wg_scratch_doubles(hn[p], mv[p]);
-      }
-      scr.ensure(sizeof(double) * (st + 1));
-      launch_syev_wg(s, b, kofs.as<int>(), eoff.as<long>(), eval.as<double>(),
-                     upload(m, mv, s), upload(zoff, zoff_, s), Z, scr.as<double>(),
-                     upload(soffb, so, s), 2);
-      DVR_HIP(hipStreamSynchronize(s));
-      return;
-    }
-    std::vector<Cluster> cl;
-    for (int p = 0; p < np; ++p) {
-      const int mp = mv[p];
-      if (mp <= 0) continue;
-      const double ctol = 1.0e-6 * std::max(htn[p], 1e-300);
-      const double* v = hval.data() + heoff[p];
-      int kb = 0;
-      for (int k = 1; k <= mp; ++k) {
-        if (k == mp || v[k] - v[k - 1] >= ctol) {
-          cl.push_back(Cluster{p, kb, k});
-          kb = k;
-        }
-      }
-    }
-    const int ncl = (int)cl.size();
-    if (ncl == 0) return;
-    const int T = ncl;
-    scr.ensure(sizeof(double) * stein_scratch_doubles(b.nmax, T));
-    DevBuf dcl;
-    dcl.ensure(sizeof(Cluster) * ncl);
-    DVR_HIP(hipMemcpyAsync(dcl.p, cl.data(), sizeof(Cluster) * ncl,
-                           hipMemcpyHostToDevice, s));
-    const long* dz = upload(zoff, zoff_, s);
-    launch_stein(s, b, dcl.as<Cluster>(), ncl, eval.as<double>(),
-                 eoff.as<long>(), Z, dz, scr.as<double>(), T);
-    // WY factors
-    const int NBk = backtransform_nb();
-    std::vector<long> ht(np, 0);
-    long tt = 0;
-    for (int p = 0; p < np; ++p) {
-      ht[p] = tt;
-      const int nr = hn[p] - 2;
-      tt += nr > 0 ? (long)((nr + NBk - 1) / NBk) * NBk * NBk : 0;
-    }
-    tbuf.ensure(sizeof(double) * (tt + 1));
-    launch_backtransform(s, b, Z, dz, upload(m, mv, s), hn.data(), mv.data(),
-                         tbuf.as<double>(), upload(toff, ht, s), tt);
-    DVR_HIP(hipStreamSynchronize(s));
-  }
-};
 
 }  // namespace
 
@@ -232,6 +43,8 @@ struct dvr_ctx {
   DevBuf H2, sdesc2, Z2;
   DevBuf cint, blk3, e2sel, cblk, sdesc3, H3, Z3, psi;
   EigWork w2, w3;
+  DavidsonSolver dav;
+  int dav_status = -1, tune_mv = 0;
   bool has_wterm = false;
   std::vector<int> h_iv1;    // [c*npntb + b]
   std::vector<int> h_nham2, h_iv2;
@@ -255,6 +68,11 @@ namespace {
 int fail(dvr_ctx_t* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
+}
+
+int env_int(const char* name, int def) {
+  const char* v = getenv(name);
+  return v ? atoi(v) : def;
 }
 
 struct DeviceGuard {
@@ -310,10 +128,10 @@ void run_pipeline(dvr_ctx_t* C) {
   C->eig1d_householder = hh;
   if (hh) {
     C->h1.ensure(sizeof(double) * (size_t)nprob * na * na);
-    C->lscr.ensure(sizeof(double) * syev_lds_scratch_doubles(na, nsolve));
     launch_build_h1(s, C->v.as<double>(), C->xl.as<double>(), C->wsum.as<double>(),
                     C->has_wterm ? C->wterm.as<double>() : nullptr, na, nsolve, d_src,
                     C->h1.as<double>());
+    C->lscr.ensure(sizeof(double) * syev_lds_scratch_doubles(na, nsolve));
     launch_syev_lds(s, C->h1.as<double>(), (long)na * na, na, na, nsolve, d_src,
                     C->e1full.as<double>(), na, C->v1full.as<double>(), (long)na * na,
                     C->cnt.as<int>(), D.zall ? na : -1, emax1, C->lscr.as<double>());
@@ -540,19 +358,45 @@ void run_pipeline(dvr_ctx_t* C) {
   // ---- final solve (diag3d, D:2661-2764)
   const int meval = std::min(D.neval, n3);
   C->meval = meval;
-  const int want = std::min(n3, meval + 16);
-  C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
-  C->w3.values(std::vector<int>{want}, s);
-  C->h_eval.assign(C->w3.hval.begin(), C->w3.hval.begin() + meval);
+  const int nvec = std::min(D.nvec, meval);
+  C->nvec_done = nvec;
+  C->dav_status = -1;
+  bool dense = D.solver != DVR_SOLVER_DAVIDSON;
+  if (!dense) {
+    // block-Davidson with certified slices; the dense path is the fallback
+    DavParams P;
+    P.nev = meval;
+    P.block = env_int("DVR_DAV_BLOCK", 64);
+    P.slice = env_int("DVR_DAV_SLICE", 128);
+    P.mmax = env_int("DVR_DAV_MMAX", 320);
+    P.max_iter = env_int("DVR_DAV_MAXIT", 2000);
+    if (const char* t = getenv("DVR_DAV_TOL")) P.tol = atof(t);
+    C->dav.tune_mv = C->tune_mv;
+    C->dav_status = C->dav.solve(C->H3.as<double>(), n3, P, s);
+    if (C->dav_status == 0) {
+      C->h_eval.assign(C->dav.eval.begin(), C->dav.eval.begin() + meval);
+    } else {
+      dense = true;   // H3 is intact: the Davidson solver works on copies
+    }
+  }
+  if (dense) {
+    const int want = std::min(n3, meval + 16);
+    C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
+    C->w3.values(std::vector<int>{want}, s);
+    C->h_eval.assign(C->w3.hval.begin(), C->w3.hval.begin() + meval);
+  }
   DVR_HIP(hipEventRecord(C->ev[6], s));
 
   // ---- eigenvectors and wavefunctions (trans, D:3650-3845)
-  const int nvec = std::min(D.nvec, meval);
-  C->nvec_done = nvec;
   if (nvec > 0) {
-    const int mext = C->w3.extend(0, nvec);
-    C->Z3.ensure(sizeof(double) * (size_t)n3 * mext);
-    C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);
+    if (dense) {
+      const int mext = C->w3.extend(0, nvec);
+      C->Z3.ensure(sizeof(double) * (size_t)n3 * mext);
+      C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);
+    } else {
+      C->Z3.ensure(sizeof(double) * (size_t)n3 * nvec);
+      C->dav.vectors(nvec, C->Z3.as<double>(), s);
+    }
     const size_t npsi = (size_t)D.idvr * D.npnt1 * D.npnt2;
     C->psi.ensure(sizeof(double) * npsi * nvec);
     DVR_HIP(hipMemsetAsync(C->psi.p, 0, sizeof(double) * npsi * nvec, s));
@@ -687,8 +531,6 @@ int dvr_run(dvr_ctx_t* c) {
   if (!c) return DVR_ERR_ARG;
   if (!c->basis_set || !c->pot_set)
     return fail(c, DVR_ERR_ARG, "dvr_run: basis and potential must be set first");
-  if (c->dims.solver != DVR_SOLVER_DENSE)
-    return fail(c, DVR_ERR_ARG, "dvr_run: requested solver is not available in this build");
   c->ran = false;
   try {
     DeviceGuard g(c->device);
@@ -789,6 +631,49 @@ int dvr_get_vectors(const dvr_ctx_t* c, double* vec) {
   const size_t n = (size_t)c->nham3 * c->nvec_done;
   if (hipMemcpy(vec, c->Z3.p, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
     return DVR_ERR_HIP;
+  return DVR_OK;
+}
+
+int dvr_get_solver_stats(const dvr_ctx_t* c, int32_t* st, int32_t n) {
+  if (!c || !c->ran || !st || n < 1) return DVR_ERR_ARG;
+  const DavStats& d = c->dav.stats;
+  const int32_t v[9] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
+                        d.slices, d.certified, d.missing,
+                        c->dims.solver == DVR_SOLVER_DAVIDSON && c->dav_status != 0};
+  for (int i = 0; i < n && i < 9; ++i) st[i] = v[i];
+  return DVR_OK;
+}
+
+int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int32_t nrhs,
+                  double* t, int32_t* inertia) {
+  if (!a || n <= 0 || nrhs < 0 || (nrhs > 0 && (!r || !t))) return DVR_ERR_ARG;
+  try {
+    hipStream_t s;
+    DVR_HIP(hipStreamCreate(&s));
+    {
+      DevBuf H, R, T;
+      H.ensure(sizeof(double) * (size_t)n * n);
+      DVR_HIP(hipMemcpyAsync(H.p, a, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, s));
+      LdlFactor f;
+      f.ensure(n);
+      const int nu = f.factor(H.as<double>(), sigma, s);
+      if (inertia) *inertia = nu;
+      if (nrhs > 0) {
+        R.ensure(sizeof(double) * (size_t)n * nrhs);
+        T.ensure(sizeof(double) * (size_t)n * nrhs);
+        DVR_HIP(hipMemcpyAsync(R.p, r, sizeof(double) * (size_t)n * nrhs, hipMemcpyHostToDevice,
+                               s));
+        f.solve(R.as<double>(), T.as<double>(), nrhs, s);
+        DVR_HIP(hipMemcpyAsync(t, T.p, sizeof(double) * (size_t)n * nrhs, hipMemcpyDeviceToHost,
+                               s));
+      }
+      DVR_HIP(hipStreamSynchronize(s));
+    }
+    DVR_HIP(hipStreamDestroy(s));
+  } catch (const std::exception& e) {
+    fprintf(stderr, "dvr_ldl_solve: %s\n", e.what());
+    return DVR_ERR_HIP;
+  }
   return DVR_OK;
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -252,5 +253,299 @@ size_t syev_lds_scratch_doubles(int n, int nsolve);
 void launch_build_h1(hipStream_t s, const double* vgrid, const double* xl,
                      const double* wsum, const double* wterm, int n, int nsolve,
                      const int* src, double* H);
+
+// ---------------------------------------------------------------------------
+// Device buffers and the batched dense eigensolver driver
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  void ensure(size_t b) {
+    if (b <= bytes && p) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    if (b == 0) b = 8;
+    DVR_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p); }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <class T>
+T* upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.ensure(sizeof(T) * std::max<size_t>(h.size(), 1));
+  if (!h.empty())
+    DVR_HIP(hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(),
+                           hipMemcpyHostToDevice, s));
+  return d.as<T>();
+}
+
+// Working storage of one batched eigensolve.
+struct EigWork {
+  DevBuf n, aoff, voff, d, e, e2, tau, vb0, vb1, wb, pb, gl, gu, tn, piv;
+  DevBuf il, kofs, eoff, eval;
+  DevBuf zoff, m, clus, scr, tbuf, toff;
+  EigBatch b{};
+  std::vector<int> hn;
+  std::vector<long> hvoff;
+  std::vector<int> hcnt;
+  std::vector<long> heoff;
+  std::vector<double> htn;
+  std::vector<double> hval;  // concatenated eigenvalues (heoff offsets)
+
+  bool wg = false;  // single-work-group solver (values and vectors split)
+  // eigenvalues closer than cluster_rel * ||T|| share one inverse-iteration
+  // thread and are Gram-Schmidt orthogonalised against each other
+  double cluster_rel = 1.0e-6;
+  DevBuf soffb;
+
+  // Reduce A_p (at A + aoff[p], n_p x n_p) to tridiagonal form.  With the
+  // single-work-group solver this only sets the batch up; values() does the
+  // work.
+  void tridiag(const std::vector<int>& n_, double* A,
+               const std::vector<long>& aoff_, hipStream_t s, bool use_wg = false) {
+    wg = use_wg;
+    hn = n_;
+    const int np = (int)hn.size();
+    hvoff.assign(np, 0);
+    long tot = 0;
+    int nmax = 0;
+    for (int p = 0; p < np; ++p) {
+      hvoff[p] = tot;
+      tot += hn[p];
+      nmax = std::max(nmax, hn[p]);
+    }
+    const size_t vb = sizeof(double) * (tot + 1);
+    for (DevBuf* x : {&d, &e, &e2, &tau, &vb0, &vb1, &wb, &pb}) x->ensure(vb);
+    for (DevBuf* x : {&gl, &gu, &tn, &piv}) x->ensure(sizeof(double) * np);
+    b.nprob = np;
+    b.nmax = nmax;
+    b.n = upload(n, hn, s);
+    b.aoff = upload(aoff, aoff_, s);
+    b.voff = upload(voff, hvoff, s);
+    b.A = A;
+    b.d = d.as<double>();
+    b.e = e.as<double>();
+    b.e2 = e2.as<double>();
+    b.tau = tau.as<double>();
+    b.vbuf0 = vb0.as<double>();
+    b.vbuf1 = vb1.as<double>();
+    b.wbuf = wb.as<double>();
+    b.pbuf = pb.as<double>();
+    b.gl = gl.as<double>();
+    b.gu = gu.as<double>();
+    b.tnrm = tn.as<double>();
+    b.pivmin = piv.as<double>();
+    if (!wg) {
+      launch_tridiag(s, b, hn.data());
+      launch_tridiag_bounds(s, b);
+    }
+  }
+
+  // Lowest cnt[p] eigenvalues of every problem -> hval (synchronises).
+  void values(const std::vector<int>& cnt, hipStream_t s) {
+    const int np = b.nprob;
+    hcnt = cnt;
+    std::vector<int> hil(np, 0), hk(np + 1, 0);
+    heoff.assign(np, 0);
+    long tot = 0;
+    for (int p = 0; p < np; ++p) {
+      hk[p + 1] = hk[p] + cnt[p];
+      heoff[p] = tot;
+      tot += cnt[p];
+    }
+    eval.ensure(sizeof(double) * (tot + 1));
+    if (wg) {
+      launch_syev_wg(s, b, upload(kofs, cnt, s), upload(eoff, heoff, s),
+                     eval.as<double>(), nullptr, nullptr, nullptr, nullptr, nullptr, 1);
+    } else {
+      launch_bisect(s, b, upload(il, hil, s), upload(kofs, hk, s), hk[np],
+                    eval.as<double>(), upload(eoff, heoff, s));
+    }
+    hval.assign(tot, 0.0);
+    htn.assign(np, 0.0);
+    if (tot)
+      DVR_HIP(hipMemcpyAsync(hval.data(), eval.p, sizeof(double) * tot,
+                             hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipMemcpyAsync(htn.data(), tn.p, sizeof(double) * np,
+                           hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipStreamSynchronize(s));
+  }
+
+  // Number of vectors to compute so that the last wanted one does not split
+  // a cluster of close eigenvalues.
+  int extend(int p, int want) const {
+    if (want <= 0) return 0;
+    const double ctol = cluster_rel * std::max(htn[p], 1e-300);
+    const double* v = hval.data() + heoff[p];
+    int k = want;
+    while (k < hcnt[p] && v[k] - v[k - 1] < ctol) ++k;
+    return k;
+  }
+
+  // Eigenvectors 0..m[p]-1 of every problem into Z (n_p x m_p at zoff[p]).
+  void vectors(const std::vector<int>& mv, double* Z,
+               const std::vector<long>& zoff_, hipStream_t s) {
+    const int np = b.nprob;
+    if (wg) {
+      std::vector<long> so(np, 0);
+      long st = 0;
+      for (int p = 0; p < np; ++p) {
+        so[p] = st;
+        st += (long)syev_wg_scratch_doubles(hn[p], mv[p]);
+      }
+      scr.ensure(sizeof(double) * (st + 1));
+      launch_syev_wg(s, b, kofs.as<int>(), eoff.as<long>(), eval.as<double>(),
+                     upload(m, mv, s), upload(zoff, zoff_, s), Z, scr.as<double>(),
+                     upload(soffb, so, s), 2);
+      DVR_HIP(hipStreamSynchronize(s));
+      return;
+    }
+    std::vector<Cluster> cl;
+    for (int p = 0; p < np; ++p) {
+      const int mp = mv[p];
+      if (mp <= 0) continue;
+      const double ctol = cluster_rel * std::max(htn[p], 1e-300);
+      const double* v = hval.data() + heoff[p];
+      int kb = 0;
+      for (int k = 1; k <= mp; ++k) {
+        if (k == mp || v[k] - v[k - 1] >= ctol) {
+          cl.push_back(Cluster{p, kb, k});
+          kb = k;
+        }
+      }
+    }
+    const int ncl = (int)cl.size();
+    if (ncl == 0) return;
+    const int T = ncl;
+    scr.ensure(sizeof(double) * stein_scratch_doubles(b.nmax, T));
+    DevBuf dcl;
+    dcl.ensure(sizeof(Cluster) * ncl);
+    DVR_HIP(hipMemcpyAsync(dcl.p, cl.data(), sizeof(Cluster) * ncl,
+                           hipMemcpyHostToDevice, s));
+    const long* dz = upload(zoff, zoff_, s);
+    launch_stein(s, b, dcl.as<Cluster>(), ncl, eval.as<double>(),
+                 eoff.as<long>(), Z, dz, scr.as<double>(), T);
+    // WY factors
+    const int NBk = backtransform_nb();
+    std::vector<long> ht(np, 0);
+    long tt = 0;
+    for (int p = 0; p < np; ++p) {
+      ht[p] = tt;
+      const int nr = hn[p] - 2;
+      tt += nr > 0 ? (long)((nr + NBk - 1) / NBk) * NBk * NBk : 0;
+    }
+    tbuf.ensure(sizeof(double) * (tt + 1));
+    launch_backtransform(s, b, Z, dz, upload(m, mv, s), hn.data(), mv.data(),
+                         tbuf.as<double>(), upload(toff, ht, s), tt);
+    DVR_HIP(hipStreamSynchronize(s));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// General FP64 GEMM on MFMA (k_gemm.hip)
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  int M, N, K;
+  const double* A;
+  long lda;
+  const double* B;
+  long ldb;
+  double* C;
+  long ldc;
+  double alpha, beta;
+  int ksplit, kchunk, lower;
+  double* work;
+};
+// C = alpha op(A) op(B) + beta C (column-major).  ksplit > 1 splits K over
+// work-groups; work then holds gemm_work_doubles(M, N, ksplit) doubles.
+// lower: only 64x64 tiles on or below the diagonal of C are formed.
+void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                 double alpha, const double* A, long lda, const double* B, long ldb,
+                 double beta, double* C, long ldc, int ksplit, double* work, bool lower);
+size_t gemm_work_doubles(int M, int N, int ksplit);
+// split of K that gives about a thousand work-groups for small outputs
+int gemm_auto_ksplit(int M, int N, int K);
+
+// ---------------------------------------------------------------------------
+// LDL^T of H - sigma I without pivoting (k_ldl.hip)
+// ---------------------------------------------------------------------------
+struct LdlFactor {
+  static constexpr int SB = 512;   // super-block of the triangular solves
+  int n = 0, nsb = 0;
+  DevBuf A, d, linv64, linvsb, w, work;
+  std::vector<double> hd;
+  void ensure(int n);
+  // factors H - sigma I (H: n x n full square, ld n); returns the inertia
+  // count (number of negative pivots = eigenvalues of H below sigma)
+  int factor(const double* H, double sigma, hipStream_t s);
+  // T <- (H - sigma I)^-1 R for nrhs columns; R is destroyed
+  void solve(double* R, double* T, int nrhs, hipStream_t s);
+};
+
+// ---------------------------------------------------------------------------
+// Block-Davidson final solve (dvr_davidson.hip, kernels in k_dav.hip)
+// ---------------------------------------------------------------------------
+void launch_gather_cols(hipStream_t s, const double* src, long lds, const int* idx, int ncol,
+                        double* dst, long ldd, int nrows);
+void launch_resid_norms(hipStream_t s, const double* X, const double* AX, long ld,
+                        const double* theta, int m, int nrows, double* norms);
+void launch_resid_gather(hipStream_t s, const double* X, const double* AX, long ld,
+                         const double* theta, const int* idx, int ncol, double* R, long ldr,
+                         int nrows);
+void launch_symmetrize(hipStream_t s, double* T, int m);
+void launch_gram_prep(hipStream_t s, double* G, int q, double eps, double* sc);
+void launch_form_m(hipStream_t s, const double* Linv, int q, const double* r, const double* sc,
+                   double* M);
+// LDL^T of one nb x nb block (nb <= 64) in place: D to d, L^-1 to Linv (ld nb)
+void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, double* Linv);
+void launch_scale_cols(hipStream_t s, double* U, long ld, int m, int ncol, const double* scal);
+
+struct DavParams {
+  int nev = 0;          // wanted levels
+  int block = 64;       // correction block
+  int slice = 128;      // levels locked per certified slice
+  int mmax = 320;       // largest search space
+  double tol = 1e-7;    // residual norm (Eh) for locking
+  int max_iter = 2000;
+};
+struct DavStats {
+  int iterations = 0, matvecs = 0, solves = 0, factorizations = 0, slices = 0;
+  int certified = 0, missing = 0;
+};
+class DavidsonSolver {
+ public:
+  // 0: certified; 2: stalled; 3: a level below the last shift was missed
+  int solve(const double* H, int n, const DavParams& P, hipStream_t s);
+  // the lowest nvec eigenvectors (ascending) into Z (n x nvec)
+  void vectors(int nvec, double* Z, hipStream_t s);
+  std::vector<double> eval;
+  DavStats stats;
+  double sigma = 0.0;
+  int tune_mv = 0;      // split-K of the H V product (0: heuristic)
+
+ private:
+  void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,
+            const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
+            bool big = false);
+  void small_eig(int m, double* S, double* U, std::vector<double>& w, hipStream_t s);
+  int expand(const double* H, int q0, hipStream_t s);
+  int cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s);
+  int N = 0, m = 0, nlocked = 0;
+  DevBuf V, AV, X, AX, XL, Rb, Tb, Tm, Y, Y2, G, G2, U, C1, thd, nrmd, idxd, scal, work;
+  DevBuf gd, gsc, gr, gm;
+  EigWork rr;
+  LdlFactor ldl;
+  std::vector<double> thL, gw;
+  std::vector<int> perm;
+};
 
 }  // namespace dvr

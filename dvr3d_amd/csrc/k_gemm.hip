@@ -1,0 +1,183 @@
+// General FP64 matrix product on MFMA, column-major:
+//
+//   C = alpha * op(A) * op(B) + beta * C,   op(X) = X or X^T
+//
+// The workhorse of the block-Davidson final solve (H V products, projections
+// V^T W, Ritz-vector formation V Y, Gram-Schmidt) and of the blocked LDL^T
+// factorisation (trailing update A22 -= L21 W21^T, lower tiles only).
+//
+// Work-group = 64x64 tile of C, 4 wavefronts in 2x2, each a 32x32 sub-tile
+// of 2x2 v_mfma_f64_16x16x4f64 accumulators; operands staged through LDS in
+// k-slices of 16 with the next slice prefetched into registers while the
+// current one is multiplied.  Lane layout of v_mfma_f64_16x16x4f64 (checked
+// on gfx950): A(i,k) from lane i + 16k, B(k,j) from lane j + 16k, result
+// register r of lane l is D(4r + l/16, l%16); the kernel feeds the B slice as
+// the first operand so that D is the transposed sub-tile.
+//
+// K can be split over blockIdx.z: each slice then writes its partial tile to
+// a workspace and a second kernel sums the slices (deterministic, no atomics).
+#include "dvr_internal.h"
+
+namespace dvr {
+
+namespace {
+constexpr int TM = 64;
+constexpr int KT = 16;
+constexpr int LDT = TM + 1;
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <bool TA>
+__device__ inline double load_a(const GemmArgs& g, int i, int k) {
+  if (i >= g.M || k >= g.K) return 0.0;
+  return TA ? g.A[k + (long)i * g.lda] : g.A[i + (long)k * g.lda];
+}
+template <bool TB>
+__device__ inline double load_b(const GemmArgs& g, int k, int j) {
+  if (j >= g.N || k >= g.K) return 0.0;
+  return TB ? g.B[j + (long)k * g.ldb] : g.B[k + (long)j * g.ldb];
+}
+}  // namespace
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ double As[KT * LDT];
+  __shared__ double Bs[KT * LDT];
+  const int ti = blockIdx.x, tj = blockIdx.y, z = blockIdx.z;
+  if (g.lower && ti < tj) return;
+  const int row0 = ti * TM, col0 = tj * TM;
+  const int kbeg = z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wy = wave >> 1, wx = wave & 1;
+  const int l16 = lane & 15, lk = lane >> 4;
+
+  // element (kk, cc) of the 16x64 slice handled by this thread in round q:
+  // the fast index follows the contiguous direction of the operand in memory
+  auto a_pos = [&](int q, int& kk, int& cc) {
+    const int idx = tid + 256 * q;
+    if (TA) { kk = idx & 15; cc = idx >> 4; } else { cc = idx & 63; kk = idx >> 6; }
+  };
+  auto b_pos = [&](int q, int& kk, int& cc) {
+    const int idx = tid + 256 * q;
+    if (TB) { cc = idx & 63; kk = idx >> 6; } else { kk = idx & 15; cc = idx >> 4; }
+  };
+
+  v4d acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  double ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int kk, cc;
+      a_pos(q, kk, cc);
+      ra[q] = (k0 + kk < kend) ? load_a<TA>(g, row0 + cc, k0 + kk) : 0.0;
+      b_pos(q, kk, cc);
+      rb[q] = (k0 + kk < kend) ? load_b<TB>(g, k0 + kk, col0 + cc) : 0.0;
+    }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += KT) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int kk, cc;
+      a_pos(q, kk, cc);
+      As[kk * LDT + cc] = ra[q];
+      b_pos(q, kk, cc);
+      Bs[kk * LDT + cc] = rb[q];
+    }
+    __syncthreads();
+    if (k0 + KT < kend) fetch(k0 + KT);
+#pragma unroll
+    for (int k4 = 0; k4 < KT; k4 += 4) {
+      const int kr = (k4 + lk) * LDT;
+      const double a0 = As[kr + wy * 32 + l16];
+      const double a1 = As[kr + wy * 32 + 16 + l16];
+      const double b0 = Bs[kr + wx * 32 + l16];
+      const double b1 = Bs[kr + wx * 32 + 16 + l16];
+      // B operand first: the accumulator holds the transposed sub-tile, so
+      // that lanes 0..15 of a result register own 16 consecutive rows of C
+      // (coalesced stores in column-major C)
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // register r of lane l holds C(i, j) with i = l%16, j = 4r + l/16 inside
+  // the 16x16 block (bi, bj)
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + wy * 32 + bi * 16 + l16;
+        const int j = col0 + wx * 32 + bj * 16 + 4 * r + lk;
+        if (i >= g.M || j >= g.N) continue;
+        if (g.ksplit > 1) {
+          g.work[(long)z * g.M * g.N + i + (long)g.M * j] = acc[bi][bj][r];
+        } else {
+          double* c = g.C + i + (long)g.ldc * j;
+          *c = g.beta == 0.0 ? g.alpha * acc[bi][bj][r] : g.alpha * acc[bi][bj][r] + g.beta * *c;
+        }
+      }
+}
+
+__global__ void __launch_bounds__(256) k_gemm_reduce(GemmArgs g) {
+  const long n = (long)g.M * g.N;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int i = (int)(t % g.M), j = (int)(t / g.M);
+    if (g.lower && (i / TM) < (j / TM)) continue;
+    double s = 0.0;
+    for (int z = 0; z < g.ksplit; ++z) s += g.work[(long)z * n + t];
+    double* c = g.C + i + (long)g.ldc * j;
+    *c = g.beta == 0.0 ? g.alpha * s : g.alpha * s + g.beta * *c;
+  }
+}
+
+int gemm_auto_ksplit(int M, int N, int K) {
+  const long tiles = (long)((M + TM - 1) / TM) * ((N + TM - 1) / TM);
+  if (tiles >= 768 || K < 512) return 1;
+  long ks = (1024 + tiles - 1) / tiles;
+  ks = std::min<long>(ks, std::max(1, K / 256));
+  return (int)std::max<long>(ks, 1);
+}
+
+size_t gemm_work_doubles(int M, int N, int ksplit) {
+  return ksplit > 1 ? (size_t)M * N * ksplit : 0;
+}
+
+void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                 double alpha, const double* A, long lda, const double* B, long ldb,
+                 double beta, double* C, long ldc, int ksplit, double* work, bool lower) {
+  if (M <= 0 || N <= 0) return;
+  if (K <= 0) ksplit = 1;
+  if (ksplit < 1) ksplit = 1;
+  GemmArgs g;
+  g.M = M; g.N = N; g.K = K; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha = alpha; g.beta = beta; g.lower = lower ? 1 : 0;
+  // slices of whole KT steps
+  int kchunk = (K + ksplit - 1) / ksplit;
+  kchunk = ((kchunk + KT - 1) / KT) * KT;
+  if (kchunk <= 0) kchunk = KT;
+  ksplit = K > 0 ? (K + kchunk - 1) / kchunk : 1;
+  if (ksplit > 1 && !work) throw HipError("gemm: split-K needs a workspace");
+  g.ksplit = ksplit; g.kchunk = kchunk; g.work = work;
+  const dim3 grid((M + TM - 1) / TM, (N + TM - 1) / TM, ksplit);
+  const double flops = 2.0 * M * (double)N * K * (lower ? 0.5 : 1.0);
+  const double bytes = 8.0 * ((double)M * K + (double)K * N + (beta != 0.0 ? 2.0 : 1.0) * M * N);
+  if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true>), grid, dim3(256), 0, s, g);
+  else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false>), grid, dim3(256), 0, s, g);
+  else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true>), grid, dim3(256), 0, s, g);
+  else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false>), grid, dim3(256), 0, s, g);
+  if (ksplit > 1) {
+    const long n = (long)M * N;
+    const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
+    DVR_LAUNCH("gemm_reduce", 8.0 * n * (ksplit + 1), 0.0, k_gemm_reduce, dim3(blocks), dim3(256),
+               0, s, g);
+  }
+  DVR_LAUNCH_CHECK();
+}
+
+}  // namespace dvr

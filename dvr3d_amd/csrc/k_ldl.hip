@@ -1,0 +1,244 @@
+// Blocked LDL^T factorisation without pivoting, lower storage, FP64:
+//
+//   A - sigma I = L D L^T,   L unit lower triangular, D diagonal.
+//
+// Used by the block-Davidson final solve (the replacement of the dense diag3d
+// call, source/dvr3drjz_segmented.f90:2699 -> 2638-2659) for two things:
+//  * Sylvester's law of inertia: the number of negative entries of D is the
+//    number of eigenvalues of A below sigma, which certifies that no level
+//    below sigma has been missed;
+//  * the shift-and-invert preconditioner (A - sigma I)^-1 = L^-T D^-1 L^-1.
+//
+// Right-looking, panels of NB = 64 columns:
+//   k_ldl_diag   factors the NB x NB diagonal block in LDS (one work-group)
+//                and writes D, the strict lower part of L11 and L11^-1;
+//   W21 = A21 L11^-T                      (k_gemm, NT)
+//   L21 = W21 D1^-1                       (k_ldl_scale_cols, in place)
+//   A22 -= L21 W21^T                      (k_gemm, NT, lower tiles only).
+// The triangular solves run over super-blocks of SB = 512 rows whose inverse
+// diagonal blocks are formed once per factorisation (forward substitution on
+// the identity), so that one application costs a few GEMMs per super-block.
+#include "dvr_internal.h"
+
+namespace dvr {
+
+namespace {
+constexpr int NB = 64;
+constexpr int LD = NB + 1;
+}  // namespace
+
+// One work-group of 256 threads.  Block [k0, k0+nb) of A (lda) in place:
+// strict lower part <- L11, d[k0 + j] <- D_j, Linv (nb x nb, ld nb) <- L11^-1.
+__global__ void __launch_bounds__(256)
+k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb, double* __restrict__ d,
+           double* __restrict__ Linv) {
+  __shared__ double S[NB * LD];   // lower: L and D; strict upper: L^-1 transposed
+  __shared__ double red[4 * NB];
+  const int tid = threadIdx.x;
+  const int r = tid & 63, cg = tid >> 6;   // row, column group (4 groups)
+  for (int c = cg; c < nb; c += 4)
+    if (r < nb) S[r + c * LD] = (r >= c) ? A[(k0 + r) + (long)(k0 + c) * lda] : 0.0;
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const double rj = 1.0 / S[j + j * LD];
+    // trailing update of the lower part: S(r,c) -= S(r,j) S(c,j) / dj
+    if (r < nb)
+      for (int c = j + 1 + cg; c <= r; c += 4) S[r + c * LD] -= S[r + j * LD] * (S[c + j * LD] * rj);
+    __syncthreads();
+    if (tid > j && tid < nb) S[tid + j * LD] *= rj;
+    __syncthreads();
+  }
+  // L^-1 row by row: X(i,c) = -(L(i,c) + sum_{c<k<i} L(i,k) X(k,c)), the sum
+  // split over the four wavefronts; X(i,c), i > c, is kept at S[c + i*LD]
+  for (int i = 1; i < nb; ++i) {
+    const int c = r;
+    if (c < i) {
+      double s = 0.0;
+      for (int k = c + 1 + cg; k < i; k += 4) s += S[i + k * LD] * S[c + k * LD];
+      red[cg * NB + c] = s;
+    }
+    __syncthreads();
+    if (cg == 0 && c < i)
+      S[c + i * LD] = -(S[i + c * LD] + ((red[c] + red[NB + c]) + (red[2 * NB + c] + red[3 * NB + c])));
+    __syncthreads();
+  }
+  for (int c = cg; c < nb; c += 4)
+    if (r < nb) {
+      if (r > c) A[(k0 + r) + (long)(k0 + c) * lda] = S[r + c * LD];
+      Linv[r + (long)c * nb] = r == c ? 1.0 : (r > c ? S[c + r * LD] : 0.0);
+    }
+  if (tid < nb) d[k0 + tid] = S[tid + tid * LD];
+}
+
+// L21 (m x nb at A, lda) = W (m x nb, ldw) / d[j]
+__global__ void __launch_bounds__(256)
+k_ldl_scale_cols(double* __restrict__ A, long lda, const double* __restrict__ W, long ldw,
+                 int m, int nb, const double* __restrict__ d) {
+  const long n = (long)m * nb;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % m), c = (int)(t / m);
+    A[r + (long)c * lda] = W[r + (long)c * ldw] / d[c];
+  }
+}
+
+// Y (m x nrhs, ldy) <- Y / d (row scaling)
+__global__ void __launch_bounds__(256)
+k_ldl_scale_rows(double* __restrict__ Y, long ldy, int m, int nrhs, const double* __restrict__ d) {
+  const long n = (long)m * nrhs;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % m), c = (int)(t / m);
+    Y[r + (long)c * ldy] /= d[r];
+  }
+}
+
+// A (A + s I) with the diagonal shifted, full square copy from H
+__global__ void __launch_bounds__(256)
+k_shift_copy(const double* __restrict__ H, double* __restrict__ A, long n, double sigma) {
+  const long nn = n * n;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < nn; t += (long)gridDim.x * 256) {
+    const long r = t % n, c = t / n;
+    A[t] = (r == c) ? H[t] - sigma : H[t];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_copy2d(const double* __restrict__ src, long lds, double* __restrict__ dst, long ldd, int m,
+         int ncol) {
+  const long n = (long)m * ncol;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % m), c = (int)(t / m);
+    dst[r + (long)c * ldd] = src[r + (long)c * lds];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_identity(double* __restrict__ X, long ld, int n) {
+  const long nn = (long)n * n;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < nn; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % n), c = (int)(t / n);
+    X[r + (long)c * ld] = (r == c) ? 1.0 : 0.0;
+  }
+}
+
+static int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
+
+void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, double* Linv) {
+  if (nb <= 0 || nb > NB) throw HipError("ldl_block: size out of range");
+  DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1), dim3(256),
+             0, s, A, lda, 0, nb, d, Linv);
+}
+
+void LdlFactor::ensure(int n_) {
+  n = n_;
+  nsb = (n + SB - 1) / SB;
+  A.ensure(sizeof(double) * (size_t)n * n);
+  d.ensure(sizeof(double) * (size_t)n);
+  linv64.ensure(sizeof(double) * (size_t)((n + NB - 1) / NB) * NB * NB);
+  linvsb.ensure(sizeof(double) * (size_t)nsb * SB * SB);
+  w.ensure(sizeof(double) * (size_t)n * NB);
+  hd.resize(n);
+}
+
+int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
+  double* a = A.as<double>();
+  const long lda = n;
+  DVR_LAUNCH("ldl_shift", 16.0 * n * (double)n, 0.0, k_shift_copy, dim3(grid_for((long)n * n)),
+             dim3(256), 0, s, H, a, (long)n, sigma);
+  double* dd = d.as<double>();
+  double* l64 = linv64.as<double>();
+  double* W = w.as<double>();
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = std::min(NB, n - k0);
+    double* li = l64 + (size_t)(k0 / NB) * NB * NB;
+    DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
+               dim3(256), 0, s, a, lda, k0, nb, dd, li);
+    const int m = n - k0 - nb;
+    if (m <= 0) break;
+    double* A21 = a + (k0 + nb) + (long)k0 * lda;
+    // W21 = A21 L11^-T
+    launch_gemm(s, "ldl_panel", false, true, m, nb, nb, 1.0, A21, lda, li, nb, 0.0, W, m, 1,
+                nullptr, false);
+    DVR_LAUNCH("ldl_scale", 24.0 * m * nb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)m * nb)),
+               dim3(256), 0, s, A21, lda, W, (long)m, m, nb, dd + k0);
+    // A22 -= L21 W21^T, lower tiles
+    double* A22 = a + (k0 + nb) + (long)(k0 + nb) * lda;
+    launch_gemm(s, "ldl_update", false, true, m, m, nb, -1.0, A21, lda, W, m, 1.0, A22, lda, 1,
+                nullptr, true);
+  }
+  // inverse of every SB x SB diagonal super-block (unit lower), from the
+  // NB-level factors: X = L_sb^-1 by forward substitution on the identity
+  double* lsb = linvsb.as<double>();
+  for (int b = 0; b < nsb; ++b) {
+    const int r0 = b * SB, ns = std::min(SB, n - r0);
+    double* X = lsb + (size_t)b * SB * SB;   // ns x ns, ld SB
+    DVR_LAUNCH("ldl_ident", 8.0 * ns * ns, 0.0, k_identity, dim3(grid_for((long)ns * ns)),
+               dim3(256), 0, s, X, (long)SB, ns);
+    for (int j0 = 0; j0 < ns; j0 += NB) {
+      const int nb = std::min(NB, ns - j0);
+      const double* li = l64 + (size_t)((r0 + j0) / NB) * NB * NB;
+      // rows j0..j0+nb of X <- Linv64 * rows j0..j0+nb (only columns <= j0+nb
+      // are non-zero; columns 0..j0+nb are processed)
+      const int ncol = j0 + nb;
+      launch_gemm(s, "ldl_inv", false, false, nb, ncol, nb, 1.0, li, nb, X + j0, SB, 0.0, W, nb, 1,
+                  nullptr, false);
+      DVR_LAUNCH("ldl_copy", 16.0 * nb * ncol, 0.0, k_copy2d, dim3(grid_for((long)nb * ncol)),
+                 dim3(256), 0, s, W, (long)nb, X + j0, (long)SB, nb, ncol);
+      const int mr = ns - j0 - nb;
+      if (mr > 0) {
+        const double* L = a + (r0 + j0 + nb) + (long)(r0 + j0) * lda;
+        launch_gemm(s, "ldl_inv", false, false, mr, ncol, nb, -1.0, L, lda, X + j0, SB, 1.0,
+                    X + j0 + nb, SB, 1, nullptr, false);
+      }
+    }
+  }
+  DVR_HIP(hipMemcpyAsync(hd.data(), dd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  DVR_HIP(hipStreamSynchronize(s));
+  int neg = 0;
+  for (int i = 0; i < n; ++i) neg += hd[i] < 0.0;
+  return neg;
+}
+
+// T <- (L D L^T)^-1 R for nrhs columns (R, T: n x nrhs, ld n).  R is used as
+// workspace and destroyed.
+// GEMM with K split for short, wide products
+static void gemm_ks(hipStream_t s, DevBuf& work, const char* name, bool ta, bool tb, int M, int N,
+                    int K, double alpha, const double* A, long lda, const double* B, long ldb,
+                    double beta, double* C, long ldc) {
+  const int ks = gemm_auto_ksplit(M, N, K);
+  if (ks > 1) work.ensure(sizeof(double) * gemm_work_doubles(M, N, ks));
+  launch_gemm(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ks,
+              ks > 1 ? work.as<double>() : nullptr, false);
+}
+
+void LdlFactor::solve(double* R, double* T, int nrhs, hipStream_t s) {
+  const double* a = A.as<double>();
+  const long lda = n;
+  const double* lsb = linvsb.as<double>();
+  // forward: Y = L^-1 R, Y stored in T
+  for (int b = 0; b < nsb; ++b) {
+    const int r0 = b * SB, ns = std::min(SB, n - r0);
+    const double* X = lsb + (size_t)b * SB * SB;
+    gemm_ks(s, work, "si_fwd_diag", false, false, ns, nrhs, ns, 1.0, X, SB, R + r0, n, 0.0,
+            T + r0, n);
+    const int m = n - r0 - ns;
+    if (m > 0)
+      gemm_ks(s, work, "si_fwd_upd", false, false, m, nrhs, ns, -1.0,
+              a + (r0 + ns) + (long)r0 * lda, lda, T + r0, n, 1.0, R + r0 + ns, n);
+  }
+  DVR_LAUNCH("si_scale", 16.0 * n * nrhs, 0.0, k_ldl_scale_rows, dim3(grid_for((long)n * nrhs)),
+             dim3(256), 0, s, T, (long)n, n, nrhs, d.as<double>());
+  // backward: Z = L^-T Y.  The result of super-block b goes to R (free now),
+  // and Y above it is updated in T.
+  for (int b = nsb - 1; b >= 0; --b) {
+    const int r0 = b * SB, ns = std::min(SB, n - r0);
+    const double* X = lsb + (size_t)b * SB * SB;
+    gemm_ks(s, work, "si_bwd_diag", true, false, ns, nrhs, ns, 1.0, X, SB, T + r0, n, 0.0, R + r0,
+            n);
+    if (r0 > 0)
+      gemm_ks(s, work, "si_bwd_upd", true, false, r0, nrhs, ns, -1.0, a + r0, lda, R + r0, n, 1.0,
+              T, n);
+  }
+  DVR_HIP(hipMemcpyAsync(T, R, sizeof(double) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace dvr

@@ -20,6 +20,7 @@
 //   4. Back-transformation Z <- Q Z in blocks of NB reflectors in WY form,
 //      Q_blk = I - V T V^T, one work-group per (problem, 8-column tile).
 #include "dvr_internal.h"
+#include "inviter.h"
 #include <cfloat>
 
 namespace dvr {
@@ -402,13 +403,6 @@ void launch_bisect(hipStream_t s, const EigBatch& b, const int* il,
 // ---------------------------------------------------------------------------
 // 3. inverse iteration
 // ---------------------------------------------------------------------------
-__device__ inline double unit_hash(unsigned long long x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  x = x ^ (x >> 31);
-  return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-}
 
 size_t stein_scratch_doubles(int nmax, int nthreads) {
   return (size_t)6 * (size_t)nmax * (size_t)nthreads;
@@ -423,101 +417,10 @@ k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
   if (t >= ncl) return;
   const Cluster C = cl[t];
   const int p = C.p;
-  const int n = b.n[p];
   const long vo = b.voff[p];
-  const double* d = b.d + vo;
-  const double* e = b.e + vo;
-  const double* ev = eval + eoff[p];
-  double* Zp = Z + zoff[p];
   const long nm = b.nmax;
-  double* DL = scratch + t;
-  double* DD = DL + nm * T;
-  double* DU = DD + nm * T;
-  double* DU2 = DU + nm * T;
-  double* PV = DU2 + nm * T;
-  double* X = PV + nm * T;
-#define AT(arr, i) arr[(long)(i) * T]
-  const double tn = b.tnrm[p] > 0.0 ? b.tnrm[p] : 1.0;
-  const double pertol = 10.0 * kEps * tn;
-  const double tiny = kEps * tn;
-  double lprev = 0.0;
-  for (int k = C.kb; k < C.ke; ++k) {
-    double lam = ev[k];
-    if (k > C.kb && lam - lprev < pertol) lam = lprev + pertol;
-    lprev = lam;
-    // T - lam I = P L U (dgttrf-style partial pivoting)
-    for (int i = 0; i < n; ++i) {
-      AT(DD, i) = d[i] - lam;
-      if (i < n - 1) {
-        AT(DL, i) = e[i];
-        AT(DU, i) = e[i];
-      }
-      AT(DU2, i) = 0.0;
-      AT(PV, i) = 0.0;
-    }
-    for (int i = 0; i < n - 1; ++i) {
-      const double di = AT(DD, i), li = AT(DL, i);
-      if (fabs(di) >= fabs(li)) {
-        if (di != 0.0) {
-          const double f = li / di;
-          AT(DL, i) = f;
-          AT(DD, i + 1) -= f * AT(DU, i);
-        } else {
-          AT(DD, i) = tiny;
-          AT(DL, i) = 0.0;
-        }
-      } else {
-        const double f = di / li;
-        AT(DD, i) = li;
-        AT(DL, i) = f;
-        const double tmp = AT(DU, i);
-        AT(DU, i) = AT(DD, i + 1);
-        AT(DD, i + 1) = tmp - f * AT(DD, i + 1);
-        if (i < n - 2) {
-          AT(DU2, i) = AT(DU, i + 1);
-          AT(DU, i + 1) = -f * AT(DU, i + 1);
-        }
-        AT(PV, i) = 1.0;
-      }
-    }
-    for (int i = 0; i < n; ++i)
-      if (AT(DD, i) == 0.0) AT(DD, i) = tiny;
-    for (int i = 0; i < n; ++i)
-      AT(X, i) = unit_hash(((unsigned long long)p << 40) ^ ((unsigned long long)k << 20) ^ (unsigned long long)i);
-    for (int it = 0; it < 3; ++it) {
-      for (int i = 0; i < n - 1; ++i) {
-        if (AT(PV, i) == 0.0) {
-          AT(X, i + 1) -= AT(DL, i) * AT(X, i);
-        } else {
-          const double tmp = AT(X, i);
-          AT(X, i) = AT(X, i + 1);
-          AT(X, i + 1) = tmp - AT(DL, i) * AT(X, i);
-        }
-      }
-      AT(X, n - 1) /= AT(DD, n - 1);
-      if (n > 1) AT(X, n - 2) = (AT(X, n - 2) - AT(DU, n - 2) * AT(X, n - 1)) / AT(DD, n - 2);
-      for (int i = n - 3; i >= 0; --i)
-        AT(X, i) = (AT(X, i) - AT(DU, i) * AT(X, i + 1) - AT(DU2, i) * AT(X, i + 2)) / AT(DD, i);
-      // scale to avoid growth across iterations, then orthogonalise
-      double mx = 0.0;
-      for (int i = 0; i < n; ++i) mx = fmax(mx, fabs(AT(X, i)));
-      const double sc = mx > 0.0 ? 1.0 / mx : 1.0;
-      for (int i = 0; i < n; ++i) AT(X, i) *= sc;
-      for (int q = C.kb; q < k; ++q) {
-        const double* zq = Zp + (long)q * n;
-        double dot = 0.0;
-        for (int i = 0; i < n; ++i) dot += AT(X, i) * zq[i];
-        for (int i = 0; i < n; ++i) AT(X, i) -= dot * zq[i];
-      }
-      double nrm = 0.0;
-      for (int i = 0; i < n; ++i) nrm += AT(X, i) * AT(X, i);
-      nrm = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
-      for (int i = 0; i < n; ++i) AT(X, i) *= nrm;
-    }
-    double* zk = Zp + (long)k * n;
-    for (int i = 0; i < n; ++i) zk[i] = AT(X, i);
-  }
-#undef AT
+  inverse_iteration_cluster<1>(b.n[p], b.d + vo, b.e + vo, eval + eoff[p], C.kb, C.ke, b.tnrm[p],
+                            scratch + t, nm * T, T, Z + zoff[p], (unsigned long long)p);
 }
 
 void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,

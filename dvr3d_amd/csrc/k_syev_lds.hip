@@ -16,6 +16,7 @@
 //    only the eigenvectors below EMAX1 (the ones cut1d keeps);
 //  * the Rayleigh-Ritz problems of the block-Davidson final solve.
 #include "dvr_internal.h"
+#include "inviter.h"
 #include <cfloat>
 
 namespace dvr {
@@ -43,13 +44,6 @@ __device__ inline double bred(double v, double* red) {
   return s;
 }
 
-__device__ inline double hmix(unsigned long long x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  x = x ^ (x >> 31);
-  return (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
-}
 }  // namespace
 
 size_t syev_lds_bytes(int n) {
@@ -274,91 +268,14 @@ k_syev_lds(const double* __restrict__ Ain, long astride, int n, int kcnt,
   if (m == 0) return;
   double* Zp = Z + (long)p * zstride;
 
-  // ---- inverse iteration, one thread per cluster (scratch per work-group)
-  for (int c = tid; c < ncl; c += NT) {
-    const int kb = cls[c], ke = cls[c + 1];
-    double* DL = scratch + (long)blockIdx.x * 6 * n * n + (long)c * 6 * n;
-    double* DD = DL + n;
-    double* DU = DD + n;
-    double* DU2 = DU + n;
-    double* PV = DU2 + n;
-    double* X = PV + n;
-    const double tnn = tn > 0.0 ? tn : 1.0;
-    const double pertol = 10.0 * kEpsL * tnn, tiny = kEpsL * tnn;
-    double lprev = 0.0;
-    for (int k = kb; k < ke; ++k) {
-      double lam = sv[k];
-      if (k > kb && lam - lprev < pertol) lam = lprev + pertol;
-      lprev = lam;
-      for (int i = 0; i < n; ++i) {
-        DD[i] = d[i] - lam;
-        if (i < n - 1) {
-          DL[i] = e[i];
-          DU[i] = e[i];
-        }
-        DU2[i] = 0.0;
-        PV[i] = 0.0;
-      }
-      for (int i = 0; i < n - 1; ++i) {
-        const double di = DD[i], li = DL[i];
-        if (fabs(di) >= fabs(li)) {
-          if (di != 0.0) {
-            const double f = li / di;
-            DL[i] = f;
-            DD[i + 1] -= f * DU[i];
-          } else {
-            DD[i] = tiny;
-            DL[i] = 0.0;
-          }
-        } else {
-          const double f = di / li;
-          DD[i] = li;
-          DL[i] = f;
-          const double t = DU[i];
-          DU[i] = DD[i + 1];
-          DD[i + 1] = t - f * DD[i + 1];
-          if (i < n - 2) {
-            DU2[i] = DU[i + 1];
-            DU[i + 1] = -f * DU[i + 1];
-          }
-          PV[i] = 1.0;
-        }
-      }
-      for (int i = 0; i < n; ++i)
-        if (DD[i] == 0.0) DD[i] = tiny;
-      for (int i = 0; i < n; ++i)
-        X[i] = hmix(((unsigned long long)p << 40) ^ ((unsigned long long)k << 20) ^ (unsigned long long)i);
-      for (int it = 0; it < 3; ++it) {
-        for (int i = 0; i < n - 1; ++i) {
-          if (PV[i] == 0.0) {
-            X[i + 1] -= DL[i] * X[i];
-          } else {
-            const double t = X[i];
-            X[i] = X[i + 1];
-            X[i + 1] = t - DL[i] * X[i];
-          }
-        }
-        X[n - 1] /= DD[n - 1];
-        if (n > 1) X[n - 2] = (X[n - 2] - DU[n - 2] * X[n - 1]) / DD[n - 2];
-        for (int i = n - 3; i >= 0; --i) X[i] = (X[i] - DU[i] * X[i + 1] - DU2[i] * X[i + 2]) / DD[i];
-        double mxv = 0.0;
-        for (int i = 0; i < n; ++i) mxv = fmax(mxv, fabs(X[i]));
-        const double sc = mxv > 0.0 ? 1.0 / mxv : 1.0;
-        for (int i = 0; i < n; ++i) X[i] *= sc;
-        for (int q = kb; q < k; ++q) {
-          const double* zq = Zp + (long)q * n;
-          double dot = 0.0;
-          for (int i = 0; i < n; ++i) dot += X[i] * zq[i];
-          for (int i = 0; i < n; ++i) X[i] -= dot * zq[i];
-        }
-        double nrm = 0.0;
-        for (int i = 0; i < n; ++i) nrm += X[i] * X[i];
-        nrm = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
-        for (int i = 0; i < n; ++i) X[i] *= nrm;
-      }
-      double* zk = Zp + (long)k * n;
-      for (int i = 0; i < n; ++i) zk[i] = X[i];
-    }
+  // ---- inverse iteration, one thread per cluster, scratch interleaved
+  // across the threads (array a, element i of thread c at (a*n + i)*n + c)
+  {
+    const int c = tid;
+    if (c < ncl)
+      inverse_iteration_cluster<1>(n, d, e, sv, cls[c], cls[c + 1], tn,
+                                scratch + (long)blockIdx.x * 6 * n * n + c, (long)n * n, n, Zp,
+                                (unsigned long long)p);
   }
   __syncthreads();
 

@@ -129,6 +129,24 @@ int dvr_get_vectors(const dvr_ctx_t* ctx, double* vec);
 /* Per-stage device times of the last dvr_run, DVR_NSTAGE doubles (ms). */
 int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
 
+/* Statistics of the last final solve (no counterpart in the reference, which
+ * only has the dense dsyev call).  Fills up to n of: [0] Davidson status
+ * (-1 not run, 0 certified, 2 stalled, 3 missed level), [1] iterations,
+ * [2] H-vector products, [3] shift-and-invert solves, [4] LDL^T
+ * factorisations, [5] slices, [6] inertia count at the last shift,
+ * [7] levels missing at the failed certification, [8] 1 when the dense
+ * solver was used as fallback. */
+int dvr_get_solver_stats(const dvr_ctx_t* ctx, int32_t* stats, int32_t n);
+
+/* Shift-and-invert kernel of the Davidson solver, exposed for testing and
+ * for callers that need eigenvalue counts (no counterpart in the reference).
+ * Factors A - sigma I = L D L^T (A: n x n column-major, full square, host
+ * memory), returns in *inertia the number of negative pivots (= eigenvalues
+ * of A below sigma by Sylvester's law) and, for nrhs > 0, solves
+ * (A - sigma I) t = r for the nrhs columns of r (n x nrhs) into t. */
+int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int32_t nrhs,
+                  double* t, int32_t* inertia);
+
 /*
  * Kernel-level timing (no counterpart in the reference, which only prints
  * the total run time, D:67/D:75-78).  With profiling on, the next dvr_run
