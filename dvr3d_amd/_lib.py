@@ -30,8 +30,11 @@ EXPORTS = (
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
     "dvr_diag", "dvr_syev_batch", "dvr_set_profiling", "dvr_get_profile",
-    "dvr_get_solver_stats", "dvr_ldl_solve",
+    "dvr_get_solver_stats", "dvr_ldl_solve", "dvr_keep_hamiltonian", "dvr_set_preconditioner",
+    "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
 )
+MLP_NPARAM = 17537
+PREC_SHIFT_INVERT, PREC_DIAGONAL, PREC_LEARNED = 0, 1, 2
 SOLVER_STATS = ("status", "iterations", "matvecs", "solves", "factorizations", "slices",
                 "certified", "missing", "dense_fallback")
 
@@ -85,6 +88,12 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
     lib.dvr_ldl_solve.argtypes = [D, C.c_int32, C.c_double, D, C.c_int32, D, I]
+    Fp = C.POINTER(C.c_float)
+    lib.dvr_keep_hamiltonian.argtypes = [P, C.c_int32]
+    lib.dvr_set_preconditioner.argtypes = [P, C.c_int32, Fp]
+    lib.dvr_mlp_train.argtypes = [P, C.c_int32, C.c_int32, C.c_uint64, C.c_float, Fp, D]
+    lib.dvr_mlp_forward.argtypes = [Fp, C.c_int64, Fp, Fp, C.c_int32]
+    lib.dvr_mlp_backward.argtypes = [Fp, C.c_int64, Fp, Fp, Fp]
     for name in EXPORTS:
         if name not in ("dvr_ctx_destroy", "dvr_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -218,6 +227,26 @@ class Context:
         self._check(self.lib.dvr_get_stage_times(self.h, _dp(out)))
         return dict(zip(STAGE_NAMES, out.tolist()))
 
+    def keep_hamiltonian(self, on=True):
+        self._check(self.lib.dvr_keep_hamiltonian(self.h, int(on)))
+
+    def set_preconditioner(self, kind, params=None):
+        p = None
+        if params is not None:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            self._keep_mlp = params
+            p = params.ctypes.data_as(C.POINTER(C.c_float))
+        self._check(self.lib.dvr_set_preconditioner(self.h, int(kind), p))
+
+    def mlp_train(self, params, steps, batch=32, seed=0, lr=2e-3):
+        """Trains params (float32, MLP_NPARAM) in place; returns the loss history."""
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        hist = np.zeros(steps)
+        self._check(self.lib.dvr_mlp_train(self.h, steps, batch, seed, lr,
+                                           params.ctypes.data_as(C.POINTER(C.c_float)),
+                                           _dp(hist)))
+        return params, hist
+
     def solver_stats(self):
         out = np.zeros(len(SOLVER_STATS), dtype=np.int32)
         self._check(self.lib.dvr_get_solver_stats(
@@ -244,6 +273,33 @@ def syev_batch(mats, nvec=None, vectors=True):
             zs.append(z[zo:zo + n[p] * nv[p]].reshape(nv[p], n[p]).T)
             zo += n[p] * nv[p]
     return (ws, zs) if vectors else ws
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(params, feats, bf16=False):
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feats, dtype=np.float32)
+    w = np.zeros(f.shape[0], dtype=np.float32)
+    rc = lib.dvr_mlp_forward(_fp(p), f.shape[0], _fp(f), _fp(w), int(bf16))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_mlp_forward failed")
+    return w
+
+
+def mlp_backward(params, feats, gw):
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    f = np.ascontiguousarray(feats, dtype=np.float32)
+    g = np.ascontiguousarray(gw, dtype=np.float32)
+    out = np.zeros(MLP_NPARAM, dtype=np.float32)
+    rc = lib.dvr_mlp_backward(_fp(p), f.shape[0], _fp(f), _fp(g), _fp(out))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_mlp_backward failed")
+    return out
 
 
 def ldl_solve(a: np.ndarray, sigma: float, r: np.ndarray | None = None):

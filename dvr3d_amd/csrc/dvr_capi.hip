@@ -45,6 +45,11 @@ struct dvr_ctx {
   EigWork w2, w3;
   DavidsonSolver dav;
   int dav_status = -1, tune_mv = 0;
+  bool keep_h = false, h_kept = false;
+  DevBuf H3keep, st3, mlpP;
+  int prec_kind = 0;
+  bool mlp_set = false;
+  MlpWork mw;
   bool has_wterm = false;
   std::vector<int> h_iv1;    // [c*npntb + b]
   std::vector<int> h_nham2, h_iv2;
@@ -328,6 +333,21 @@ void run_pipeline(dvr_ctx_t* C) {
       col += iv2[c];
     }
   }
+  {
+    // static features of the learned preconditioner: parent 2D eigenvalue,
+    // r1 point, index of the function inside its r1 block (fractions)
+    int nmax2 = 1;
+    for (int c = 0; c < nc; ++c) nmax2 = std::max(nmax2, iv2[c]);
+    std::vector<double> st(3 * (size_t)n3);
+    int col = 0;
+    for (int c = 0; c < nc; ++c)
+      for (int j = 0; j < iv2[c]; ++j, ++col) {
+        st[col] = e2sel[col];
+        st[n3 + col] = nc > 1 ? (double)c / (nc - 1) : 0.0;
+        st[2 * (size_t)n3 + col] = nmax2 > 1 ? (double)j / (nmax2 - 1) : 0.0;
+      }
+    upload(C->st3, st, s);
+  }
   const long G = (long)na * nb;
   C->cint.ensure(sizeof(double) * (size_t)G * n3);
   const CBlock* d_cb = upload(C->cblk, cb, s);
@@ -366,10 +386,17 @@ void run_pipeline(dvr_ctx_t* C) {
     // block-Davidson with certified slices; the dense path is the fallback
     DavParams P;
     P.nev = meval;
-    P.block = env_int("DVR_DAV_BLOCK", 64);
-    P.slice = env_int("DVR_DAV_SLICE", 128);
-    P.mmax = env_int("DVR_DAV_MMAX", 320);
+    P.block = env_int("DVR_DAV_BLOCK", P.block);
+    C->dav.rr_single_wg = env_int("DVR_DAV_RRWG", 0) != 0;
+    P.slice = env_int("DVR_DAV_SLICE", P.slice);
+    P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.max_iter = env_int("DVR_DAV_MAXIT", 2000);
+    P.prec = C->prec_kind;
+    if (P.prec == 2) {
+      if (!C->mlp_set) throw std::runtime_error("learned preconditioner selected without parameters");
+      P.mlp = C->mlpP.as<float>();
+      P.st = C->st3.as<double>();
+    }
     if (const char* t = getenv("DVR_DAV_TOL")) P.tol = atof(t);
     C->dav.tune_mv = C->tune_mv;
     C->dav_status = C->dav.solve(C->H3.as<double>(), n3, P, s);
@@ -378,6 +405,13 @@ void run_pipeline(dvr_ctx_t* C) {
     } else {
       dense = true;   // H3 is intact: the Davidson solver works on copies
     }
+  }
+  C->h_kept = false;
+  if (C->keep_h) {
+    C->H3keep.ensure(sizeof(double) * (size_t)n3 * n3);
+    DVR_HIP(hipMemcpyAsync(C->H3keep.p, C->H3.p, sizeof(double) * (size_t)n3 * n3,
+                           hipMemcpyDeviceToDevice, s));
+    C->h_kept = true;
   }
   if (dense) {
     const int want = std::min(n3, meval + 16);
@@ -641,6 +675,133 @@ int dvr_get_solver_stats(const dvr_ctx_t* c, int32_t* st, int32_t n) {
                         d.slices, d.certified, d.missing,
                         c->dims.solver == DVR_SOLVER_DAVIDSON && c->dav_status != 0};
   for (int i = 0; i < n && i < 9; ++i) st[i] = v[i];
+  return DVR_OK;
+}
+
+int dvr_keep_hamiltonian(dvr_ctx_t* c, int32_t on) {
+  if (!c) return DVR_ERR_ARG;
+  c->keep_h = on != 0;
+  return DVR_OK;
+}
+
+int dvr_set_preconditioner(dvr_ctx_t* c, int32_t kind, const float* params) {
+  if (!c || kind < 0 || kind > 2 || (kind == 2 && !params)) return DVR_ERR_ARG;
+  try {
+    DeviceGuard g(c->device);
+    c->prec_kind = kind;
+    if (kind == 2) {
+      c->mlpP.ensure(sizeof(float) * MLP_NPARAM);
+      DVR_HIP(hipMemcpy(c->mlpP.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+      c->mlp_set = true;
+    }
+  } catch (const std::exception& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  }
+  return DVR_OK;
+}
+
+int dvr_mlp_train(dvr_ctx_t* c, int32_t steps, int32_t batch, uint64_t seed, float lr,
+                  float* params, double* loss_hist) {
+  if (!c || !params || steps < 1 || batch < 1) return DVR_ERR_ARG;
+  if (!c->ran || !c->h_kept || c->nvec_done < 2)
+    return fail(c, DVR_ERR_ARG,
+                "dvr_mlp_train: needs a run with dvr_keep_hamiltonian and nvec >= 2");
+  try {
+    DeviceGuard g(c->device);
+    hipStream_t s = c->stream;
+    const int n3 = c->nham3, nz = c->nvec_done;
+    DevBuf P, Dd;
+    P.ensure(sizeof(float) * MLP_NPARAM);
+    DVR_HIP(hipMemcpyAsync(P.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, s));
+    std::vector<double> dg(n3);
+    DVR_HIP(hipMemcpy2DAsync(dg.data(), sizeof(double), c->H3keep.p, sizeof(double) * (n3 + 1),
+                             sizeof(double), n3, hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipStreamSynchronize(s));
+    upload(Dd, dg, s);
+    std::vector<double> lam(c->h_eval.begin(), c->h_eval.begin() + nz);
+    std::vector<double> hist;
+    const int rc = mlp_train(s, c->H3keep.as<double>(), n3, c->Z3.as<double>(), nz, lam,
+                             Dd.as<double>(), c->st3.as<double>(), P.as<float>(), steps, batch,
+                             (unsigned long long)seed, lr, hist, c->mw);
+    if (rc != 0) return fail(c, DVR_ERR_ARG, "dvr_mlp_train: bad sizes");
+    DVR_HIP(hipMemcpyAsync(params, P.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, s));
+    DVR_HIP(hipStreamSynchronize(s));
+    if (loss_hist) std::memcpy(loss_hist, hist.data(), sizeof(double) * hist.size());
+  } catch (const std::exception& e) {
+    return fail(c, DVR_ERR_HIP, e.what());
+  }
+  return DVR_OK;
+}
+
+int dvr_mlp_forward(const float* params, int64_t nrows, const float* feats, float* w,
+                    int32_t bf16) {
+  if (!params || nrows <= 0 || !feats || !w) return DVR_ERR_ARG;
+  try {
+    hipStream_t s;
+    DVR_HIP(hipStreamCreate(&s));
+    {
+      DevBuf P, F, W;
+      P.ensure(sizeof(float) * MLP_NPARAM);
+      F.ensure(sizeof(float) * nrows * MLP_NF);
+      W.ensure(sizeof(float) * nrows);
+      DVR_HIP(hipMemcpyAsync(P.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, s));
+      DVR_HIP(hipMemcpyAsync(F.p, feats, sizeof(float) * nrows * MLP_NF, hipMemcpyHostToDevice, s));
+      if (bf16)
+        launch_mlp_infer_bf16(s, P.as<float>(), nrows, F.as<float>(), W.as<float>());
+      else
+        launch_mlp_fwd(s, P.as<float>(), nrows, F.as<float>(), W.as<float>(), nullptr, nullptr);
+      DVR_HIP(hipMemcpyAsync(w, W.p, sizeof(float) * nrows, hipMemcpyDeviceToHost, s));
+      DVR_HIP(hipStreamSynchronize(s));
+    }
+    DVR_HIP(hipStreamDestroy(s));
+  } catch (const std::exception& e) {
+    fprintf(stderr, "dvr_mlp_forward: %s\n", e.what());
+    return DVR_ERR_HIP;
+  }
+  return DVR_OK;
+}
+
+int dvr_mlp_backward(const float* params, int64_t nrows, const float* feats, const float* gw,
+                     float* grad) {
+  if (!params || nrows <= 0 || !feats || !gw || !grad) return DVR_ERR_ARG;
+  try {
+    hipStream_t s;
+    DVR_HIP(hipStreamCreate(&s));
+    {
+      const long R = nrows;
+      DevBuf P, F, W, GW, H1, H2, GA1, GA2, G, part, ones;
+      P.ensure(sizeof(float) * MLP_NPARAM);
+      F.ensure(sizeof(float) * R * MLP_NF);
+      W.ensure(sizeof(float) * R);
+      GW.ensure(sizeof(float) * R);
+      H1.ensure(sizeof(float) * R * MLP_WIDTH);
+      H2.ensure(sizeof(float) * R * MLP_WIDTH);
+      GA1.ensure(sizeof(float) * R * MLP_WIDTH);
+      GA2.ensure(sizeof(float) * R * MLP_WIDTH);
+      G.ensure(sizeof(float) * MLP_NPARAM);
+      const long pcap = 64L * MLP_WIDTH * MLP_WIDTH;
+      part.ensure(sizeof(float) * pcap);
+      std::vector<float> one(R, 1.0f);
+      ones.ensure(sizeof(float) * R);
+      DVR_HIP(hipMemcpyAsync(ones.p, one.data(), sizeof(float) * R, hipMemcpyHostToDevice, s));
+      DVR_HIP(hipMemcpyAsync(P.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, s));
+      DVR_HIP(hipMemcpyAsync(F.p, feats, sizeof(float) * R * MLP_NF, hipMemcpyHostToDevice, s));
+      DVR_HIP(hipMemcpyAsync(GW.p, gw, sizeof(float) * R, hipMemcpyHostToDevice, s));
+      launch_mlp_fwd(s, P.as<float>(), R, F.as<float>(), W.as<float>(), H1.as<float>(),
+                     H2.as<float>());
+      launch_mlp_bwd(s, P.as<float>(), R, F.as<float>(), H1.as<float>(), H2.as<float>(),
+                     GW.as<float>(), GA1.as<float>(), GA2.as<float>(), G.as<float>(),
+                     part.as<float>(), pcap);
+      launch_mlp_bias_grads(s, R, ones.as<float>(), GA1.as<float>(), GA2.as<float>(),
+                            GW.as<float>(), G.as<float>(), part.as<float>(), pcap);
+      DVR_HIP(hipMemcpyAsync(grad, G.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, s));
+      DVR_HIP(hipStreamSynchronize(s));
+    }
+    DVR_HIP(hipStreamDestroy(s));
+  } catch (const std::exception& e) {
+    fprintf(stderr, "dvr_mlp_backward: %s\n", e.what());
+    return DVR_ERR_HIP;
+  }
   return DVR_OK;
 }
 

@@ -48,7 +48,7 @@ void DavidsonSolver::small_eig(int m, double* S, double* U, std::vector<double>&
   // columns in proportion to their overlap and so keep the residuals at the
   // eps ||T|| level
   rr.cluster_rel = 1e-10;
-  rr.tridiag(std::vector<int>{m}, S, std::vector<long>{0}, s);
+  rr.tridiag(std::vector<int>{m}, S, std::vector<long>{0}, s, rr_single_wg);
   rr.values(std::vector<int>{m}, s);
   w.assign(rr.hval.begin(), rr.hval.begin() + m);
   if (U) {
@@ -195,8 +195,19 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
     DVR_HIP(hipStreamSynchronize(s));
   };
 
-  // first shift: the lowest Ritz value of the start block
-  {
+  const bool si = P.prec == 0;
+  if (!si) {
+    Dd.ensure(sizeof(double) * N);
+    DVR_HIP(hipMemcpyAsync(Dd.p, dg.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    if (P.prec == 2) {
+      if (!P.mlp || !P.st) throw HipError("davidson: learned preconditioner without parameters");
+      Fd.ensure(sizeof(float) * (size_t)N * b * MLP_NF);
+      Wd.ensure(sizeof(float) * (size_t)N * b);
+    }
+  }
+  // first shift (shift-and-invert only): the lowest Ritz value of the start
+  // block; the diagonal preconditioners factor only to certify slices
+  if (si) {
     std::vector<double> w0;
     double* T = Tm.as<double>();
     gemm(s, "proj", true, false, m, m, N, 1.0, V.as<double>(), N, AV.as<double>(), N, 0.0, T, m);
@@ -245,7 +256,20 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
       const int na = (int)act.size();
       launch_resid_gather(s, X.as<double>(), AX.as<double>(), N, thd.as<double>(),
                           upload(idxd, act, s), na, Rb.as<double>(), N, N);
-      ldl.solve(Rb.as<double>(), Tb.as<double>(), na, s);
+      if (si) {
+        ldl.solve(Rb.as<double>(), Tb.as<double>(), na, s);
+      } else {
+        std::vector<double> ta(na);
+        for (int c = 0; c < na; ++c) ta[c] = th[act[c]];
+        const double* thp = upload(tha, ta, s);
+        const float* wp = nullptr;
+        if (P.prec == 2) {
+          launch_mlp_features(s, Dd.as<double>(), P.st, N, na, thp, thp, Fd.as<float>());
+          launch_mlp_infer_bf16(s, P.mlp, (long)N * na, Fd.as<float>(), Wd.as<float>());
+          wp = Wd.as<float>();
+        }
+        launch_mlp_apply(s, wp, Dd.as<double>(), thp, Rb.as<double>(), N, na, Tb.as<double>());
+      }
       stats.solves += na;
       // restart on the kept Ritz vectors (contiguous columns nl .. nl+mk)
       if (mk > 0) {

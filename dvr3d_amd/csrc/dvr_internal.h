@@ -479,7 +479,7 @@ int gemm_auto_ksplit(int M, int N, int K);
 // LDL^T of H - sigma I without pivoting (k_ldl.hip)
 // ---------------------------------------------------------------------------
 struct LdlFactor {
-  static constexpr int SB = 512;   // super-block of the triangular solves
+  static constexpr int SB = 1024;  // super-block of the triangular solves
   int n = 0, nsb = 0;
   DevBuf A, d, linv64, linvsb, w, work;
   std::vector<double> hd;
@@ -511,9 +511,12 @@ void launch_scale_cols(hipStream_t s, double* U, long ld, int m, int ncol, const
 
 struct DavParams {
   int nev = 0;          // wanted levels
+  int prec = 0;         // 0 shift-and-invert, 1 diagonal, 2 learned diagonal
+  const float* mlp = nullptr;     // learned preconditioner parameters (device)
+  const double* st = nullptr;     // static features of the basis (device, 3 x N)
   int block = 64;       // correction block
-  int slice = 128;      // levels locked per certified slice
-  int mmax = 320;       // largest search space
+  int slice = 256;      // levels locked per certified slice
+  int mmax = 448;       // largest search space
   double tol = 1e-7;    // residual norm (Eh) for locking
   int max_iter = 2000;
 };
@@ -531,6 +534,7 @@ class DavidsonSolver {
   DavStats stats;
   double sigma = 0.0;
   int tune_mv = 0;      // split-K of the H V product (0: heuristic)
+  bool rr_single_wg = false;   // Rayleigh-Ritz through the one-work-group solver
 
  private:
   void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,
@@ -541,11 +545,54 @@ class DavidsonSolver {
   int cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s);
   int N = 0, m = 0, nlocked = 0;
   DevBuf V, AV, X, AX, XL, Rb, Tb, Tm, Y, Y2, G, G2, U, C1, thd, nrmd, idxd, scal, work;
-  DevBuf gd, gsc, gr, gm;
+  DevBuf gd, gsc, gr, gm, Dd, Fd, Wd, tha;
   EigWork rr;
   LdlFactor ldl;
   std::vector<double> thL, gw;
   std::vector<int> perm;
 };
+
+// ---------------------------------------------------------------------------
+// Learned preconditioner (k_mlp.hip, dvr_mlp.hip)
+// ---------------------------------------------------------------------------
+#define MLP_NF 6
+#define MLP_WIDTH 128
+#define MLP_NPARAM (MLP_NF * MLP_WIDTH + MLP_WIDTH + MLP_WIDTH * MLP_WIDTH + MLP_WIDTH + MLP_WIDTH + 1)
+#define MLP_DELTA 1.0e-3   // energy scale of the features (Eh)
+#define MLP_DMIN 1.0e-4    // smallest |D_j - theta| of the diagonal correction (Eh)
+void launch_mlp_fwd(hipStream_t s, const float* P, long R, const float* F, float* W, float* H1,
+                    float* H2);
+void launch_mlp_infer_bf16(hipStream_t s, const float* P, long R, const float* F, float* W);
+void launch_mlp_bwd(hipStream_t s, const float* P, long R, const float* F, const float* H1,
+                    const float* H2, const float* GW, float* GA1, float* GA2, float* grad,
+                    float* partial, long partial_floats);
+void launch_mlp_bias_grads(hipStream_t s, long R, const float* ones, const float* GA1,
+                           const float* GA2, const float* GW, float* grad, float* partial,
+                           long partial_floats);
+void launch_adam(hipStream_t s, float* P, const float* g, float* m, float* v, int n, float lr,
+                 int step);
+void launch_mlp_features(hipStream_t s, const double* D, const double* st, int N, int S,
+                         const double* theta, const double* sigma, float* F);
+void launch_mlp_apply(hipStream_t s, const float* w, const double* D, const double* theta,
+                      const double* R, int N, int S, double* T);
+void launch_mlp_gw(hipStream_t s, const double* gt, const double* D, const double* theta,
+                   const double* R, int N, int S, double scale, float* gw);
+void launch_mlp_loss(hipStream_t s, const double* T, const double* X, const double* E, int N,
+                     int S, double* G, double* loss);
+void launch_coldot(hipStream_t s, const double* X, const double* Y, int N, int S, double* out);
+void launch_col_axpy(hipStream_t s, const double* X, const double* c, double alpha, int N, int S,
+                     double* Y);
+
+struct MlpWork {
+  DevBuf X, AX, Rr, E, T, G, Cm, th, c, loss, F, W, GW, H1, H2, GA1, GA2, partial, grad, m, v,
+      ones, kidx;
+};
+// Trains the parameters P (device, MLP_NPARAM floats) in place on the
+// eigenpairs (lam, Z: N x nz) of H; D = diag H, st = static features
+// (e2, r1 fraction, 2D index fraction; 3 x N).  hist: mean loss per step.
+int mlp_train(hipStream_t s, const double* H, int N, const double* Z, int nz,
+              const std::vector<double>& lam, const double* D, const double* st, float* P,
+              int steps, int S, unsigned long long seed, float lr, std::vector<double>& hist,
+              MlpWork& w);
 
 }  // namespace dvr

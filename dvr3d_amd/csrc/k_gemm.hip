@@ -166,7 +166,8 @@ void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N
   g.ksplit = ksplit; g.kchunk = kchunk; g.work = work;
   const dim3 grid((M + TM - 1) / TM, (N + TM - 1) / TM, ksplit);
   const double flops = 2.0 * M * (double)N * K * (lower ? 0.5 : 1.0);
-  const double bytes = 8.0 * ((double)M * K + (double)K * N + (beta != 0.0 ? 2.0 : 1.0) * M * N);
+  const double bytes = 8.0 * ((double)M * K + (double)K * N +
+                              (beta != 0.0 ? 2.0 : 1.0) * M * N * (lower ? 0.5 : 1.0));
   if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true>), grid, dim3(256), 0, s, g);
   else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false>), grid, dim3(256), 0, s, g);
   else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true>), grid, dim3(256), 0, s, g);

@@ -135,7 +135,7 @@ void LdlFactor::ensure(int n_) {
   d.ensure(sizeof(double) * (size_t)n);
   linv64.ensure(sizeof(double) * (size_t)((n + NB - 1) / NB) * NB * NB);
   linvsb.ensure(sizeof(double) * (size_t)nsb * SB * SB);
-  w.ensure(sizeof(double) * (size_t)n * NB);
+  w.ensure(sizeof(double) * (size_t)n * 2 * NB);
   hd.resize(n);
 }
 
@@ -146,24 +146,41 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
              dim3(256), 0, s, H, a, (long)n, sigma);
   double* dd = d.as<double>();
   double* l64 = linv64.as<double>();
-  double* W = w.as<double>();
-  for (int k0 = 0; k0 < n; k0 += NB) {
-    const int nb = std::min(NB, n - k0);
-    double* li = l64 + (size_t)(k0 / NB) * NB * NB;
-    DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
-               dim3(256), 0, s, a, lda, k0, nb, dd, li);
-    const int m = n - k0 - nb;
-    if (m <= 0) break;
-    double* A21 = a + (k0 + nb) + (long)k0 * lda;
-    // W21 = A21 L11^-T
-    launch_gemm(s, "ldl_panel", false, true, m, nb, nb, 1.0, A21, lda, li, nb, 0.0, W, m, 1,
-                nullptr, false);
-    DVR_LAUNCH("ldl_scale", 24.0 * m * nb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)m * nb)),
-               dim3(256), 0, s, A21, lda, W, (long)m, m, nb, dd + k0);
-    // A22 -= L21 W21^T, lower tiles
-    double* A22 = a + (k0 + nb) + (long)(k0 + nb) * lda;
-    launch_gemm(s, "ldl_update", false, true, m, m, nb, -1.0, A21, lda, W, m, 1.0, A22, lda, 1,
-                nullptr, true);
+  double* W = w.as<double>();   // n x 128, row index = global row
+  auto linv_of = [&](int k0) { return l64 + (size_t)(k0 / NB) * NB * NB; };
+  // panels of 2 NB columns: the second half is updated by the first half
+  // only, the trailing matrix once per panel with K = 2 NB
+  for (int p0 = 0; p0 < n; p0 += 2 * NB) {
+    const int nba = std::min(NB, n - p0);
+    DVR_LAUNCH("ldl_diag", 16.0 * nba * nba, nba * (double)nba * nba / 3.0, k_ldl_diag, dim3(1),
+               dim3(256), 0, s, a, lda, p0, nba, dd, linv_of(p0));
+    const int ma = n - p0 - nba;
+    if (ma <= 0) break;
+    double* L21a = a + (p0 + nba) + (long)p0 * lda;
+    double* Wa = W + (p0 + nba);
+    launch_gemm(s, "ldl_panel", false, true, ma, nba, nba, 1.0, L21a, lda, linv_of(p0), NB, 0.0, Wa,
+                n, 1, nullptr, false);
+    DVR_LAUNCH("ldl_scale", 24.0 * ma * nba, 0.0, k_ldl_scale_cols, dim3(grid_for((long)ma * nba)),
+               dim3(256), 0, s, L21a, lda, Wa, (long)n, ma, nba, dd + p0);
+    const int k1 = p0 + nba;
+    const int nbb = std::min(NB, ma);
+    // second half of the panel (all rows below the first half)
+    launch_gemm(s, "ldl_update", false, true, ma, nbb, nba, -1.0, L21a, lda, Wa, n, 1.0,
+                a + k1 + (long)k1 * lda, lda, 1, nullptr, false);
+    DVR_LAUNCH("ldl_diag", 16.0 * nbb * nbb, nbb * (double)nbb * nbb / 3.0, k_ldl_diag, dim3(1),
+               dim3(256), 0, s, a, lda, k1, nbb, dd, linv_of(k1));
+    const int mb = ma - nbb;
+    if (mb <= 0) break;
+    const int r2 = k1 + nbb;
+    double* L21b = a + r2 + (long)k1 * lda;
+    double* Wb = W + r2 + (long)NB * n;
+    launch_gemm(s, "ldl_panel", false, true, mb, nbb, nbb, 1.0, L21b, lda, linv_of(k1), NB, 0.0, Wb,
+                n, 1, nullptr, false);
+    DVR_LAUNCH("ldl_scale", 24.0 * mb * nbb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)mb * nbb)),
+               dim3(256), 0, s, L21b, lda, Wb, (long)n, mb, nbb, dd + k1);
+    // A22 -= [L21a L21b] [W21a W21b]^T over the rows below the panel
+    launch_gemm(s, "ldl_update", false, true, mb, mb, nba + nbb, -1.0, a + r2 + (long)p0 * lda, lda,
+                W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, 1, nullptr, true);
   }
   // inverse of every SB x SB diagonal super-block (unit lower), from the
   // NB-level factors: X = L_sb^-1 by forward substitution on the identity

@@ -419,7 +419,7 @@ k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
   const int p = C.p;
   const long vo = b.voff[p];
   const long nm = b.nmax;
-  inverse_iteration_cluster<1>(b.n[p], b.d + vo, b.e + vo, eval + eoff[p], C.kb, C.ke, b.tnrm[p],
+  inverse_iteration_cluster<16>(b.n[p], b.d + vo, b.e + vo, eval + eoff[p], C.kb, C.ke, b.tnrm[p],
                             scratch + t, nm * T, T, Z + zoff[p], (unsigned long long)p);
 }
 

@@ -147,6 +147,37 @@ int dvr_get_solver_stats(const dvr_ctx_t* ctx, int32_t* stats, int32_t n);
 int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int32_t nrhs,
                   double* t, int32_t* inertia);
 
+/* ---- learned Davidson preconditioner (no counterpart in the reference) ----
+ * A 3-layer perceptron (6 features -> 128 -> 128 -> 1, tanh) gives per basis
+ * function j of a Ritz pair (theta, x, r) the weight w_j of the correction
+ * t_j = w_j r_j / (D_j - theta), D = diag H3.  Parameters: DVR_MLP_NPARAM
+ * floats, layout W1[f*128+o], b1[128], W2[k*128+o], b2[128], W3[128], b3. */
+#define DVR_MLP_NPARAM 17537
+#define DVR_PREC_SHIFT_INVERT 0  /* (H - sigma I)^-1 r, LDL^T per slice (default) */
+#define DVR_PREC_DIAGONAL 1      /* r / (D - theta)                             */
+#define DVR_PREC_LEARNED 2       /* w(features) r / (D - theta), BF16 inference */
+
+/* Keep a copy of H3 after the next dvr_run (needed by dvr_mlp_train). */
+int dvr_keep_hamiltonian(dvr_ctx_t* ctx, int32_t on);
+
+/* Correction of the Davidson solver (solver DVR_SOLVER_DAVIDSON); params
+ * (DVR_MLP_NPARAM floats, copied) for DVR_PREC_LEARNED. */
+int dvr_set_preconditioner(dvr_ctx_t* ctx, int32_t kind, const float* params);
+
+/* Trains params in place (Adam, steps x batch training pairs) on the
+ * eigenpairs of the last dense run (nvec >= 2, dvr_keep_hamiltonian set);
+ * loss_hist (optional) receives the mean loss of every step. */
+int dvr_mlp_train(dvr_ctx_t* ctx, int32_t steps, int32_t batch, uint64_t seed, float lr,
+                  float* params, double* loss_hist);
+
+/* Test entry points: network output for nrows feature rows (6 floats each),
+ * FP32 (bf16 = 0) or BF16-MFMA (bf16 = 1); gradients of sum(gw * w) with
+ * respect to the parameters (FP32). */
+int dvr_mlp_forward(const float* params, int64_t nrows, const float* feats, float* w,
+                    int32_t bf16);
+int dvr_mlp_backward(const float* params, int64_t nrows, const float* feats, const float* gw,
+                     float* grad);
+
 /*
  * Kernel-level timing (no counterpart in the reference, which only prints
  * the total run time, D:67/D:75-78).  With profiling on, the next dvr_run

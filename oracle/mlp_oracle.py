@@ -11,8 +11,11 @@ Network (per basis function j of the contracted 3D basis, per Ritz pair s):
   f = features(j, s)                            (NF = 6)
   h1 = tanh(W1^T f + b1)                        (128)
   h2 = tanh(W2^T h1 + b2)                       (128)
-  w  = exp(clip(W3^T h2 + b3, -ZMAX, ZMAX))      preconditioner weight
-The correction of Ritz pair s is t = (H - sigma I)^-1 (w o r).
+  w  = W3^T h2 + b3                             preconditioner weight
+The correction of Ritz pair (theta, x) with residual r is the learned
+diagonal preconditioner t_j = w_j r_j / (D_j - theta) (D = diag H, the
+denominator clamped away from zero); with W3 = 0, b3 = 1 it is the classical
+Davidson preconditioner.
 
 Training loss per pair: 1 - cos^2 between the correction, projected out of the
 Ritz vector x, and the true error e (x* - x, projected out of x).
@@ -23,8 +26,8 @@ import numpy as np
 
 NF = 6
 WIDTH = 128
-ZMAX = 4.0
 DELTA = 1.0e-3      # energy scale of the features (Eh)
+DMIN = 1.0e-4       # smallest |D_j - theta| of the classical denominator (Eh)
 
 
 def features(diag, e2, ir1, n2d, nr1, nmax, theta, sigma):
@@ -52,7 +55,7 @@ def init_params(seed=0):
         W2=rng.standard_normal((WIDTH, WIDTH)) / np.sqrt(WIDTH),
         b2=np.zeros(WIDTH),
         W3=np.zeros(WIDTH),
-        b3=np.zeros(1),
+        b3=np.ones(1),
     )
 
 
@@ -61,16 +64,14 @@ def forward(p, f):
     h1 = np.tanh(a1)
     a2 = h1 @ p["W2"] + p["b2"]
     h2 = np.tanh(a2)
-    z = h2 @ p["W3"] + p["b3"][0]
-    zc = np.clip(z, -ZMAX, ZMAX)
-    w = np.exp(zc)
-    return w, dict(f=f, h1=h1, h2=h2, z=z, w=w)
+    w = h2 @ p["W3"] + p["b3"][0]
+    return w, dict(f=f, h1=h1, h2=h2, w=w)
 
 
 def backward(p, cache, gw):
     """Gradients of sum(gw * w) with respect to every parameter."""
-    f, h1, h2, z, w = (cache[k] for k in ("f", "h1", "h2", "z", "w"))
-    gz = gw * w * ((z > -ZMAX) & (z < ZMAX))
+    f, h1, h2 = (cache[k] for k in ("f", "h1", "h2"))
+    gz = gw
     g = dict(W3=h2.T @ gz, b3=np.array([gz.sum()]))
     ga2 = np.outer(gz, p["W3"]) * (1.0 - h2 * h2)
     g["W2"] = h1.T @ ga2
@@ -79,6 +80,35 @@ def backward(p, cache, gw):
     g["W1"] = f.T @ ga1
     g["b1"] = ga1.sum(axis=0)
     return g
+
+
+def classical(diag, theta):
+    """1 / (D - theta) with the denominator kept at least DMIN from zero."""
+    den = diag - theta
+    den = np.where(np.abs(den) < DMIN, np.copysign(DMIN, den), den)
+    return 1.0 / den
+
+
+def flatten(p):
+    """Parameter layout of the device kernels (float32): W1 (NF x 128,
+    row-major), b1, W2 (128 x 128, row-major: [k, o]), b2, W3, b3."""
+    return np.concatenate([p["W1"].ravel(), p["b1"], p["W2"].ravel(), p["b2"], p["W3"],
+                           p["b3"]]).astype(np.float32)
+
+
+def unflatten(v):
+    v = np.asarray(v, dtype=np.float64)
+    o = 0
+    out = {}
+    for k, shp in (("W1", (NF, WIDTH)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)),
+                   ("b2", (WIDTH,)), ("W3", (WIDTH,)), ("b3", (1,))):
+        n = int(np.prod(shp))
+        out[k] = v[o:o + n].reshape(shp)
+        o += n
+    return out
+
+
+NPARAM = NF * WIDTH + WIDTH + WIDTH * WIDTH + WIDTH + WIDTH + 1
 
 
 def loss_and_grad_t(t, x, e):
