@@ -165,7 +165,7 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
     const int k1 = p0 + nba;
     const int nbb = std::min(NB, ma);
     // second half of the panel (all rows below the first half)
-    launch_gemm(s, "ldl_update", false, true, ma, nbb, nba, -1.0, L21a, lda, Wa, n, 1.0,
+    launch_gemm(s, "ldl_panel_upd", false, true, ma, nbb, nba, -1.0, L21a, lda, Wa, n, 1.0,
                 a + k1 + (long)k1 * lda, lda, 1, nullptr, false);
     DVR_LAUNCH("ldl_diag", 16.0 * nbb * nbb, nbb * (double)nbb * nbb / 3.0, k_ldl_diag, dim3(1),
                dim3(256), 0, s, a, lda, k1, nbb, dd, linv_of(k1));
