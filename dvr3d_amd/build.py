@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
-HIP_SOURCES = ["k_grid.hip", "k_jacobi.hip", "k_syrk.hip", "k_syev.hip", "k_syev_wg.hip", "k_gemm.hip", "k_ldl.hip", "k_dav.hip", "k_mlp.hip", "dvr_davidson.hip", "dvr_mlp.hip",
+HIP_SOURCES = ["k_grid.hip", "k_jacobi.hip", "k_syrk.hip", "k_syev.hip", "k_syev_wg.hip", "k_gemm.hip", "k_ldl.hip", "k_dav.hip", "k_mlp.hip", "dvr_davidson.hip", "dvr_mlp.hip", "dvr_refine.hip",
                "k_syev_lds.hip", "dvr_capi.hip"]
 ARCH = os.environ.get("DVR_OFFLOAD_ARCH", "gfx950")
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -44,8 +44,11 @@ struct dvr_ctx {
   DevBuf cint, blk3, e2sel, cblk, sdesc3, H3, Z3, psi;
   EigWork w2, w3;
   DavidsonSolver dav;
-  int dav_status = -1, tune_mv = 0;
-  bool keep_h = false, h_kept = false;
+  int dav_status = -1;
+  bool keep_h = false, h_kept = false, split32 = false;
+  double refine_resid = 0.0;
+  DevBuf Zr;
+  RefineWork rw;
   DevBuf H3keep, st3, mlpP;
   int prec_kind = 0;
   bool mlp_set = false;
@@ -368,10 +371,19 @@ void run_pipeline(dvr_ctx_t* C) {
     desc[0].diag = upload(C->e2sel, e2sel, s);
     desc[0].tile0 = 0;
     desc[0].ntile = nt * (nt + 1) / 2;
-    launch_syrk_scaled(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile,
-                       C->xkc.as<double>(), nc,
-                       8.0 * ((double)G * n3 + (double)n3 * n3),
-                       (double)G * n3 * (n3 + 1.0));
+    // split-precision FP32 assembly (with FP64 refinement after the
+    // Davidson solve) when DVR_ASSEMBLY=split32 and the chosen form runs
+    const char* asmenv = getenv("DVR_ASSEMBLY");
+    C->split32 = asmenv && std::string(asmenv) == "split32" && D.solver == DVR_SOLVER_DAVIDSON;
+    if (C->split32)
+      launch_syrk_split32(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile, C->xkc.as<double>(),
+                          nc, 8.0 * ((double)G * n3 + (double)n3 * n3),
+                          (double)G * n3 * (n3 + 1.0));
+    else
+      launch_syrk_scaled(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile,
+                         C->xkc.as<double>(), nc,
+                         8.0 * ((double)G * n3 + (double)n3 * n3),
+                         (double)G * n3 * (n3 + 1.0));
   }
   DVR_HIP(hipEventRecord(C->ev[5], s));
 
@@ -388,6 +400,7 @@ void run_pipeline(dvr_ctx_t* C) {
     P.nev = meval;
     P.block = env_int("DVR_DAV_BLOCK", P.block);
     C->dav.rr_single_wg = env_int("DVR_DAV_RRWG", 0) != 0;
+    C->dav.use_graph = env_int("DVR_GRAPH", 1) != 0;
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.max_iter = env_int("DVR_DAV_MAXIT", 2000);
@@ -398,10 +411,28 @@ void run_pipeline(dvr_ctx_t* C) {
       P.st = C->st3.as<double>();
     }
     if (const char* t = getenv("DVR_DAV_TOL")) P.tol = atof(t);
-    C->dav.tune_mv = C->tune_mv;
+    C->dav.tuner.enabled = env_int("DVR_TUNE", 1) != 0;
     C->dav_status = C->dav.solve(C->H3.as<double>(), n3, P, s);
     if (C->dav_status == 0) {
       C->h_eval.assign(C->dav.eval.begin(), C->dav.eval.begin() + meval);
+      if (C->split32) {
+        // the levels of the split-precision H3, refined with the exact FP64
+        // operator: Rayleigh-Ritz on the span of the meval Davidson vectors
+        prof_phase("refine");
+        C->Zr.ensure(sizeof(double) * (size_t)n3 * meval);
+        C->dav.vectors(meval, C->Zr.as<double>(), s);
+        std::vector<int> col0(nc), iv2v(nc);
+        for (int c = 0; c < nc; ++c) {
+          col0[c] = cb[c].col0;
+          iv2v[c] = cb[c].iv2;
+        }
+        std::vector<double> ev;
+        C->refine_resid = refine_pairs(s, C->cint.as<double>(), G, col0, iv2v, C->xkc.as<double>(),
+                                       C->e2sel.as<double>(), n3, C->Zr.as<double>(), meval, ev,
+                                       C->rw);
+        C->h_eval.assign(ev.begin(), ev.begin() + meval);
+        prof_phase("3d");
+      }
     } else {
       dense = true;   // H3 is intact: the Davidson solver works on copies
     }
@@ -427,6 +458,10 @@ void run_pipeline(dvr_ctx_t* C) {
       const int mext = C->w3.extend(0, nvec);
       C->Z3.ensure(sizeof(double) * (size_t)n3 * mext);
       C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);
+    } else if (C->split32) {
+      C->Z3.ensure(sizeof(double) * (size_t)n3 * nvec);
+      DVR_HIP(hipMemcpyAsync(C->Z3.p, C->Zr.p, sizeof(double) * (size_t)n3 * nvec,
+                             hipMemcpyDeviceToDevice, s));
     } else {
       C->Z3.ensure(sizeof(double) * (size_t)n3 * nvec);
       C->dav.vectors(nvec, C->Z3.as<double>(), s);

@@ -29,12 +29,8 @@ namespace dvr {
 
 void DavidsonSolver::gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
                           double alpha, const double* A, long lda, const double* B, long ldb,
-                          double beta, double* C, long ldc, bool big) {
-  int ks = gemm_auto_ksplit(M, N, K);
-  if (big && tune_mv > 0) ks = tune_mv;
-  if (ks > 1) work.ensure(sizeof(double) * gemm_work_doubles(M, N, ks));
-  launch_gemm(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ks,
-              ks > 1 ? work.as<double>() : nullptr, false);
+                          double beta, double* C, long ldc) {
+  tuner.run(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, false);
 }
 
 // eigen-decomposition of the symmetric m x m matrix at S (destroyed):
@@ -93,8 +89,7 @@ int DavidsonSolver::expand(const double* H, int q0, hipStream_t s) {
   if (q == 0) return 0;
   DVR_HIP(hipMemcpyAsync(Vn, Wt, sizeof(double) * (size_t)N * q, hipMemcpyDeviceToDevice, s));
   // AV for the new columns
-  gemm(s, "matvec", false, false, N, q, N, 1.0, H, N, Vn, N, 0.0, AV.as<double>() + (long)m * N, N,
-       true);
+  gemm(s, "matvec", false, false, N, q, N, 1.0, H, N, Vn, N, 0.0, AV.as<double>() + (long)m * N, N);
   stats.matvecs += q;
   return q;
 }
@@ -156,6 +151,8 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
   thd.ensure(sizeof(double) * mmax);
   nrmd.ensure(sizeof(double) * mmax);
   ldl.ensure(N);
+  ldl.use_graph = use_graph;
+  ldl.tuner = &tuner;
   thL.clear();
   nlocked = 0;
 
@@ -257,7 +254,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
       launch_resid_gather(s, X.as<double>(), AX.as<double>(), N, thd.as<double>(),
                           upload(idxd, act, s), na, Rb.as<double>(), N, N);
       if (si) {
-        ldl.solve(Rb.as<double>(), Tb.as<double>(), na, s);
+        ldl.solve_graph(Rb.as<double>(), Tb.as<double>(), na, s);
       } else {
         std::vector<double> ta(na);
         for (int c = 0; c < na; ++c) ta[c] = th[act[c]];

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -182,6 +184,9 @@ void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
                         int total_tiles, const double* xk, int ldxk,
                         double bytes, double flops);
 int syrk_tile();
+// same product with FP32 split (hi + lo) operands on FP32 MFMA
+void launch_syrk_split32(hipStream_t s, const SyrkDesc* d_desc, int ngroup, int total_tiles,
+                         const double* xk, int ldxk, double bytes, double flops);
 
 // ---------------------------------------------------------------------------
 // batched dense symmetric eigensolver (k_syev.hip)
@@ -475,6 +480,29 @@ size_t gemm_work_doubles(int M, int N, int ksplit);
 // split of K that gives about a thousand work-groups for small outputs
 int gemm_auto_ksplit(int M, int N, int K);
 
+// Per-shape choice of the GEMM k-slice depth (16 or 32) and K split, timed on
+// first use (GemmTuner::run); enabled = false keeps the fixed rule.
+struct GemmTuner {
+  struct Key {
+    bool ta, tb, lower;
+    int M, N, K;
+    bool operator<(const Key& o) const {
+      return std::tie(ta, tb, lower, M, N, K) < std::tie(o.ta, o.tb, o.lower, o.M, o.N, o.K);
+    }
+  };
+  struct Choice {
+    int kt, ksplit;
+  };
+  std::map<Key, Choice> cache;
+  bool enabled = true;
+  DevBuf scratch, work;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void run(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,
+           const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
+           bool lower);
+  ~GemmTuner();
+};
+
 // ---------------------------------------------------------------------------
 // LDL^T of H - sigma I without pivoting (k_ldl.hip)
 // ---------------------------------------------------------------------------
@@ -489,6 +517,20 @@ struct LdlFactor {
   int factor(const double* H, double sigma, hipStream_t s);
   // T <- (H - sigma I)^-1 R for nrhs columns; R is destroyed
   void solve(double* R, double* T, int nrhs, hipStream_t s);
+  // the same through a captured HIP graph per nrhs (the sequence of ~160
+  // launches is replayed; recaptured when any buffer it uses moved)
+  void solve_graph(double* R, double* T, int nrhs, hipStream_t s);
+  bool use_graph = true;
+  GemmTuner* tuner = nullptr;   // per-shape GEMM choices (nullptr: fixed rule)
+  ~LdlFactor();
+
+ private:
+  struct Captured {
+    hipGraphExec_t exec = nullptr;
+    const void* key[6] = {};
+  };
+  std::vector<std::pair<int, Captured>> graphs;
+  void clear_graphs();
 };
 
 // ---------------------------------------------------------------------------
@@ -533,13 +575,13 @@ class DavidsonSolver {
   std::vector<double> eval;
   DavStats stats;
   double sigma = 0.0;
-  int tune_mv = 0;      // split-K of the H V product (0: heuristic)
+  GemmTuner tuner;      // per-shape GEMM choices shared with the factorisations
   bool rr_single_wg = false;   // Rayleigh-Ritz through the one-work-group solver
+  bool use_graph = true;       // shift-and-invert solves replayed from HIP graphs
 
  private:
   void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,
-            const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
-            bool big = false);
+            const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc);
   void small_eig(int m, double* S, double* U, std::vector<double>& w, hipStream_t s);
   int expand(const double* H, int q0, hipStream_t s);
   int cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s);
@@ -594,5 +636,24 @@ int mlp_train(hipStream_t s, const double* H, int N, const double* Z, int nz,
               const std::vector<double>& lam, const double* D, const double* st, float* P,
               int steps, int S, unsigned long long seed, float lr, std::vector<double>& hist,
               MlpWork& w);
+
+// ---------------------------------------------------------------------------
+// FP64 refinement with the exact operator (dvr_refine.hip)
+// ---------------------------------------------------------------------------
+void launch_add_diag(hipStream_t s, double* Y, const double* X, const double* e, int N, int nv);
+struct RefineWork {
+  DevBuf U, V, Y, T, Z, X2, Y2, th, nrm, work;
+  EigWork eig;
+};
+// Y = H3 X from the contraction coefficients (no H3 needed)
+void apply_h_exact(hipStream_t s, const double* cint, long G, const std::vector<int>& col0,
+                   const std::vector<int>& iv2, const double* xk, const double* e2, int N3,
+                   const double* X, int nv, double* Y, RefineWork& w);
+// Rayleigh-Ritz of span(X) (N3 x nv, orthonormal) with the exact operator: X
+// is replaced by the refined vectors, eval by the refined values; returns the
+// largest exact residual norm.
+double refine_pairs(hipStream_t s, const double* cint, long G, const std::vector<int>& col0,
+                    const std::vector<int>& iv2, const double* xk, const double* e2, int N3,
+                    double* X, int nv, std::vector<double>& eval, RefineWork& w);
 
 }  // namespace dvr

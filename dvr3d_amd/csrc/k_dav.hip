@@ -149,3 +149,22 @@ void launch_scale_cols(hipStream_t s, double* U, long ld, int m, int ncol, const
 }
 
 }  // namespace dvr
+
+namespace dvr {
+
+// Y[i, v] += e[i] * X[i, v]  (N x nv, ld N)
+__global__ void __launch_bounds__(256)
+k_add_diag(double* __restrict__ Y, const double* __restrict__ X, const double* __restrict__ e,
+           int N, int nv) {
+  const long n = (long)N * nv;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256)
+    Y[t] += e[t % N] * X[t];
+}
+
+void launch_add_diag(hipStream_t s, double* Y, const double* X, const double* e, int N, int nv) {
+  const long n = (long)N * nv;
+  DVR_LAUNCH("add_diag", 24.0 * n, 2.0 * n, k_add_diag,
+             dim3((int)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, Y, X, e, N, nv);
+}
+
+}  // namespace dvr

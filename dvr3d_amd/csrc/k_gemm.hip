@@ -22,7 +22,7 @@ namespace dvr {
 
 namespace {
 constexpr int TM = 64;
-constexpr int KT = 16;
+constexpr int KT0 = 16;   // k-slice granularity of the split-K chunks
 constexpr int LDT = TM + 1;
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -38,8 +38,9 @@ __device__ inline double load_b(const GemmArgs& g, int k, int j) {
 }
 }  // namespace
 
-template <bool TA, bool TB>
+template <bool TA, bool TB, int KT>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  constexpr int NQ = KT / 4;   // elements of each operand per thread and slice
   __shared__ double As[KT * LDT];
   __shared__ double Bs[KT * LDT];
   const int ti = blockIdx.x, tj = blockIdx.y, z = blockIdx.z;
@@ -55,21 +56,21 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   // the fast index follows the contiguous direction of the operand in memory
   auto a_pos = [&](int q, int& kk, int& cc) {
     const int idx = tid + 256 * q;
-    if (TA) { kk = idx & 15; cc = idx >> 4; } else { cc = idx & 63; kk = idx >> 6; }
+    if (TA) { kk = idx % KT; cc = idx / KT; } else { cc = idx & 63; kk = idx >> 6; }
   };
   auto b_pos = [&](int q, int& kk, int& cc) {
     const int idx = tid + 256 * q;
-    if (TB) { cc = idx & 63; kk = idx >> 6; } else { kk = idx & 15; cc = idx >> 4; }
+    if (TB) { cc = idx & 63; kk = idx >> 6; } else { kk = idx % KT; cc = idx / KT; }
   };
 
   v4d acc[2][2];
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  double ra[4], rb[4];
+  double ra[NQ], rb[NQ];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       int kk, cc;
       a_pos(q, kk, cc);
       ra[q] = (k0 + kk < kend) ? load_a<TA>(g, row0 + cc, k0 + kk) : 0.0;
@@ -80,7 +81,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   if (kbeg < kend) fetch(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += KT) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NQ; ++q) {
       int kk, cc;
       a_pos(q, kk, cc);
       As[kk * LDT + cc] = ra[q];
@@ -148,19 +149,20 @@ size_t gemm_work_doubles(int M, int N, int ksplit) {
   return ksplit > 1 ? (size_t)M * N * ksplit : 0;
 }
 
-void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
-                 double alpha, const double* A, long lda, const double* B, long ldb,
-                 double beta, double* C, long ldc, int ksplit, double* work, bool lower) {
+static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                           double alpha, const double* A, long lda, const double* B, long ldb,
+                           double beta, double* C, long ldc, int ksplit, double* work, bool lower,
+                           int kt) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0) ksplit = 1;
   if (ksplit < 1) ksplit = 1;
   GemmArgs g;
   g.M = M; g.N = N; g.K = K; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
   g.alpha = alpha; g.beta = beta; g.lower = lower ? 1 : 0;
-  // slices of whole KT steps
+  // chunks of whole 32-deep slices (a multiple of both slice depths)
   int kchunk = (K + ksplit - 1) / ksplit;
-  kchunk = ((kchunk + KT - 1) / KT) * KT;
-  if (kchunk <= 0) kchunk = KT;
+  kchunk = ((kchunk + 31) / 32) * 32;
+  if (kchunk <= 0) kchunk = 32;
   ksplit = K > 0 ? (K + kchunk - 1) / kchunk : 1;
   if (ksplit > 1 && !work) throw HipError("gemm: split-K needs a workspace");
   g.ksplit = ksplit; g.kchunk = kchunk; g.work = work;
@@ -168,10 +170,17 @@ void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N
   const double flops = 2.0 * M * (double)N * K * (lower ? 0.5 : 1.0);
   const double bytes = 8.0 * ((double)M * K + (double)K * N +
                               (beta != 0.0 ? 2.0 : 1.0) * M * N * (lower ? 0.5 : 1.0));
-  if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true>), grid, dim3(256), 0, s, g);
-  else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false>), grid, dim3(256), 0, s, g);
-  else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true>), grid, dim3(256), 0, s, g);
-  else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false>), grid, dim3(256), 0, s, g);
+  if (kt == 32) {
+    if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true, 32>), grid, dim3(256), 0, s, g);
+    else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false, 32>), grid, dim3(256), 0, s, g);
+    else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true, 32>), grid, dim3(256), 0, s, g);
+    else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false, 32>), grid, dim3(256), 0, s, g);
+  } else {
+    if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true, 16>), grid, dim3(256), 0, s, g);
+    else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false, 16>), grid, dim3(256), 0, s, g);
+    else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true, 16>), grid, dim3(256), 0, s, g);
+    else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false, 16>), grid, dim3(256), 0, s, g);
+  }
   if (ksplit > 1) {
     const long n = (long)M * N;
     const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
@@ -179,6 +188,79 @@ void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N
                0, s, g);
   }
   DVR_LAUNCH_CHECK();
+}
+
+void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                 double alpha, const double* A, long lda, const double* B, long ldb,
+                 double beta, double* C, long ldc, int ksplit, double* work, bool lower) {
+  launch_gemm_kt(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ksplit, work, lower,
+                 16);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: the first call of a shape times the candidate (k-slice depth,
+// K split) pairs on a scratch output and keeps the fastest; later calls of
+// the shape reuse the choice.
+// ---------------------------------------------------------------------------
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  DVR_HIP(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
+GemmTuner::~GemmTuner() {
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+}
+
+void GemmTuner::run(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                    double alpha, const double* A, long lda, const double* B, long ldb,
+                    double beta, double* C, long ldc, bool lower) {
+  if (M <= 0 || N <= 0) return;
+  const Key key{ta, tb, lower, M, N, K};
+  Choice ch{16, gemm_auto_ksplit(M, N, K)};
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    ch = it->second;
+  } else if (enabled && !g_prof && (double)M * N * K >= 1e7 && !capturing(s)) {
+    if (!ev0) {
+      DVR_HIP(hipEventCreate(&ev0));
+      DVR_HIP(hipEventCreate(&ev1));
+    }
+    const int ka = gemm_auto_ksplit(M, N, K);
+    std::vector<Choice> cand;
+    for (int kt : {16, 32})
+      for (int ks : {1, ka, 2 * ka, 4 * ka}) {
+        if (ks > 1 && K / ks < 64) continue;
+        bool dup = false;
+        for (const Choice& c : cand) dup |= (c.kt == kt && c.ksplit == ks);
+        if (!dup) cand.push_back(Choice{kt, ks});
+      }
+    scratch.ensure(sizeof(double) * (size_t)M * N);
+    float best = 1e30f;
+    for (const Choice& c : cand) {
+      if (c.ksplit > 1) work.ensure(sizeof(double) * gemm_work_doubles(M, N, c.ksplit));
+      float t = 1e30f;
+      for (int rep = 0; rep < 2; ++rep) {
+        DVR_HIP(hipEventRecord(ev0, s));
+        launch_gemm_kt(s, "tune", ta, tb, M, N, K, 1.0, A, lda, B, ldb, 0.0, scratch.as<double>(), M,
+                       c.ksplit, c.ksplit > 1 ? work.as<double>() : nullptr, lower, c.kt);
+        DVR_HIP(hipEventRecord(ev1, s));
+        DVR_HIP(hipEventSynchronize(ev1));
+        float ms = 0.f;
+        DVR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+        t = std::min(t, ms);
+      }
+      if (t < best) {
+        best = t;
+        ch = c;
+      }
+    }
+    cache[key] = ch;
+  }
+  if (ch.ksplit > 1) work.ensure(sizeof(double) * gemm_work_doubles(M, N, ch.ksplit));
+  launch_gemm_kt(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ch.ksplit,
+                 ch.ksplit > 1 ? work.as<double>() : nullptr, lower, ch.kt);
 }
 
 }  // namespace dvr

@@ -179,8 +179,12 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
     DVR_LAUNCH("ldl_scale", 24.0 * mb * nbb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)mb * nbb)),
                dim3(256), 0, s, L21b, lda, Wb, (long)n, mb, nbb, dd + k1);
     // A22 -= [L21a L21b] [W21a W21b]^T over the rows below the panel
-    launch_gemm(s, "ldl_update", false, true, mb, mb, nba + nbb, -1.0, a + r2 + (long)p0 * lda, lda,
-                W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, 1, nullptr, true);
+    if (tuner)
+      tuner->run(s, "ldl_update", false, true, mb, mb, nba + nbb, -1.0, a + r2 + (long)p0 * lda,
+                 lda, W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, true);
+    else
+      launch_gemm(s, "ldl_update", false, true, mb, mb, nba + nbb, -1.0, a + r2 + (long)p0 * lda,
+                  lda, W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, 1, nullptr, true);
   }
   // inverse of every SB x SB diagonal super-block (unit lower), from the
   // NB-level factors: X = L_sb^-1 by forward substitution on the identity
@@ -217,10 +221,14 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
 
 // T <- (L D L^T)^-1 R for nrhs columns (R, T: n x nrhs, ld n).  R is used as
 // workspace and destroyed.
-// GEMM with K split for short, wide products
-static void gemm_ks(hipStream_t s, DevBuf& work, const char* name, bool ta, bool tb, int M, int N,
-                    int K, double alpha, const double* A, long lda, const double* B, long ldb,
-                    double beta, double* C, long ldc) {
+// GEMM with K split for short, wide products (tuned when a tuner is set)
+static void gemm_ks(hipStream_t s, DevBuf& work, GemmTuner* tuner, const char* name, bool ta,
+                    bool tb, int M, int N, int K, double alpha, const double* A, long lda,
+                    const double* B, long ldb, double beta, double* C, long ldc) {
+  if (tuner) {
+    tuner->run(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, false);
+    return;
+  }
   const int ks = gemm_auto_ksplit(M, N, K);
   if (ks > 1) work.ensure(sizeof(double) * gemm_work_doubles(M, N, ks));
   launch_gemm(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, ks,
@@ -235,11 +243,11 @@ void LdlFactor::solve(double* R, double* T, int nrhs, hipStream_t s) {
   for (int b = 0; b < nsb; ++b) {
     const int r0 = b * SB, ns = std::min(SB, n - r0);
     const double* X = lsb + (size_t)b * SB * SB;
-    gemm_ks(s, work, "si_fwd_diag", false, false, ns, nrhs, ns, 1.0, X, SB, R + r0, n, 0.0,
+    gemm_ks(s, work, tuner, "si_fwd_diag", false, false, ns, nrhs, ns, 1.0, X, SB, R + r0, n, 0.0,
             T + r0, n);
     const int m = n - r0 - ns;
     if (m > 0)
-      gemm_ks(s, work, "si_fwd_upd", false, false, m, nrhs, ns, -1.0,
+      gemm_ks(s, work, tuner, "si_fwd_upd", false, false, m, nrhs, ns, -1.0,
               a + (r0 + ns) + (long)r0 * lda, lda, T + r0, n, 1.0, R + r0 + ns, n);
   }
   DVR_LAUNCH("si_scale", 16.0 * n * nrhs, 0.0, k_ldl_scale_rows, dim3(grid_for((long)n * nrhs)),
@@ -249,13 +257,61 @@ void LdlFactor::solve(double* R, double* T, int nrhs, hipStream_t s) {
   for (int b = nsb - 1; b >= 0; --b) {
     const int r0 = b * SB, ns = std::min(SB, n - r0);
     const double* X = lsb + (size_t)b * SB * SB;
-    gemm_ks(s, work, "si_bwd_diag", true, false, ns, nrhs, ns, 1.0, X, SB, T + r0, n, 0.0, R + r0,
+    gemm_ks(s, work, tuner, "si_bwd_diag", true, false, ns, nrhs, ns, 1.0, X, SB, T + r0, n, 0.0, R + r0,
             n);
     if (r0 > 0)
-      gemm_ks(s, work, "si_bwd_upd", true, false, r0, nrhs, ns, -1.0, a + r0, lda, R + r0, n, 1.0,
+      gemm_ks(s, work, tuner, "si_bwd_upd", true, false, r0, nrhs, ns, -1.0, a + r0, lda, R + r0, n, 1.0,
               T, n);
   }
   DVR_HIP(hipMemcpyAsync(T, R, sizeof(double) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace dvr
+
+namespace dvr {
+
+void LdlFactor::clear_graphs() {
+  for (auto& g : graphs)
+    if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+  graphs.clear();
+}
+
+LdlFactor::~LdlFactor() { clear_graphs(); }
+
+void LdlFactor::solve_graph(double* R, double* T, int nrhs, hipStream_t s) {
+  if (!use_graph || g_prof) {
+    solve(R, T, nrhs, s);
+    return;
+  }
+  const void* wk = tuner ? tuner->work.p : work.p;
+  const void* key[6] = {A.p, d.p, linvsb.p, wk, R, T};
+  for (auto& g : graphs) {
+    if (g.first != nrhs) continue;
+    if (std::equal(key, key + 6, g.second.key)) {
+      DVR_HIP(hipGraphLaunch(g.second.exec, s));
+      return;
+    }
+  }
+  // first use of this shape (or a buffer moved): run eagerly, which also
+  // sizes the workspace, then capture the identical sequence for next time
+  solve(R, T, nrhs, s);
+  const void* key2[6] = {A.p, d.p, linvsb.p, tuner ? tuner->work.p : work.p, R, T};
+  if (!std::equal(key, key + 6, key2)) clear_graphs();
+  for (auto it = graphs.begin(); it != graphs.end(); ++it)
+    if (it->first == nrhs) {
+      (void)hipGraphExecDestroy(it->second.exec);
+      graphs.erase(it);
+      break;
+    }
+  hipGraph_t g = nullptr;
+  DVR_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  solve(R, T, nrhs, s);
+  DVR_HIP(hipStreamEndCapture(s, &g));
+  Captured c;
+  DVR_HIP(hipGraphInstantiate(&c.exec, g, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(g);
+  std::copy(key2, key2 + 6, c.key);
+  graphs.emplace_back(nrhs, c);
 }
 
 }  // namespace dvr
