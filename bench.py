@@ -120,10 +120,16 @@ def roofline_of(kprof):
         ach = k["bytes"] / k["seconds"] / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+    # HBM bytes per launch of the dominant kernel, from the committed counter
+    # runs (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "ldl_update_traffic.json")
+    if name.endswith("ldl_update") and os.path.exists(tpath):
+        traffic = round(json.load(open(tpath))["avg_measured_bytes"])
     roof.update({"kernel": name, "launches": k["launches"],
                  "avg_launch_us": round(k["seconds"] / k["launches"] * 1e6, 3),
                  "kernel_share": round(k["seconds"] / sum(v["seconds"] for v in kprof.values()), 3),
-                 "traffic": None})
+                 "traffic": traffic})
     return roof
 
 
@@ -140,7 +146,20 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 512 MiB three times (calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--ldl-probe", type=int, default=0, metavar="N",
+                    help="only factor one N x N symmetric matrix with the shift-and-invert "
+                         "LDL^T (counter runs of its trailing-update GEMMs)")
     args = ap.parse_args()
+    if args.ldl_probe:
+        from dvr3d_amd import _lib
+        n = args.ldl_probe
+        rng = np.random.default_rng(0)
+        a = rng.standard_normal((n, n)) * (0.05 / np.sqrt(n))
+        a = 0.5 * (a + a.T)
+        a[np.diag_indices(n)] += np.linspace(0.0, 2.0, n)
+        nu, _ = _lib.ldl_solve(a, 1.0)
+        print(json.dumps({"ldl_probe_n": n, "inertia": nu}))
+        return
     if args.copy_probe:
         import torch
         x = torch.ones(64 << 20, dtype=torch.float64, device="cuda")

@@ -467,7 +467,7 @@ struct GemmArgs {
   double* C;
   long ldc;
   double alpha, beta;
-  int ksplit, kchunk, lower;
+  int ksplit, kchunk, lower, ntile;
   double* work;
 };
 // C = alpha op(A) op(B) + beta C (column-major).  ksplit > 1 splits K over

@@ -43,8 +43,34 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   constexpr int NQ = KT / 4;   // elements of each operand per thread and slice
   __shared__ double As[KT * LDT];
   __shared__ double Bs[KT * LDT];
-  const int ti = blockIdx.x, tj = blockIdx.y, z = blockIdx.z;
-  if (g.lower && ti < tj) return;
+  const int z = blockIdx.z;
+  int ti = blockIdx.x, tj = blockIdx.y;
+  if (g.lower) {
+    // 1D grid over the lower tiles, in groups of 8 row-tiles walked column by
+    // column: neighbouring work-groups (dealt round-robin to the XCDs) share
+    // their column panel, and each XCD keeps one row panel per group in L2
+    const long L = blockIdx.x;
+    const int nt = g.ntile;
+    int r0 = (int)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
+    r0 -= r0 & 7;
+    while (r0 > 0 && (long)r0 * (r0 + 1) / 2 > L) r0 -= 8;
+    while (r0 + 8 < nt && (long)(r0 + 8) * (r0 + 9) / 2 <= L) r0 += 8;
+    const int r1 = min(r0 + 8, nt), gm = r1 - r0;
+    long q = L - (long)r0 * (r0 + 1) / 2;
+    if (q < (long)r0 * gm) {
+      tj = (int)(q / gm);
+      ti = r0 + (int)(q % gm);
+    } else {
+      q -= (long)r0 * gm;
+      int c = r0;
+      while (q >= r1 - c) {
+        q -= r1 - c;
+        ++c;
+      }
+      tj = c;
+      ti = c + (int)q;
+    }
+  }
   const int row0 = ti * TM, col0 = tj * TM;
   const int kbeg = z * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
@@ -166,7 +192,10 @@ static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, in
   ksplit = K > 0 ? (K + kchunk - 1) / kchunk : 1;
   if (ksplit > 1 && !work) throw HipError("gemm: split-K needs a workspace");
   g.ksplit = ksplit; g.kchunk = kchunk; g.work = work;
-  const dim3 grid((M + TM - 1) / TM, (N + TM - 1) / TM, ksplit);
+  if (lower && M != N) throw HipError("gemm: lower-tile product needs a square output");
+  g.ntile = (M + TM - 1) / TM;
+  const dim3 grid(lower ? (unsigned)((long)g.ntile * (g.ntile + 1) / 2) : (M + TM - 1) / TM,
+                  lower ? 1u : (unsigned)((N + TM - 1) / TM), ksplit);
   const double flops = 2.0 * M * (double)N * K * (lower ? 0.5 : 1.0);
   const double bytes = 8.0 * ((double)M * K + (double)K * N +
                               (beta != 0.0 ? 2.0 : 1.0) * M * N * (lower ? 0.5 : 1.0));
