@@ -34,7 +34,9 @@ CONFIGS = {
     "c0": ("c0.job", "ab2_c0.par"),
     "c1": ("c1.job", "ab2_c0.par"),
     "c3": ("c3.job", "ab2_c3.par"),
+    "c4": ("c1.job", "ab2_c0.par"),   # 256 seeded perturbations of c1, 300 levels each
 }
+C4_LEVELS = 300
 TOL_CM = 1e-3
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
@@ -133,6 +135,68 @@ def roofline_of(kprof):
     return roof
 
 
+def perturbed_par(base, seed):
+    """PES parameters (De, a, re, kb, theta_e, c) each scaled by 1 + N(0, 0.01)."""
+    rng = np.random.default_rng(seed)
+    f = 1.0 + 0.01 * rng.standard_normal(6)
+    return f
+
+
+def sweep(args):
+    """c4: seeded perturbations of the c1 potential solved one after another
+    in one context (the plain dense path: at N3 = 4000 it is the faster
+    form), lowest 300 levels each.  Levels of the first instances are checked
+    against the FP64 oracle run on the same perturbed potential."""
+    import math
+    from dvr3d_amd import _lib, host
+    from oracle import dvr3d_oracle as O
+    job = host.read_job(os.path.join(REPO, "oracle", "jobs", "c1.job"))
+    b = host.Basis(job)
+    lines = [l for l in open(os.path.join(REPO, "oracle", "jobs", "ab2_c0.par")).read().split("\n")
+             if l.strip()]
+    de, a, re_ = (float(v) for v in lines[1].split()[:3])
+    kb, te, c = (float(v) for v in lines[2].split()[:3])
+    base = np.array([de, a, re_, kb, te, c])
+    dims = b.dims(nvec=0, solver=_lib.DVR_SOLVER_DENSE)
+    dims["neval"] = C4_LEVELS
+    ctx = _lib.Context(dims)
+    na = b.sz.npnta
+    ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, None,
+                  job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
+    xa = b.xalf[:na].copy()
+
+    def params(i):
+        p = base * perturbed_par(base, 1000 + i)
+        return p, np.array([p[0], p[1], p[2], p[3],
+                            math.cos(p[4] * (4.0 * math.atan(1.0)) / 180.0), p[5]])
+
+    n = args.instances
+    ctx.eval_potential_ab2(b.rc, b.rb, xa, params(0)[1])
+    ctx.run()                                   # warm-up (allocations)
+    levels = []
+    t0 = time.perf_counter()
+    for i in range(n):
+        ctx.eval_potential_ab2(b.rc, b.rb, xa, params(i)[1])
+        ctx.run()
+        if i < args.check:
+            levels.append(ctx.eigenvalues().copy())
+    dt = time.perf_counter() - t0
+    ojob = O.load_job(os.path.join(REPO, "oracle", "jobs", "c1.job"))
+    ojob["neval"] = C4_LEVELS
+    errs = []
+    for i in range(min(args.check, n)):
+        p = params(i)[0]
+        pes = O.PES(1, *p)
+        ref = O.run(ojob, pes)["eval"][:C4_LEVELS]
+        errs.append(float(np.abs(levels[i] - ref).max() * O.AUTOCM))
+    print(json.dumps({
+        "metric": "solves/s (c4 sweep)", "value": round(n / dt, 3), "unit": "solves/s",
+        "n_gpus": 1, "instances": n, "ms_per_solve": round(dt / n * 1e3, 3),
+        "levels_per_instance": C4_LEVELS, "higher_is_better": True, "dtype": "f64",
+        "data": "synthetic", "solver": "dense",
+        "checked_instances": len(errs), "max_err_cm_per_checked_instance": errs}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +204,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--solver", default="davidson", choices=["dense", "davidson"])
+    ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
+    ap.add_argument("--check", type=int, default=2,
+                    help="c4: instances checked against the oracle on the host")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain FP64 dense path run beside the chosen form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -174,6 +241,9 @@ def main():
                                    "(multi-GPU sharding is described in DESIGN.md, not built)"}))
         sys.exit(2)
 
+    if args.config == "c4":
+        sweep(args)
+        return
     from dvr3d_amd import _lib, host
     cfg = args.config
     job = host.read_job(os.path.join(REPO, "oracle", "jobs", CONFIGS[cfg][0]))

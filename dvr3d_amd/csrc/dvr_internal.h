@@ -522,6 +522,8 @@ struct LdlFactor {
   void solve_graph(double* R, double* T, int nrhs, hipStream_t s);
   bool use_graph = true;
   GemmTuner* tuner = nullptr;   // per-shape GEMM choices (nullptr: fixed rule)
+  hipStream_t aux = nullptr;    // trailing updates overlapped with the next panel
+  hipEvent_t ev_panel = nullptr, ev_trail = nullptr;
   ~LdlFactor();
 
  private:

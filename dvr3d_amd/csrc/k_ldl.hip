@@ -135,7 +135,7 @@ void LdlFactor::ensure(int n_) {
   d.ensure(sizeof(double) * (size_t)n);
   linv64.ensure(sizeof(double) * (size_t)((n + NB - 1) / NB) * NB * NB);
   linvsb.ensure(sizeof(double) * (size_t)nsb * SB * SB);
-  w.ensure(sizeof(double) * (size_t)n * 2 * NB);
+  w.ensure(sizeof(double) * (size_t)n * 4 * NB);
   hd.resize(n);
 }
 
@@ -146,11 +146,24 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
              dim3(256), 0, s, H, a, (long)n, sigma);
   double* dd = d.as<double>();
   double* l64 = linv64.as<double>();
-  double* W = w.as<double>();   // n x 128, row index = global row
+  // two n x 128 panel buffers (row index = global row), alternating between
+  // panels: the trailing update of panel p still reads its own while the
+  // panel operations of panel p+1 fill the other
+  double* Wbuf[2] = {w.as<double>(), w.as<double>() + (size_t)n * 2 * NB};
   auto linv_of = [&](int k0) { return l64 + (size_t)(k0 / NB) * NB * NB; };
-  // panels of 2 NB columns: the second half is updated by the first half
-  // only, the trailing matrix once per panel with K = 2 NB
-  for (int p0 = 0; p0 < n; p0 += 2 * NB) {
+  if (!aux) {
+    DVR_HIP(hipStreamCreate(&aux));
+    DVR_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+    DVR_HIP(hipEventCreateWithFlags(&ev_trail, hipEventDisableTiming));
+  }
+  // Panels of 2 NB columns: the second half is updated by the first half
+  // only.  The trailing update is split in two: the next panel's columns (on
+  // the main stream, so that the next panel can start at once) and the rest
+  // (on the auxiliary stream, overlapping the next panel's operations).
+  bool trail_pending = false;
+  int ip = 0;
+  for (int p0 = 0; p0 < n; p0 += 2 * NB, ip ^= 1) {
+    double* W = Wbuf[ip];
     const int nba = std::min(NB, n - p0);
     DVR_LAUNCH("ldl_diag", 16.0 * nba * nba, nba * (double)nba * nba / 3.0, k_ldl_diag, dim3(1),
                dim3(256), 0, s, a, lda, p0, nba, dd, linv_of(p0));
@@ -178,14 +191,28 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
                 n, 1, nullptr, false);
     DVR_LAUNCH("ldl_scale", 24.0 * mb * nbb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)mb * nbb)),
                dim3(256), 0, s, L21b, lda, Wb, (long)n, mb, nbb, dd + k1);
-    // A22 -= [L21a L21b] [W21a W21b]^T over the rows below the panel
-    if (tuner)
-      tuner->run(s, "ldl_update", false, true, mb, mb, nba + nbb, -1.0, a + r2 + (long)p0 * lda,
-                 lda, W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, true);
-    else
-      launch_gemm(s, "ldl_update", false, true, mb, mb, nba + nbb, -1.0, a + r2 + (long)p0 * lda,
-                  lda, W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, 1, nullptr, true);
+    // A22 -= [L21a L21b] [W21a W21b]^T over the rows below the panel:
+    // first the next panel's columns ...
+    const int K = nba + nbb;
+    const int n1 = std::min(2 * NB, mb);
+    if (trail_pending) DVR_HIP(hipStreamWaitEvent(s, ev_trail, 0));   // rest of panel p-1
+    launch_gemm(s, "ldl_update_next", false, true, mb, n1, K, -1.0, a + r2 + (long)p0 * lda, lda,
+                W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, 1, nullptr, false);
+    // ... then the rest, on the auxiliary stream
+    const int m2 = mb - n1;
+    trail_pending = false;
+    if (m2 > 0) {
+      const int r3 = r2 + n1;
+      DVR_HIP(hipEventRecord(ev_panel, s));
+      DVR_HIP(hipStreamWaitEvent(aux, ev_panel, 0));
+      launch_gemm(aux, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda, lda,
+                  W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, 1, nullptr, true);
+      DVR_HIP(hipEventRecord(ev_trail, aux));
+      trail_pending = true;
+    }
   }
+  if (trail_pending) DVR_HIP(hipStreamWaitEvent(s, ev_trail, 0));
+  double* W = Wbuf[0];
   // inverse of every SB x SB diagonal super-block (unit lower), from the
   // NB-level factors: X = L_sb^-1 by forward substitution on the identity
   double* lsb = linvsb.as<double>();
@@ -276,7 +303,12 @@ void LdlFactor::clear_graphs() {
   graphs.clear();
 }
 
-LdlFactor::~LdlFactor() { clear_graphs(); }
+LdlFactor::~LdlFactor() {
+  clear_graphs();
+  if (aux) (void)hipStreamDestroy(aux);
+  if (ev_panel) (void)hipEventDestroy(ev_panel);
+  if (ev_trail) (void)hipEventDestroy(ev_trail);
+}
 
 void LdlFactor::solve_graph(double* R, double* T, int nrhs, hipStream_t s) {
   if (!use_graph || g_prof) {
