@@ -26,7 +26,7 @@ STAGE_NAMES = ("1d", "2d_build", "2d_eig", "2d_vec", "3d_build", "3d_eig", "3d_v
 
 EXPORTS = (
     "dvr_ctx_create", "dvr_ctx_destroy", "dvr_last_error", "dvr_set_basis",
-    "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_get_sizes",
+    "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_run_blocks", "dvr_get_sizes",
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
     "dvr_diag", "dvr_syev_batch", "dvr_set_profiling", "dvr_get_profile",
@@ -75,6 +75,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_set_potential.argtypes = [P, D]
     lib.dvr_eval_potential_ab2.argtypes = [P, D, D, D, D]
     lib.dvr_run.argtypes = [P]
+    lib.dvr_run_blocks.argtypes = [C.POINTER(P), C.c_int32]
     lib.dvr_get_sizes.argtypes = [P, I, I, I]
     lib.dvr_get_eigenvalues.argtypes = [P, D, I]
     lib.dvr_get_1d_eigenvalues.argtypes = [P, D]
@@ -252,6 +253,20 @@ class Context:
         self._check(self.lib.dvr_get_solver_stats(
             self.h, out.ctypes.data_as(C.POINTER(C.c_int32)), len(SOLVER_STATS)))
         return dict(zip(SOLVER_STATS, out.tolist()))
+
+
+def run_blocks(ctxs):
+    """Run several contexts (the k blocks of one J > 0 run) concurrently."""
+    lib = load()
+    arr = (C.c_void_p * len(ctxs))(*[c.h for c in ctxs])
+    rc = lib.dvr_run_blocks(arr, len(ctxs))
+    for c in ctxs:
+        if rc != DVR_OK:
+            msg = lib.dvr_last_error(c.h)
+            if msg:
+                raise DvrError(rc, msg.decode())
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_run_blocks failed")
 
 
 def syev_batch(mats, nvec=None, vectors=True):

@@ -56,18 +56,21 @@ def build_host(verbose=False):
     fc = ["amdflang", "-O2", "-fPIC", "-module-dir", mod]
     setup_o = os.path.join(BUILD, "dvrh_setup.o")
     api_o = os.path.join(BUILD, "dvr3d_amd_api.o")
-    potv_o = os.path.join(BUILD, "potv_ab2.o")
+    potv_o = os.path.join(BUILD, "potv_example.o")
+    noise_o = os.path.join(BUILD, "potv_noise.o")
     main_o = os.path.join(BUILD, "dvr3drjz_amd.o")
     _run(fc + ["-c", os.path.join(HOST, "dvrh_setup.f90"), "-o", setup_o], verbose=verbose)
     _run(fc + ["-c", os.path.join(HOST, "dvr3d_amd_api.f90"), "-o", api_o], verbose=verbose)
-    _run(fc + ["-c", os.path.join(HOST, "potv_ab2.f90"), "-o", potv_o], verbose=verbose)
+    _run(fc + ["-c", os.path.join(HOST, "potv_example.f90"), "-o", potv_o], verbose=verbose)
+    _run(["gcc", "-O2", "-fPIC", "-c", os.path.join(HOST, "potv_noise.c"), "-o", noise_o],
+         verbose=verbose)
     _run(fc + ["-c", os.path.join(HOST, "dvr3drjz_amd.f90"), "-o", main_o], verbose=verbose)
     lib = os.path.join(HERE, "libdvr3d_host.so")
     _run(["amdflang", "-shared", setup_o, "-o", lib + ".tmp"], verbose=verbose)
     os.replace(lib + ".tmp", lib)
     exe = os.path.join(HOST, "dvr3drjz_amd")
     rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
-    _run(["amdflang", main_o, setup_o, api_o, potv_o, "-L" + HERE, "-ldvr3d_amd",
+    _run(["amdflang", main_o, setup_o, api_o, potv_o, noise_o, "-lm", "-L" + HERE, "-ldvr3d_amd",
           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + rocm_lib, "-o", exe], verbose=verbose)
     return lib, exe
 

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -644,6 +645,26 @@ int dvr_run(dvr_ctx_t* c) {
     return fail(c, DVR_ERR_HIP, e.what());
   }
   c->ran = true;
+  return DVR_OK;
+}
+
+// The k blocks of one rotational run are independent problems of different
+// sizes (the ccmain loop, D:857-1007, solves them one after another).  Each
+// context owns its stream and buffers, so one host thread per context lets
+// the blocks share the GPU: a small block's launches fill the gaps of a
+// larger one's.
+int dvr_run_blocks(dvr_ctx_t* const* ctxs, int n) {
+  if (!ctxs || n < 0) return DVR_ERR_ARG;
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i]) return DVR_ERR_ARG;
+  if (n == 1) return dvr_run(ctxs[0]);
+  std::vector<int> rc(n, DVR_OK);
+  std::vector<std::thread> th;
+  th.reserve(n);
+  for (int i = 0; i < n; ++i) th.emplace_back([&, i] { rc[i] = dvr_run(ctxs[i]); });
+  for (auto& t : th) t.join();
+  for (int i = 0; i < n; ++i)
+    if (rc[i] != DVR_OK) return rc[i];
   return DVR_OK;
 }
 

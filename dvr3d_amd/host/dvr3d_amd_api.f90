@@ -47,6 +47,11 @@ module dvr3d_amd_api
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
     end function
+    integer(c_int) function dvr_run_blocks(ctxs, n) bind(C, name="dvr_run_blocks")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int), value :: n
+    end function
     integer(c_int) function dvr_get_sizes(ctx, iv1, iv2, nham3) bind(C, name="dvr_get_sizes")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -1,15 +1,21 @@
 ! dvr3drjz_amd -- drop-in replacement for the DVR3DRJZ executable on the
-! theta-first jhmain path (J = 0, or J > 0 at a fixed k with zrot = F),
-! running the Hamiltonian stages on one MI355X through libdvr3d_amd.
+! theta-first jhmain path (J = 0; J > 0 at a fixed k with zrot = F; J > 0 with
+! zrot = T over the k blocks of ccmain), running the Hamiltonian stages on one
+! MI355X through libdvr3d_amd.
 !
 ! Same contract as the reference program (source/dvr3drjz_segmented.f90):
 !   stdin   the .job deck (namelist &PRT + cards, docum.tex:72-235)
 !   link    the user's SUBROUTINE POTV(V,R1,R2,XCOS) with COMMON /MASS/
-!   stdout  the eigenvalue report of diag3d (D:2703-2761 formats)
+!   stdout  the eigenvalue report of diag3d per k block (D:2703-2761 formats)
 !   ILEV    formatted levels when zpfun (D:469-471, D:2708-2719)
 !   IWAVE   unformatted wavefunction file when ztran (records of setcon
-!           D:1178-1180, ccmain D:823-845 and D:973-977, trans D:3733-3841),
+!           D:1162-1180, ccmain D:820-845 and D:973-977, trans D:3733-3841),
 !           in the compiler's default sequential record format.
+!
+! The k blocks of a zrot run (ccmain loop, D:857-1007) are independent: every
+! block gets its own device context, all of them run concurrently
+! (dvr_run_blocks), and the reports and IWAVE records are written afterwards
+! in k order, as the reference writes them.
 program dvr3drjz_amd
   use iso_c_binding
   use dvrh_setup
@@ -19,14 +25,15 @@ program dvr3drjz_amd
   type(dvrh_sizes_t) :: sz
   type(dvrh_consts_t) :: cst
   type(dvr_dims_t) :: dims
-  type(c_ptr) :: ctx
-  real(c_double), allocatable :: r1(:), r2(:), xk1(:, :), xk2(:, :), xalf(:), walf(:)
-  real(c_double), allocatable :: pleg(:, :), xl(:, :), wsum(:), wterm(:), vgrid(:, :, :)
+  type(c_ptr), allocatable :: ctx(:)
+  real(c_double), allocatable :: r1(:), r2(:), xk1(:, :), xk2(:, :), xalf(:, :), walf(:, :)
+  real(c_double), allocatable :: pleg(:, :, :), xl(:, :), wsum(:), wterm(:), vgrid(:, :, :)
   real(c_double), allocatable :: xla(:, :), eval(:), evalcm(:), psi(:), xkb(:, :), xkc(:, :)
-  real(c_double) :: xmass(3), g1, g2, xmassr(3), v, ezero, ms(7)
+  real(c_double) :: xmass(3), g1, g2, xmassr(3), v, ezero, ms(7), mst(7)
   common /mass/ xmass, g1, g2, xmassr
   character(len=72) :: title
-  integer :: ierr, i, a, b, c, kz, meval, ii, jdia, jpar, isym, ip, jstart, nang, mbass
+  integer :: ierr, i, a, b, c, kz, ik, nk, meval, ii, jdia, jpar, isym, ip, jstart, nang
+  integer :: mbass, nlim, lincr
   integer(c_int) :: rc
   integer(8) :: t0, t1, trate
   logical :: zpfun, ztran, zembed, zmors1, zmors2, zncor, zquad2
@@ -51,42 +58,28 @@ program dvr3drjz_amd
             /5x,i5,3x,'angular dvr points used, with',/5x,i5,3x,'lowest eigenvectors chosen from', &
             /5x,i5,3x,'maximum dimension secular problem'/)
   write(6, '(5x,a,a)') 'Title: ', title
-  kz = sz%kmin
-  if (sz%zrot /= 0 .and. sz%ku > sz%kd) then
-    write(6, '(5x,a)') 'zrot k-loop over several k blocks is not available in this build'
+  nk = sz%ku - sz%kd + 1
+  ! with zladd = F the angular basis shrinks with k (D:930-950): not built
+  if (nk > 1 .and. job%zladd == 0) then
+    write(6, '(5x,a)') 'zrot k-loop with zladd = F is not available in this build'
     stop 1
   end if
+  if (sz%jrot > 0 .and. sz%zrot /= 0) write(6, '(/5x,a/5x,a,i4,a,i3,a,i3)') &
+      '***  vibrational part of rot-vib calculation  ***', 'J =', sz%jrot, ',  k =', sz%kd, &
+      ' ..', sz%ku
 
   xmass = job%xmass; xmassr = job%xmassr
-  allocate(r1(job%npnt1), r2(job%npnt2), xk1(job%npnt1, job%npnt1), xk2(job%npnt2, job%npnt2))
-  allocate(xalf(sz%idvr), walf(sz%idvr), pleg(0:sz%maxleg, sz%idvr), xl(sz%idvr, sz%idvr))
-  ierr = dvrh_setup_basis(job, sz, kz, r1, r2, xk1, xk2, xalf, walf, pleg, xl, cst)
-  if (ierr /= 0) then
-    write(6, '(//5x,a,i4//)') 'points & weights in error, adjust algorithm; code', ierr
-    stop 1
-  end if
-  g1 = cst%g1; g2 = cst%g2
-  allocate(wsum(sz%npntb * sz%npntc), wterm(sz%npntb * sz%npntc))
-  ierr = dvrh_wsum(job, sz, cst, kz, r1, r2, wsum, wterm)
-
-  ! potential on the grid through the user's POTV (mkham1, D:2178)
-  allocate(vgrid(sz%npnta, sz%npntb, sz%npntc))
-  do c = 1, sz%npntc
-    do b = 1, sz%npntb
-      do a = 1, sz%npnta
-        if (job%zr2r1 /= 0) then
-          call potv(v, r1(c), r2(b), xalf(a))
-        else
-          call potv(v, r1(b), r2(c), xalf(a))
-        end if
-        vgrid(a, b, c) = v
-      end do
-    end do
-  end do
-
   ztran = job%ztran /= 0
   zpfun = job%zpfun /= 0
   if (abs(job%jrot) > 1) zpfun = .false.        ! D:466
+  allocate(r1(job%npnt1), r2(job%npnt2), xk1(job%npnt1, job%npnt1), xk2(job%npnt2, job%npnt2))
+  allocate(xalf(sz%idvr, nk), walf(sz%idvr, nk), pleg(0:sz%maxleg, sz%idvr, nk))
+  allocate(xl(sz%idvr, sz%idvr), wsum(sz%npntb * sz%npntc), wterm(sz%npntb * sz%npntc))
+  allocate(vgrid(sz%npnta, sz%npntb, sz%npntc))
+  allocate(xla(sz%npnta, sz%npnta), xkb(sz%npntb, sz%npntb), xkc(sz%npntc, sz%npntc))
+  allocate(ctx(nk))
+  ctx = c_null_ptr
+
   dims%npnta = sz%npnta; dims%npntb = sz%npntb; dims%npntc = sz%npntc
   dims%max2d = sz%max2d; dims%max3d = sz%max3d; dims%neval = sz%neval
   dims%zcut = job%zcut; dims%zall = job%zall; dims%zr2r1 = job%zr2r1
@@ -94,36 +87,68 @@ program dvr3drjz_amd
   dims%nvec = 0
   if (ztran) dims%nvec = sz%neval
   dims%solver = DVR_SOLVER_DENSE
-  rc = dvr_ctx_create(dims, 0_c_int, ctx)
-  if (rc /= DVR_OK) then
-    write(6, '(5x,a,i4)') 'dvr_ctx_create failed, code', rc
-    stop 1
-  end if
-  allocate(xla(sz%npnta, sz%npnta), xkb(sz%npntb, sz%npntb), xkc(sz%npntc, sz%npntc))
-  xla = xl(1:sz%npnta, 1:sz%npnta)
-  if (job%zr2r1 /= 0) then
-    xkb = xk2; xkc = xk1
-  else
-    xkb = xk1; xkc = xk2
-  end if
-  rc = dvr_set_basis(ctx, xla, xkb, xkc, wsum, wterm, job%emax1 / autocm, job%emax2 / autocm)
-  if (rc == DVR_OK) rc = dvr_set_potential(ctx, vgrid)
-  if (rc == DVR_OK) rc = dvr_run(ctx)
-  if (rc == DVR_ERR_CORE_2D .or. rc == DVR_ERR_CORE_3D) then
-    write(6, '(//6x,a)') trim(dvr_error_text(ctx))
-    stop
-  else if (rc /= DVR_OK) then
-    write(6, '(5x,a,i4,2x,a)') 'device pipeline failed, code', rc, trim(dvr_error_text(ctx))
-    stop 1
-  end if
-  allocate(eval(sz%neval), evalcm(sz%neval))
-  rc = dvr_get_eigenvalues(ctx, eval, meval)
 
-  ! ---- report (diag3d, D:2703-2761)
-  if (job%zpmin == 0) then
-    write(6, 1010) meval
-    write(6, 1020) (eval(i), i = 1, meval)
+  ! ---- one context per k block: basis, rotational term, potential
+  do ik = 1, nk
+    kz = sz%kd + ik - 1
+    ierr = dvrh_setup_basis(job, sz, kz, r1, r2, xk1, xk2, xalf(:, ik), walf(:, ik), &
+                            pleg(:, :, ik), xl, cst)
+    if (ierr /= 0) then
+      write(6, '(//5x,a,i4//)') 'points & weights in error, adjust algorithm; code', ierr
+      stop 1
+    end if
+    g1 = cst%g1; g2 = cst%g2
+    ierr = dvrh_wsum(job, sz, cst, kz, r1, r2, wsum, wterm)
+    ! potential on the grid through the user's POTV (mkham1, D:2178)
+    do c = 1, sz%npntc
+      do b = 1, sz%npntb
+        do a = 1, sz%npnta
+          if (job%zr2r1 /= 0) then
+            call potv(v, r1(c), r2(b), xalf(a, ik))
+          else
+            call potv(v, r1(b), r2(c), xalf(a, ik))
+          end if
+          vgrid(a, b, c) = v
+        end do
+      end do
+    end do
+    rc = dvr_ctx_create(dims, 0_c_int, ctx(ik))
+    if (rc /= DVR_OK) then
+      write(6, '(5x,a,i4)') 'dvr_ctx_create failed, code', rc
+      stop 1
+    end if
+    xla = xl(1:sz%npnta, 1:sz%npnta)
+    if (job%zr2r1 /= 0) then
+      xkb = xk2; xkc = xk1
+    else
+      xkb = xk1; xkc = xk2
+    end if
+    rc = dvr_set_basis(ctx(ik), xla, xkb, xkc, wsum, wterm, job%emax1 / autocm, &
+                       job%emax2 / autocm)
+    if (rc == DVR_OK) rc = dvr_set_potential(ctx(ik), vgrid)
+    if (rc /= DVR_OK) then
+      write(6, '(5x,a,i4,2x,a)') 'device set-up failed, code', rc, trim(dvr_error_text(ctx(ik)))
+      stop 1
+    end if
+  end do
+
+  rc = dvr_run_blocks(ctx, int(nk, c_int))
+  if (rc /= DVR_OK) then
+    do ik = 1, nk
+      if (len_trim(dvr_error_text(ctx(ik))) == 0) cycle
+      if (rc == DVR_ERR_CORE_2D .or. rc == DVR_ERR_CORE_3D) then
+        write(6, '(//6x,a)') trim(dvr_error_text(ctx(ik)))
+        stop
+      end if
+      write(6, '(5x,a,i3,a,i4,2x,a)') 'device pipeline failed at k =', sz%kd + ik - 1, &
+          ', code', rc, trim(dvr_error_text(ctx(ik)))
+      stop 1
+    end do
+    write(6, '(5x,a,i4)') 'device pipeline failed, code', rc
+    stop 1
   end if
+
+  ! ---- ILEV (opened once, D:467-478), IWAVE set-up records (D:1162-1180, D:820-845)
   if (zpfun) then
     open(unit=job%ilev, form='formatted')
     if (sz%jrot == 0 .and. mod(sz%ipar, 2) == 0) then
@@ -134,71 +159,109 @@ program dvr3drjz_amd
         if (ierr /= 0) exit
       end do
     end if
-    ip = sz%jrot - sz%kmin
-    if (sz%jrot == 0 .or. ip == 1) then
-      jdia = max(0, sz%idia)
-      jpar = min(jdia, sz%ipar)
-      isym = abs(min(0, sz%idia))
-      if (sz%ipar == 1) isym = -isym
-      write(job%ilev, '(6i4)') sz%jrot, ip, jdia, jpar, isym, meval
-      write(job%ilev, '(4d20.12)') (eval(i), i = 1, meval)
-    end if
-    close(job%ilev)
   end if
-  do i = 1, meval
-    evalcm(i) = eval(i) * autocm
-  end do
-  write(6, 1031) meval
-  write(6, 1020) (evalcm(i), i = 1, meval)
-  if (sz%jrot == 0) then
-    if (abs(sz%idia) == 2 .and. sz%ipar == 1) then
-      ii = 1
-      ezero = job%ezero
+  if (ztran) then
+    zembed = sz%zembed /= 0; zmors1 = job%zmors1 /= 0; zmors2 = job%zmors2 /= 0
+    zncor = sz%zrot == 0; zquad2 = job%zquad2 /= 0
+    nlim = 0
+    if (sz%zrot /= 0) then
+      if (zembed) then
+        nlim = job%npnt2
+      else
+        nlim = job%npnt1
+      end if
+    end if
+    open(unit=job%iwave, form='unformatted')
+    write(job%iwave) sz%idia, sz%ipar, sz%idvr, job%npnt1, job%npnt2, sz%jrot, sz%kmin, &
+                     sz%neval, nlim
+    write(job%iwave) zembed, zmors1, zmors2, xmass, g1, g2, zncor, zquad2
+    write(job%iwave) job%re1, job%diss1, job%we1, job%re2, job%diss2, job%we2
+    if (nk > 1) then
+      ! r**(-2) term for the second variational step (D:823-833)
+      if (zembed) then
+        write(job%iwave) (0.5d0 / (r2(i) * r2(i) * cst%urr2), i = 1, job%npnt2)
+      else
+        write(job%iwave) (0.5d0 / (r1(i) * r1(i) * cst%urr1), i = 1, job%npnt1)
+      end if
     else
-      ezero = evalcm(1)
-      ii = 2
+      jdia = max(1, sz%idia)
+      jstart = sz%kmin
+      if (mod(jstart, jdia) /= sz%ipar) jstart = jstart + 1
+      nang = (sz%maxleg - jstart) / jdia + 1
+      mbass = sz%idvr * job%npnt1 * job%npnt2
+      write(job%iwave) mbass, jstart, nang, mbass
     end if
-    write(6, 1070)
-    write(6, 1021) (evalcm(i) - ezero, i = ii, meval)
+    write(job%iwave) r1
+    write(job%iwave) r2
   end if
+
+  ! ---- per k block: report (diag3d, D:2703-2761) and IWAVE records
+  allocate(eval(sz%neval), evalcm(sz%neval))
+  mst = 0.0d0
+  do ik = 1, nk
+    kz = sz%kd + ik - 1
+    if (nk > 1) write(6, '(//5x,a,i3)') 'k block, k =', kz
+    rc = dvr_get_eigenvalues(ctx(ik), eval, meval)
+    if (job%zpmin == 0) then
+      write(6, 1010) meval
+      write(6, 1020) (eval(i), i = 1, meval)
+    end if
+    if (zpfun) then
+      ip = sz%jrot - sz%kmin
+      if (sz%jrot == 0 .or. ip == 1) then
+        jdia = max(0, sz%idia)
+        jpar = min(jdia, sz%ipar)
+        isym = abs(min(0, sz%idia))
+        if (sz%ipar == 1) isym = -isym
+        write(job%ilev, '(6i4)') sz%jrot, ip, jdia, jpar, isym, meval
+        write(job%ilev, '(4d20.12)') (eval(i), i = 1, meval)
+      end if
+    end if
+    do i = 1, meval
+      evalcm(i) = eval(i) * autocm
+    end do
+    write(6, 1031) meval
+    write(6, 1020) (evalcm(i), i = 1, meval)
+    if (sz%jrot == 0) then
+      if (abs(sz%idia) == 2 .and. sz%ipar == 1) then
+        ii = 1
+        ezero = job%ezero
+      else
+        ezero = evalcm(1)
+        ii = 2
+      end if
+      write(6, 1070)
+      write(6, 1021) (evalcm(i) - ezero, i = ii, meval)
+    end if
+    if (ztran) then
+      ! angular records of ccmain (D:973-977) then trans (D:3733-3841)
+      lincr = kz
+      write(job%iwave) xalf(:, ik)
+      write(job%iwave) kz, sz%maxleg, sz%idvr, lincr
+      write(job%iwave) ((pleg(i, a, ik) * walf(a, ik), i = 0, sz%maxleg), a = 1, sz%idvr)
+      write(job%iwave) meval
+      write(job%iwave) (eval(i), i = 1, meval)
+      mbass = sz%idvr * job%npnt1 * job%npnt2
+      if (.not. allocated(psi)) allocate(psi(int(mbass, 8) * sz%neval))
+      rc = dvr_get_wavefunctions(ctx(ik), psi)
+      do i = 1, meval
+        write(job%iwave) psi(int(i - 1, 8) * mbass + 1:int(i, 8) * mbass)
+      end do
+    end if
+    rc = dvr_get_stage_times(ctx(ik), ms)
+    mst = mst + ms
+  end do
 1010 format(/5x,'lowest',i4,' eigenvalues in hartrees:',/)
 1031 format(//5x,'Lowest',i4,' eigenvalues in wavenumbers:'/)
 1020 format(5d24.12/)
 1021 format(5f15.6/)
 1070 format(//'  Band origins in wavenumbers:'/)
-
-  ! ---- IWAVE (ztran)
-  if (ztran) then
-    zembed = sz%zembed /= 0; zmors1 = job%zmors1 /= 0; zmors2 = job%zmors2 /= 0
-    zncor = sz%zrot == 0; zquad2 = job%zquad2 /= 0
-    open(unit=job%iwave, form='unformatted')
-    write(job%iwave) sz%idia, sz%ipar, sz%idvr, job%npnt1, job%npnt2, sz%jrot, sz%kmin, sz%neval, 0
-    write(job%iwave) zembed, zmors1, zmors2, xmass, g1, g2, zncor, zquad2
-    write(job%iwave) job%re1, job%diss1, job%we1, job%re2, job%diss2, job%we2
-    jdia = max(1, sz%idia)
-    jstart = sz%kmin
-    if (mod(jstart, jdia) /= sz%ipar) jstart = jstart + 1
-    nang = (sz%maxleg - jstart) / jdia + 1
-    mbass = sz%idvr * job%npnt1 * job%npnt2
-    write(job%iwave) mbass, jstart, nang, mbass
-    write(job%iwave) r1
-    write(job%iwave) r2
-    write(job%iwave) xalf
-    write(job%iwave) kz, sz%maxleg, sz%idvr, kz
-    write(job%iwave) ((pleg(i, a) * walf(a), i = 0, sz%maxleg), a = 1, sz%idvr)
-    write(job%iwave) meval
-    write(job%iwave) (eval(i), i = 1, meval)
-    allocate(psi(int(sz%idvr, 8) * job%npnt1 * job%npnt2 * meval))
-    rc = dvr_get_wavefunctions(ctx, psi)
-    mbass = sz%idvr * job%npnt1 * job%npnt2
-    do i = 1, meval
-      write(job%iwave) psi(int(i - 1, 8) * mbass + 1:int(i, 8) * mbass)
-    end do
-    close(job%iwave)
-  end if
-  rc = dvr_get_stage_times(ctx, ms)
-  write(6, '(/5x,a,7f10.3)') 'device stage times (ms):', ms
-  call dvr_ctx_destroy(ctx)
+  if (zpfun) close(job%ilev)
+  if (ztran) close(job%iwave)
+  write(6, '(/5x,a,7f10.3)') 'device stage times (ms):', mst
+  do ik = 1, nk
+    call dvr_ctx_destroy(ctx(ik))
+  end do
   call system_clock(t1)
   write(6, '(/i10,a/)') int((t1 - t0) / trate), ' secs CPU time used'
 end program dvr3drjz_amd

@@ -105,6 +105,14 @@ int dvr_eval_potential_ab2(dvr_ctx_t* ctx, const double* rc, const double* rb,
 /* Run 1D -> 2D -> selection -> 3D -> final solve (-> wavefunctions). */
 int dvr_run(dvr_ctx_t* ctx);
 
+/* dvr_run on n contexts at once (the k blocks of a J > 0 run with zrot:
+ * the ccmain k loop, source/dvr3drjz_segmented.f90:857-1007, which calls
+ * jhmain once per k).  Every context must be distinct and ready to run; they
+ * execute concurrently on their own streams.  Returns DVR_OK, or the status
+ * of the first context (in array order) that failed; each context keeps its
+ * own error text. */
+int dvr_run_blocks(dvr_ctx_t* const* ctxs, int n);
+
 /* Truncation results: iv1 as the reference's iv1(ndimc,ndimb) (column-major,
  * iv1[c + npntc*b]), iv2[npntc], nham3 (D:1997, D:2053, D:2086). */
 int dvr_get_sizes(const dvr_ctx_t* ctx, int32_t* iv1, int32_t* iv2,

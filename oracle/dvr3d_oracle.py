@@ -642,12 +642,67 @@ class PES:
         kb, te, c = (float(v) for v in lines[2].split()[:3])
         return PES(kind, de, a, re_, kb, te, c)
 
+    # kind 2 (non-symmetric ABC in Jacobi coordinates, IDIA=1): the fields
+    # are read as de, a, re = r1 Morse minimum, kb, te_deg = A-B Morse
+    # minimum (bohr), c = noise amplitude (Eh); g1 is the Jacobi mass factor
+    # m2/(m2+m3) and NOISE_SEED keys the noise.
+    g1: float = 0.0
+    NOISE_SEED = 20251203
+
     def __call__(self, r1, r2, x):
+        if self.kind == 2:
+            return self._abc(r1, r2, x)
         if self.kind != 1:
             raise ValueError("unknown PES kind")
         m1 = (1.0 - np.exp(-self.a * (r1 - self.re))) ** 2
         m2 = (1.0 - np.exp(-self.a * (r2 - self.re))) ** 2
         return self.de * (m1 + m2) + self.kb * (x - self.cte) ** 2 + self.c * ((r1 - self.re) * (r2 - self.re))
+
+    def _abc(self, r1, r2, x):
+        """Two Morse terms, in r1 (the B-C distance, atoms 2-3) and in the A-B
+        distance d = |atom 1 - atom 2| = sqrt(yy^2 + r2^2 + 2 yy r2 x),
+        yy = r1 (1 - g1) (the water convention of the reference's
+        pot_water.f for g2 = 0), plus kb (x - 1)^2 and index-free noise: a
+        standard normal drawn from a hash of the bit patterns of (r1, r2, x)
+        and NOISE_SEED, times c."""
+        r1, r2, x = np.broadcast_arrays(np.asarray(r1, float), np.asarray(r2, float),
+                                        np.asarray(x, float))
+        yy = r1 * (1.0 - self.g1)
+        d = np.sqrt(yy * yy + r2 * r2 + 2.0 * yy * r2 * x)
+        m1 = (1.0 - np.exp(-self.a * (r1 - self.re))) ** 2
+        m2 = (1.0 - np.exp(-self.a * (d - self.te_deg))) ** 2
+        v = self.de * (m1 + m2) + self.kb * (x - 1.0) ** 2
+        if self.c != 0.0:
+            v = v + self.c * noise_normal(r1, r2, x, self.NOISE_SEED)
+        return v
+
+
+_M64 = (1 << 64) - 1
+
+
+def _splitmix(z):
+    """splitmix64 finaliser on uint64 arrays (wrapping arithmetic)."""
+    z = (z + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(_M64)
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(_M64)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(_M64)
+    return z ^ (z >> np.uint64(31))
+
+
+def noise_normal(r1, r2, x, seed):
+    """Standard normal numbers keyed by the IEEE bit patterns of (r1, r2, x):
+    h = mix(bits(x) ^ mix(bits(r2) ^ mix(bits(r1) ^ seed))), two uniforms
+    from h and mix(h), Box-Muller (cosine branch)."""
+    with np.errstate(over="ignore"):
+        b1 = np.ascontiguousarray(r1, dtype=np.float64).view(np.uint64)
+        b2 = np.ascontiguousarray(r2, dtype=np.float64).view(np.uint64)
+        b3 = np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)
+        h = _splitmix(b1 ^ np.uint64(seed))
+        h = _splitmix(b2 ^ h)
+        h = _splitmix(b3 ^ h)
+        h2 = _splitmix(h)
+    u1 = ((h >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+    u2 = (h2 >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
 
 
 def potential_grid(basis: Basis, pes) -> np.ndarray:
@@ -774,11 +829,32 @@ def cint_block(basis: Basis, vecs1d, iv1_row, c2) -> np.ndarray:
     return out
 
 
-def run(job: dict, pes, keep_vectors: bool = False, neval_vectors: int | None = None):
-    """Full theta-first jhmain contraction for one k block (J=0).
-    Returns a dict of every intermediate that parity tests compare."""
-    basis = setup_basis(job)
+def k_blocks(job: dict):
+    """The k values of the ccmain rotational loop, D:811-819 and 857-865:
+    zrot -> k = 1-min(kmin,1) .. jrot (kmin = 2 adds a second k = 1 block,
+    not restated); otherwise the single block k = kmin."""
+    s = insize(job)
+    if not s.zrot:
+        return [s.kmin]
+    if s.kmin >= 2:
+        raise NotImplementedError("kmin=2 (both parities of k=1) not restated")
+    return list(range(1 - min(s.kmin, 1), s.jrot + 1))
+
+
+def run_all_k(job: dict, pes, **kw):
+    """jhmain for every k block of the run (one result dict per k)."""
+    return [run(job, pes, kz=k, **kw) for k in k_blocks(job)]
+
+
+def run(job: dict, pes, keep_vectors: bool = False, neval_vectors: int | None = None,
+        kz: int | None = None):
+    """Full theta-first jhmain contraction for one k block (default: the
+    first k of the run).  Returns a dict of every intermediate that parity
+    tests compare."""
+    basis = setup_basis(job, kz)
     s = basis.sizes
+    if getattr(pes, "kind", 1) == 2:
+        pes.g1 = basis.consts.g1
     vgrid = potential_grid(basis, pes)
     one = solve_1d(basis, vgrid, job["emax1"])
     iv1 = one["iv1"]

@@ -13,6 +13,11 @@
 !     line 2: De a re         (Eh, 1/bohr, bohr)
 !     line 3: kb thetae_deg c (Eh, degrees, Eh)
 !
+! kind 2 (ABC in Jacobi coordinates, IDIA=1; line 2: De a re1, line 3:
+!   kb re2 noise):  V = De[(1-exp(-a(r1-re1)))^2 + (1-exp(-a(d-re2)))^2]
+!   + kb (cos t - 1)^2 with d = sqrt(yy^2 + r2^2 + 2 yy r2 cos t),
+!   yy = r1 (1 - g1): the atom 1 - atom 2 distance.  Only noise = 0 is
+!   available here (the hashed noise is restated in the numpy oracle).
 ! kind 1 (AB2, Radau or Jacobi coordinates, r1/r2 symmetric):
 !   V = De[(1-exp(-a(r1-re)))^2 + (1-exp(-a(r2-re)))^2]
 !       + kb (cos t - cos te)^2 + c (r1-re)(r2-re)
@@ -29,7 +34,7 @@
       logical, save :: loaded = .false.
       character(len=512) :: fname
       integer :: n, ios
-      double precision :: m1, m2
+      double precision :: m1, m2, yy, dab
       if (.not. loaded) then
          call get_environment_variable('DVR_PES', fname, n, ios)
          if (ios /= 0 .or. n == 0) fname = 'pes.par'
@@ -50,6 +55,16 @@
          m1 = (1.0d0 - exp(-a*(r1-re)))**2
          m2 = (1.0d0 - exp(-a*(r2-re)))**2
          v = de*(m1+m2) + kb*(xcos-cte)**2 + c*((r1-re)*(r2-re))
+      case (2)
+         if (c /= 0.0d0) then
+            write(6,*) 'potv: kind 2 noise is not available in this build'
+            stop
+         endif
+         yy = r1 * (1.0d0 - g1)
+         dab = sqrt(yy*yy + r2*r2 + 2.0d0*yy*r2*xcos)
+         m1 = (1.0d0 - exp(-a*(r1-re)))**2
+         m2 = (1.0d0 - exp(-a*(dab-te)))**2
+         v = de*(m1+m2) + kb*(xcos-1.0d0)**2
       case default
          write(6,*) 'potv: unknown PES kind', kind
          stop

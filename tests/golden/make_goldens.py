@@ -10,9 +10,15 @@ cm^-1 (D:2727-2730, format 5d24.12).  The output is plain JSON data.
 The IWAVE fixture (c0_iwave_ref.npz) holds the header records 1-11 and the
 first two wavefunction records of the reference's fort.26 for c0.
 
+J > 0 decks with zrot (k2s) print one eigenvalue report per k block; their
+JSON holds eval_au per k, and their IWAVE fixture (k2s_iwave_ref.npz) the
+set-up records 1-6 (record 4 is the r**-2 array, D:823-833) and, per k, the
+angular records, the eigenvalues and the first wavefunction record.
+
 usage: python tests/golden/make_goldens.py c0 [c1 ...]
        python tests/golden/make_goldens.py --parse c1 OUTFILE   (existing run)
        python tests/golden/make_goldens.py --iwave c0 FORT26    (IWAVE fixture)
+       python tests/golden/make_goldens.py --jrot k2s           (J > 0 run + IWAVE)
 """
 from __future__ import annotations
 
@@ -27,7 +33,7 @@ import time
 REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 REF = os.path.join(REPO, "oracle", "_ref", "dvr3drjz_ref")
 JOBS = os.path.join(REPO, "oracle", "jobs")
-PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par"}
+PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par", "k2s": "abc_k2s.par"}
 
 
 def _fortran_float(tok: str) -> float:
@@ -84,6 +90,42 @@ def parse_output(text: str) -> dict:
     return out
 
 
+def parse_levels_per_k(text: str) -> list:
+    """The hartree eigenvalue list of every k block, in k order."""
+    blocks = []
+    for part in text.split("eigenvalues in hartrees")[1:]:
+        vals = []
+        for line in part.split("\n")[1:]:
+            if "wavenumbers" in line:
+                break
+            if line.strip():
+                vals += _floats(line)
+        blocks.append(vals)
+    return blocks
+
+
+def write_iwave_fixture_k(cfg: str, path: str, nk: int):
+    import numpy as np
+    sys.path.insert(0, REPO)
+    from tests import fortran_io as F
+    recs = F.read_records(path)
+    out = dict(rec1=F.ints(recs[0]), rec2=np.frombuffer(recs[1], dtype=np.uint8),
+               rec3=F.doubles(recs[2]), rec4_r2m2=F.doubles(recs[3]),
+               rec5_r1=F.doubles(recs[4]), rec6_r2=F.doubles(recs[5]),
+               nrec=np.array([len(recs)]))
+    pos = 6
+    for ik in range(nk):
+        meval = int(F.ints(recs[pos + 3])[0])
+        out[f"k{ik}_xalf"] = F.doubles(recs[pos])
+        out[f"k{ik}_kzrec"] = F.ints(recs[pos + 1])
+        out[f"k{ik}_plegw"] = F.doubles(recs[pos + 2])
+        out[f"k{ik}_meval"] = np.array([meval])
+        out[f"k{ik}_eval"] = F.doubles(recs[pos + 4])
+        out[f"k{ik}_psi1"] = F.doubles(recs[pos + 5])
+        pos += 5 + meval
+    np.savez_compressed(os.path.join(REPO, "tests", "golden", cfg + "_iwave_ref.npz"), **out)
+
+
 def run_config(cfg: str, rundir: str) -> dict:
     os.makedirs(rundir, exist_ok=True)
     shutil.copy(os.path.join(JOBS, cfg + ".job"), os.path.join(rundir, cfg + ".job"))
@@ -119,6 +161,26 @@ def write_iwave_fixture(cfg: str, path: str):
 
 
 def main(argv):
+    if argv and argv[0] == "--jrot":
+        cfg = argv[1]
+        rundir = os.path.join(REPO, "_work", "runs", cfg)
+        os.makedirs(rundir, exist_ok=True)
+        shutil.copy(os.path.join(JOBS, cfg + ".job"), os.path.join(rundir, cfg + ".job"))
+        shutil.copy(os.path.join(JOBS, PES_OF[cfg]), os.path.join(rundir, "pes.par"))
+        with open(os.path.join(rundir, cfg + ".job")) as fi, \
+                open(os.path.join(rundir, cfg + ".out"), "w") as fo:
+            subprocess.run([REF], stdin=fi, stdout=fo, cwd=rundir, check=True)
+        text = open(os.path.join(rundir, cfg + ".out")).read()
+        blocks = parse_levels_per_k(text)
+        jrot = int(open(os.path.join(JOBS, cfg + ".job")).read().split("\n")[2].split()[1])
+        k0 = jrot + 1 - len(blocks)
+        res = dict(source="oracle/_ref/dvr3drjz_ref on oracle/jobs/%s.job with oracle/jobs/%s"
+                   % (cfg, PES_OF[cfg]), jrot=jrot,
+                   eval_au={str(k0 + i): b for i, b in enumerate(blocks)})
+        json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"))
+        write_iwave_fixture_k(cfg, os.path.join(rundir, "fort.26"), len(blocks))
+        print(cfg, "k blocks", len(blocks), "levels", [len(b) for b in blocks])
+        return
     if argv and argv[0] == "--iwave":
         write_iwave_fixture(argv[1], argv[2])
         return

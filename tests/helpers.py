@@ -9,7 +9,8 @@ from oracle import dvr3d_oracle as O
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 JOBS = os.path.join(REPO, "oracle", "jobs")
 GOLD = os.path.join(REPO, "tests", "golden")
-PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par"}
+PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par", "c2": "abc_c2.par",
+          "k2s": "abc_k2s.par"}
 
 
 def job_and_pes(cfg):
@@ -23,9 +24,10 @@ def golden(cfg):
         return json.load(f)
 
 
-def device_inputs(job, pes, nvec=0, solver=0, basis=None):
-    """Inputs of the C ABI computed from the oracle's basis arrays."""
-    b = basis or O.setup_basis(job)
+def device_inputs(job, pes, nvec=0, solver=0, basis=None, kz=None):
+    """Inputs of the C ABI computed from the oracle's basis arrays (one k
+    block; kz defaults to the first block of the run)."""
+    b = basis or O.setup_basis(job, kz)
     s, c = b.sizes, b.consts
     if s.zr2r1:
         rc, rb = b.r1, b.r2
@@ -39,6 +41,18 @@ def device_inputs(job, pes, nvec=0, solver=0, basis=None):
         w2 = 0.5 / (rc * rc * c.ur2)
         wsum = (w1[None, :] + w2[:, None]).ravel()
         xkb, xkc = b.xk1, b.xk2
+    # rotational term of mkham1 for J > 0 (D:2159-2173), same (c, b) layout
+    wterm = None
+    if s.jrot > 0:
+        term = float(s.jrot * s.jrot + s.jrot - 2 * b.kz * b.kz)
+        if s.zr2r1:
+            w1g = np.broadcast_to(w1[:, None], (len(rc), len(rb)))
+            w2b = np.broadcast_to(w2[None, :], (len(rc), len(rb)))
+        else:
+            w1g = np.broadcast_to(w1[None, :], (len(rc), len(rb)))
+            w2b = np.broadcast_to(w2[:, None], (len(rc), len(rb)))
+        wterm = np.ascontiguousarray(
+            (term * w2b * c.ur2 / c.urr2) if s.zembed else (term * w1g * c.ur1 / c.urr1)).ravel()
     dims = dict(npnta=s.npnta, npntb=s.npntb, npntc=s.npntc, max2d=s.max2d,
                 max3d=s.max3d, neval=s.neval, zcut=int(s.zcut), zall=int(s.zall),
                 zr2r1=int(s.zr2r1), idvr=s.idvr, npnt1=s.npnt1, npnt2=s.npnt2,
@@ -49,13 +63,13 @@ def device_inputs(job, pes, nvec=0, solver=0, basis=None):
     return dict(dims=dims, xlmatr=xl, xkb=np.asfortranarray(xkb).ravel(order="F"),
                 xkc=np.asfortranarray(xkc).ravel(order="F"), wsum=wsum, rc=rc, rb=rb,
                 xa=b.xalf[:na], par=par, emax1=job["emax1"] / O.AUTOCM,
-                emax2=job["emax2"] / O.AUTOCM, basis=b)
+                emax2=job["emax2"] / O.AUTOCM, basis=b, wterm=wterm)
 
 
 def run_device(inp, analytic=True, vgrid=None):
     from dvr3d_amd import _lib
     ctx = _lib.Context(inp["dims"])
-    ctx.set_basis(inp["xlmatr"], inp["xkb"], inp["xkc"], inp["wsum"], None,
+    ctx.set_basis(inp["xlmatr"], inp["xkb"], inp["xkc"], inp["wsum"], inp.get("wterm"),
                   inp["emax1"], inp["emax2"])
     if analytic:
         ctx.eval_potential_ab2(inp["rc"], inp["rb"], inp["xa"], inp["par"])

@@ -1,6 +1,7 @@
 """The drop-in executable dvr3drjz_amd against the reference executable's
 outputs on the same .job deck: the stdout eigenvalue report, the ILEV file
 and the IWAVE records that ROTLEV3/DIPOLE3/XPECT3 read."""
+import json
 import os
 import re
 import shutil
@@ -101,3 +102,82 @@ def test_c1_pipeline_matches_reference():
     err = np.abs(ev - np.array(g["eval_cm"]))
     print("c1 max |dE| (cm-1):", err.max(), "stage ms:", ctx.stage_times())
     assert len(ev) == 500 and err.max() < 1e-4
+
+
+def _run_exe(tmp, cfg, par, edit=None):
+    shutil.copy(os.path.join(H.JOBS, par), tmp / "pes.par")
+    text = open(os.path.join(H.JOBS, cfg + ".job")).read()
+    if edit:
+        text = edit(text)
+    (tmp / (cfg + ".job")).write_text(text)
+    with open(tmp / (cfg + ".job")) as fi, open(tmp / (cfg + ".out"), "w") as fo:
+        r = subprocess.run([EXE], stdin=fi, stdout=fo, stderr=subprocess.STDOUT, cwd=tmp,
+                           timeout=300)
+    out = open(tmp / (cfg + ".out")).read()
+    assert r.returncode == 0, out[-2000:]
+    return out
+
+
+def _levels_per_k(out):
+    blocks = []
+    for part in out.split("eigenvalues in hartrees")[1:]:
+        vals = []
+        for line in part.split("\n")[1:]:
+            if "wavenumbers" in line:
+                break
+            vals += [float(x.replace("D", "E")) for x in line.split()]
+        blocks.append(np.array(vals))
+    return blocks
+
+
+def test_k2s_jrot_levels_and_iwave_match_reference(tmp_path):
+    """J = 2 with zrot (k = 0, 1, 2 blocks run concurrently): per-k levels and
+    the IWAVE layout of the reference (r**-2 record, angular records per k)."""
+    out = _run_exe(tmp_path, "k2s", "abc_k2s.par")
+    g = json.load(open(os.path.join(H.GOLD, "k2s_ref.json")))["eval_au"]
+    blocks = _levels_per_k(out)
+    assert len(blocks) == 3
+    for k, b in enumerate(blocks):
+        assert len(b) == 30
+        assert np.abs(b - np.array(g[str(k)])).max() * O.AUTOCM < 1e-4
+    ref = np.load(os.path.join(H.GOLD, "k2s_iwave_ref.npz"))
+    recs = F.read_records(tmp_path / "fort.26")
+    assert len(recs) == int(ref["nrec"][0])
+    assert np.array_equal(F.ints(recs[0]), ref["rec1"])
+    r2 = np.frombuffer(recs[1], np.uint8)
+    assert len(r2) == len(ref["rec2"])
+    assert np.array_equal(np.frombuffer(recs[1][12:52], "<f8"),
+                          np.frombuffer(ref["rec2"][12:52].tobytes(), "<f8"))
+    assert np.array_equal(F.doubles(recs[2]), ref["rec3"])
+    assert np.abs(F.doubles(recs[3]) - ref["rec4_r2m2"]).max() < 1e-15
+    assert np.array_equal(F.doubles(recs[4]), ref["rec5_r1"])
+    assert np.array_equal(F.doubles(recs[5]), ref["rec6_r2"])
+    pos = 6
+    for k in range(3):
+        assert np.abs(F.doubles(recs[pos]) - ref[f"k{k}_xalf"]).max() < 1e-14
+        assert np.array_equal(F.ints(recs[pos + 1]), ref[f"k{k}_kzrec"])
+        assert np.abs(F.doubles(recs[pos + 2]) - ref[f"k{k}_plegw"]).max() < 1e-13
+        assert np.array_equal(F.ints(recs[pos + 3]), ref[f"k{k}_meval"])
+        assert np.abs(F.doubles(recs[pos + 4]) - ref[f"k{k}_eval"]).max() < 1e-9
+        a, b = F.doubles(recs[pos + 5]), ref[f"k{k}_psi1"]
+        s = np.sign(a @ b)
+        assert np.abs(a - s * b).max() < 1e-7
+        pos += 5 + int(ref[f"k{k}_meval"][0])
+
+
+def test_c2_dropin_levels_match_oracle(tmp_path):
+    """The c2 deck (ABC with hashed PES noise, J = 2, N3 ~ 7400-7700 per k)
+    through the executable and its example POTV; IWAVE off (all N3 levels of
+    three blocks would be tens of GB)."""
+    path = os.path.join(H.GOLD, "c2_oracle.json")
+    if not os.path.exists(path):
+        pytest.skip("c2 oracle golden not generated")
+    gold = json.load(open(path))["k"]
+    out = _run_exe(tmp_path, "c2", "abc_c2.par",
+                   edit=lambda t: t.replace("ztran=.true.", "ztran=.false."))
+    blocks = _levels_per_k(out)
+    assert len(blocks) == 3
+    for k, b in enumerate(blocks):
+        ref = np.array(gold[str(k)]["eval_au"])
+        assert len(b) == len(ref)
+        assert np.abs(b - ref).max() * O.AUTOCM < 1e-4, k

@@ -100,3 +100,46 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"],
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_fortran_host_jrot_blocks_match_oracle(k):
+    """J = 2, zrot, each k block of the ccmain loop (D:857-1007): the
+    associated-Legendre basis and the rotational term of mkham1 (D:2159-2173)
+    from the Fortran host equal the oracle's."""
+    from dvr3d_amd import host
+    job = host.read_job(os.path.join(H.JOBS, "k2s.job"))
+    sz = host.sizes(job)
+    assert (sz.kd, sz.ku, sz.kmin, sz.zrot) == (0, 2, 1, 1)
+    b = host.Basis(job, k)
+    ojob = O.load_job(os.path.join(H.JOBS, "k2s.job"))
+    ob = O.setup_basis(ojob, k)
+    for name in ("xalf", "walf"):
+        assert np.abs(np.asarray(getattr(b, name)) - getattr(ob, name)).max() < 1e-15, name
+    for name in ("xlmatr", "pleg"):
+        x, y = np.asarray(getattr(b, name)), np.asarray(getattr(ob, name))
+        assert np.abs(x - y).max() <= 1e-13 * max(1.0, np.abs(y).max()), name
+    _, pes = H.job_and_pes("k2s")
+    inp = H.device_inputs(ojob, pes, kz=k)
+    assert np.abs(b.wterm - inp["wterm"]).max() <= 1e-15 * np.abs(inp["wterm"]).max()
+    assert np.array_equal(b.wsum, inp["wsum"])
+
+
+def test_example_potv_noise_equals_oracle_noise(tmp_path):
+    """The hashed PES noise of the example POTV (C, linked into the
+    executable) and the oracle's numpy restatement give the same numbers."""
+    import ctypes
+    import subprocess
+    so = str(tmp_path / "noise.so")
+    subprocess.run(["gcc", "-O2", "-shared", "-fPIC",
+                    os.path.join(H.REPO, "dvr3d_amd", "host", "potv_noise.c"), "-o", so, "-lm"],
+                   check=True)
+    lib = ctypes.CDLL(so)
+    f = lib.dvr_synth_normal
+    f.restype = ctypes.c_double
+    f.argtypes = [ctypes.c_double] * 3
+    rng = np.random.default_rng(7)
+    r1, r2, x = rng.uniform(1, 4, 500), rng.uniform(1, 6, 500), rng.uniform(-1, 1, 500)
+    a = np.array([f(*t) for t in zip(r1, r2, x)])
+    b = O.noise_normal(r1, r2, x, O.PES.NOISE_SEED)
+    assert np.abs(a - b).max() < 1e-14

@@ -153,3 +153,44 @@ def test_c1_oracle_matches_reference_golden():
     ev = r["eval"] * O.AUTOCM
     assert len(ev) == len(g["eval_cm"]) == 500
     assert np.abs(ev - np.array(g["eval_cm"])).max() < 1e-6
+
+
+def test_jrot_k_blocks_match_reference_golden():
+    """J = 2 with zrot: the oracle's k loop (k = 0, 1, 2) against the levels
+    the reference executable prints for every k block (k2s deck)."""
+    job, pes = H.job_and_pes("k2s")
+    assert O.k_blocks(job) == [0, 1, 2]
+    g = H.golden("k2s")["eval_au"]
+    for r, k in zip(O.run_all_k(job, pes), O.k_blocks(job)):
+        ref = np.array(g[str(k)])
+        assert len(r["eval"]) == len(ref)
+        assert np.abs(r["eval"] - ref).max() < 1e-12            # 12 printed digits
+
+
+def test_abc_surface_distance_is_cartesian_ab_distance():
+    """The A-B distance of the kind-2 surface equals the Cartesian distance
+    between atom 1 (A) and atom 2 (B) in Jacobi coordinates: B and C on the
+    z axis r1 apart with their centre of mass at the origin (B at
+    z = -(1 - g1) r1, C on the +z side), A at r2 (sin t, 0, cos t)."""
+    pes = O.PES(2, 0.30, 1.30, 2.18, 0.05, 2.01, 0.0)
+    m2, m3 = 12.0, 14.00307
+    pes.g1 = m2 / (m2 + m3)
+    rng = np.random.default_rng(3)
+    r1, r2, x = rng.uniform(1.5, 3, 50), rng.uniform(1, 5, 50), rng.uniform(-1, 1, 50)
+    zb = -m3 / (m2 + m3) * r1                      # B's z (centre of mass at 0)
+    za, xa = r2 * x, r2 * np.sqrt(1 - x * x)
+    d = np.sqrt(xa ** 2 + (za - zb) ** 2)
+    v = pes(r1, r2, x)
+    expect = 0.30 * ((1 - np.exp(-1.3 * (r1 - 2.18))) ** 2 + (1 - np.exp(-1.3 * (d - 2.01))) ** 2) \
+        + 0.05 * (x - 1) ** 2
+    assert np.abs(v - expect).max() < 1e-14
+
+
+def test_noise_is_standard_normal_and_keyed_by_point():
+    rng = np.random.default_rng(11)
+    r1, r2, x = rng.uniform(1, 4, 200000), rng.uniform(1, 6, 200000), rng.uniform(-1, 1, 200000)
+    n = O.noise_normal(r1, r2, x, O.PES.NOISE_SEED)
+    assert abs(n.mean()) < 0.01 and abs(n.std() - 1.0) < 0.01
+    assert abs((n ** 4).mean() - 3.0) < 0.05                 # normal kurtosis
+    assert np.array_equal(n[:10], O.noise_normal(r1[:10], r2[:10], x[:10], O.PES.NOISE_SEED))
+    assert not np.array_equal(n[:10], O.noise_normal(r1[:10], r2[:10], x[:10], 1))
