@@ -847,15 +847,19 @@ def run_all_k(job: dict, pes, **kw):
 
 
 def run(job: dict, pes, keep_vectors: bool = False, neval_vectors: int | None = None,
-        kz: int | None = None):
+        kz: int | None = None, vgrid: np.ndarray | None = None):
     """Full theta-first jhmain contraction for one k block (default: the
-    first k of the run).  Returns a dict of every intermediate that parity
-    tests compare."""
+    first k of the run).  vgrid, when given, is the potential V[c, b, a] on
+    this block's grid (tabulated from another POTV); pes is then unused.
+    Returns a dict of every intermediate that parity tests compare."""
     basis = setup_basis(job, kz)
     s = basis.sizes
-    if getattr(pes, "kind", 1) == 2:
-        pes.g1 = basis.consts.g1
-    vgrid = potential_grid(basis, pes)
+    if vgrid is None:
+        if getattr(pes, "kind", 1) == 2:
+            pes.g1 = basis.consts.g1
+        vgrid = potential_grid(basis, pes)
+    else:
+        vgrid = np.asarray(vgrid, dtype=np.float64).reshape(s.npntc, s.npntb, s.npnta)
     one = solve_1d(basis, vgrid, job["emax1"])
     iv1 = one["iv1"]
     eig2_all, vec2_all = [], []

@@ -59,7 +59,7 @@ def device_inputs(job, pes, nvec=0, solver=0, basis=None, kz=None):
                 nvec=nvec, solver=solver)
     na = s.npnta
     xl = np.ascontiguousarray(b.xlmatr[:na, :na])
-    par = np.array([pes.de, pes.a, pes.re, pes.kb, pes.cte, pes.c])
+    par = None if pes is None else np.array([pes.de, pes.a, pes.re, pes.kb, pes.cte, pes.c])
     return dict(dims=dims, xlmatr=xl, xkb=np.asfortranarray(xkb).ravel(order="F"),
                 xkc=np.asfortranarray(xkc).ravel(order="F"), wsum=wsum, rc=rc, rb=rb,
                 xa=b.xalf[:na], par=par, emax1=job["emax1"] / O.AUTOCM,

@@ -143,3 +143,25 @@ def test_example_potv_noise_equals_oracle_noise(tmp_path):
     a = np.array([f(*t) for t in zip(r1, r2, x)])
     b = O.noise_normal(r1, r2, x, O.PES.NOISE_SEED)
     assert np.abs(a - b).max() < 1e-14
+
+
+@pytest.mark.parametrize("name,k", [("h3p_J0", 0), ("h3p_J1", 0), ("h3p_J1", 1)])
+def test_fortran_host_h3p_blocks_match_oracle(name, k):
+    """The reference's H3+ jobs (IDIA=2, zr2r1=F, zembed=F at J=1): grids,
+    L matrix and the (w1 + w2) / rotational terms of the Fortran host equal
+    the oracle's."""
+    from dvr3d_amd import host
+    job = host.read_job(os.path.join(H.JOBS, name + ".job"))
+    b = host.Basis(job, k)
+    ojob = O.load_job(os.path.join(H.JOBS, name + ".job"))
+    ob = O.setup_basis(ojob, k)
+    for nm in ("r1", "r2", "xalf", "walf"):
+        assert np.abs(np.asarray(getattr(b, nm)) - getattr(ob, nm)).max() < 1e-15, nm
+    x, y = np.asarray(b.xlmatr), ob.xlmatr
+    assert np.abs(x - y).max() <= 1e-13 * np.abs(y).max()
+    inp = H.device_inputs(ojob, None, kz=k)
+    assert np.abs(b.wsum - inp["wsum"]).max() <= 1e-15 * np.abs(inp["wsum"]).max()
+    if inp["wterm"] is None:
+        assert not b.wterm.any()
+    else:
+        assert np.abs(b.wterm - inp["wterm"]).max() <= 1e-15 * np.abs(inp["wterm"]).max()

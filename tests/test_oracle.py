@@ -6,6 +6,7 @@ quadratures and of the L matrix, and a brute-force direct-product DVR that
 the contraction must reproduce exactly when nothing is truncated.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -194,3 +195,35 @@ def test_noise_is_standard_normal_and_keyed_by_point():
     assert abs((n ** 4).mean() - 3.0) < 0.05                 # normal kurtosis
     assert np.array_equal(n[:10], O.noise_normal(r1[:10], r2[:10], x[:10], O.PES.NOISE_SEED))
     assert not np.array_equal(n[:10], O.noise_normal(r1[:10], r2[:10], x[:10], 1))
+
+
+def _h3p():
+    import json
+    import os
+    with open(os.path.join(H.GOLD, "h3p_ref.json")) as f:
+        g = json.load(f)
+    return g, np.load(os.path.join(H.GOLD, "h3p_vgrid.npz"))
+
+
+def test_reference_build_reproduces_shipped_h3p_result():
+    """The reference build used for the goldens prints, for the reference's
+    own H3+ J=0 job, the levels of its shipped result file."""
+    g, _ = _h3p()
+    ref = np.array(g["h3p_J0"]["0"]["eval_au"]) * O.AUTOCM
+    assert np.abs(ref - np.array(g["shipped_J0_cm"])).max() < 1e-6
+
+
+@pytest.mark.parametrize("name", ["h3p_J0", "h3p_J1"])
+def test_h3p_jobs_match_reference(name):
+    """The reference's own H3+ Jacobi jobs (IDIA=2, zr2r1=F; J=0 odd parity;
+    J=1 with zembed=F over k = 0, 1), on the reference H3+ POTV tabulated on
+    each k block's grid: truncation identical, 400 levels per block to the
+    printed digits."""
+    g, V = _h3p()
+    job = O.load_job(os.path.join(H.JOBS, name + ".job"))
+    for k in O.k_blocks(job):
+        r = O.run(job, None, kz=k, vgrid=V[f"{name}_k{k}"])
+        gb = g[name][str(k)]
+        assert r["ndim2d"].tolist() == gb["nham2"]
+        assert r["iv2"].tolist() == gb["iv2"] and r["nham3"] == gb["nham3"]
+        assert np.abs(r["eval"] - np.array(gb["eval_au"])).max() < 1e-12
