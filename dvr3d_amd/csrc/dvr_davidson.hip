@@ -16,10 +16,11 @@
 //    to the dense solver.
 //
 // Everything runs on the device in FP64: MFMA GEMMs for H V, the projections
-// and the Gram-Schmidt passes; the projected Rayleigh-Ritz eigenproblems and
-// the orthonormalisation of each new block (SVQB: eigen-decomposition of its
-// Gram matrix) through the batched dense eigensolver.  The host only makes
-// the locking and restart decisions from the Ritz values and residual norms.
+// and the Gram-Schmidt passes; the projected Rayleigh-Ritz eigenproblems
+// through the dense eigensolver; each new block orthonormalised by Cholesky
+// QR (twice, with classical Gram-Schmidt against the basis in between).  The
+// host only makes the locking and restart decisions from the Ritz values and
+// residual norms.
 #include "dvr_internal.h"
 
 #include <cmath>
@@ -152,6 +153,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
   nrmd.ensure(sizeof(double) * mmax);
   ldl.ensure(N);
   ldl.use_graph = use_graph;
+  ldl.panel_sub = ldl_sub;
   ldl.tuner = &tuner;
   thL.clear();
   nlocked = 0;

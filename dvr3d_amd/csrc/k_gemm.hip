@@ -38,39 +38,42 @@ __device__ inline double load_b(const GemmArgs& g, int k, int j) {
 }
 }  // namespace
 
-template <bool TA, bool TB, int KT>
+// Lower-tile enumeration shared by the tile kernels: block L of a one-
+// dimensional grid over the nt(nt+1)/2 tiles on or below the diagonal, in
+// groups of 8 row-tiles walked column by column.
+__device__ inline void lower_tile(long L, int nt, int& ti, int& tj) {
+  int r0 = (int)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
+  r0 -= r0 & 7;
+  while (r0 > 0 && (long)r0 * (r0 + 1) / 2 > L) r0 -= 8;
+  while (r0 + 8 < nt && (long)(r0 + 8) * (r0 + 9) / 2 <= L) r0 += 8;
+  const int r1 = min(r0 + 8, nt), gm = r1 - r0;
+  long q = L - (long)r0 * (r0 + 1) / 2;
+  if (q < (long)r0 * gm) {
+    tj = (int)(q / gm);
+    ti = r0 + (int)(q % gm);
+  } else {
+    q -= (long)r0 * gm;
+    int c = r0;
+    while (q >= r1 - c) {
+      q -= r1 - c;
+      ++c;
+    }
+    tj = c;
+    ti = c + (int)q;
+  }
+}
+
+template <bool TA, bool TB, int KT, bool CP>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   constexpr int NQ = KT / 4;   // elements of each operand per thread and slice
   __shared__ double As[KT * LDT];
   __shared__ double Bs[KT * LDT];
   const int z = blockIdx.z;
   int ti = blockIdx.x, tj = blockIdx.y;
-  if (g.lower) {
-    // 1D grid over the lower tiles, in groups of 8 row-tiles walked column by
-    // column: neighbouring work-groups (dealt round-robin to the XCDs) share
-    // their column panel, and each XCD keeps one row panel per group in L2
-    const long L = blockIdx.x;
-    const int nt = g.ntile;
-    int r0 = (int)((sqrt(8.0 * (double)L + 1.0) - 1.0) * 0.5);
-    r0 -= r0 & 7;
-    while (r0 > 0 && (long)r0 * (r0 + 1) / 2 > L) r0 -= 8;
-    while (r0 + 8 < nt && (long)(r0 + 8) * (r0 + 9) / 2 <= L) r0 += 8;
-    const int r1 = min(r0 + 8, nt), gm = r1 - r0;
-    long q = L - (long)r0 * (r0 + 1) / 2;
-    if (q < (long)r0 * gm) {
-      tj = (int)(q / gm);
-      ti = r0 + (int)(q % gm);
-    } else {
-      q -= (long)r0 * gm;
-      int c = r0;
-      while (q >= r1 - c) {
-        q -= r1 - c;
-        ++c;
-      }
-      tj = c;
-      ti = c + (int)q;
-    }
-  }
+  // 1D grid over the lower tiles: neighbouring work-groups (dealt round-robin
+  // to the XCDs) share their column panel, and each XCD keeps one row panel
+  // per group of 8 row-tiles in L2
+  if (g.lower) lower_tile(blockIdx.x, g.ntile, ti, tj);
   const int row0 = ti * TM, col0 = tj * TM;
   const int kbeg = z * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
@@ -93,8 +96,61 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
 
+  // C of this thread's 16 outputs, read now so that the load latency is
+  // hidden behind the k loop (beta != 0 without a K split)
+  const bool cpre = CP && g.beta != 0.0 && g.ksplit == 1;
+  double cv[2][2][4];
+  if (cpre) {
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = row0 + wy * 32 + bi * 16 + l16;
+          const int j = col0 + wx * 32 + bj * 16 + 4 * r + lk;
+          cv[bi][bj][r] = (i < g.M && j < g.N) ? g.C[i + (long)g.ldc * j] : 0.0;
+        }
+  }
+
+  // the next operand slice is staged through registers while the current
+  // one is multiplied
   double ra[NQ], rb[NQ];
+  // tiles entirely inside C with whole k slices load through running
+  // pointers (no per-element bounds arithmetic); edge tiles go element-wise
+  const bool full = row0 + TM <= g.M && col0 + TM <= g.N && (kend - kbeg) % KT == 0;
+  const double* pa;
+  const double* pb;
+  long qa, qb, sa, sb;   // pointer steps per round q and per k slice
+  if (TA) {
+    pa = g.A + kbeg + tid % KT + (long)(row0 + tid / KT) * g.lda;
+    qa = (long)(256 / KT) * g.lda;
+    sa = KT;
+  } else {
+    pa = g.A + row0 + (tid & 63) + (long)(kbeg + (tid >> 6)) * g.lda;
+    qa = 4 * g.lda;
+    sa = (long)KT * g.lda;
+  }
+  if (TB) {
+    pb = g.B + col0 + (tid & 63) + (long)(kbeg + (tid >> 6)) * g.ldb;
+    qb = 4 * g.ldb;
+    sb = (long)KT * g.ldb;
+  } else {
+    pb = g.B + kbeg + tid % KT + (long)(col0 + tid / KT) * g.ldb;
+    qb = (long)(256 / KT) * g.ldb;
+    sb = KT;
+  }
   auto fetch = [&](int k0) {
+    if (full) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        ra[q] = pa[q * qa];
+        rb[q] = pb[q * qb];
+      }
+      pa += sa;
+      pb += sb;
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       int kk, cc;
@@ -104,8 +160,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       rb[q] = (k0 + kk < kend) ? load_b<TB>(g, k0 + kk, col0 + cc) : 0.0;
     }
   };
-  if (kbeg < kend) fetch(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += KT) {
+  auto stage = [&]() {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       int kk, cc;
@@ -114,30 +169,48 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       b_pos(q, kk, cc);
       Bs[kk * LDT + cc] = rb[q];
     }
-    __syncthreads();
-    if (k0 + KT < kend) fetch(k0 + KT);
+  };
+  auto multiply = [&]() {
+    // all fragments of the slice first (distinct registers), so that the LDS
+    // reads of later k steps are in flight while the first MFMAs run
+    double fa[KT / 4][2], fb[KT / 4][2];
 #pragma unroll
-    for (int k4 = 0; k4 < KT; k4 += 4) {
-      const int kr = (k4 + lk) * LDT;
-      const double a0 = As[kr + wy * 32 + l16];
-      const double a1 = As[kr + wy * 32 + 16 + l16];
-      const double b0 = Bs[kr + wx * 32 + l16];
-      const double b1 = Bs[kr + wx * 32 + 16 + l16];
+    for (int t = 0; t < KT / 4; ++t) {
+      const int kr = (4 * t + lk) * LDT;
+      fa[t][0] = As[kr + wy * 32 + l16];
+      fa[t][1] = As[kr + wy * 32 + 16 + l16];
+      fb[t][0] = Bs[kr + wx * 32 + l16];
+      fb[t][1] = Bs[kr + wx * 32 + 16 + l16];
+    }
+    // keep the scheduler from sinking the reads back between the MFMAs
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < KT / 4; ++t) {
       // B operand first: the accumulator holds the transposed sub-tile, so
       // that lanes 0..15 of a result register own 16 consecutive rows of C
       // (coalesced stores in column-major C)
-      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[t][0], fa[t][0], acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[t][1], fa[t][0], acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[t][0], fa[t][1], acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(fb[t][1], fa[t][1], acc[1][1], 0, 0, 0);
     }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += KT) {
+    stage();
+    __syncthreads();
+    if (k0 + KT < kend) fetch(k0 + KT);
+    multiply();
     __syncthreads();
   }
 
   // register r of lane l holds C(i, j) with i = l%16, j = 4r + l/16 inside
   // the 16x16 block (bi, bj)
+#pragma unroll
   for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
     for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = row0 + wy * 32 + bi * 16 + l16;
         const int j = col0 + wx * 32 + bj * 16 + 4 * r + lk;
@@ -146,7 +219,10 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
           g.work[(long)z * g.M * g.N + i + (long)g.M * j] = acc[bi][bj][r];
         } else {
           double* c = g.C + i + (long)g.ldc * j;
-          *c = g.beta == 0.0 ? g.alpha * acc[bi][bj][r] : g.alpha * acc[bi][bj][r] + g.beta * *c;
+          if (cpre)
+            *c = g.alpha * acc[bi][bj][r] + g.beta * cv[bi][bj][r];
+          else
+            *c = g.beta == 0.0 ? g.alpha * acc[bi][bj][r] : g.alpha * acc[bi][bj][r] + g.beta * *c;
         }
       }
 }
@@ -178,7 +254,7 @@ size_t gemm_work_doubles(int M, int N, int ksplit) {
 static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
                            double alpha, const double* A, long lda, const double* B, long ldb,
                            double beta, double* C, long ldc, int ksplit, double* work, bool lower,
-                           int kt) {
+                           int kt, bool cp = false) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0) ksplit = 1;
   if (ksplit < 1) ksplit = 1;
@@ -199,17 +275,25 @@ static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, in
   const double flops = 2.0 * M * (double)N * K * (lower ? 0.5 : 1.0);
   const double bytes = 8.0 * ((double)M * K + (double)K * N +
                               (beta != 0.0 ? 2.0 : 1.0) * M * N * (lower ? 0.5 : 1.0));
+#define DVR_GEMM_LAUNCH(KT_, CP_)                                                          \
+  do {                                                                                          \
+    if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true, KT_, CP_>), grid,      \
+                             dim3(256), 0, s, g);                                               \
+    else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false, KT_, CP_>), grid,      \
+                            dim3(256), 0, s, g);                                                \
+    else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true, KT_, CP_>), grid,      \
+                            dim3(256), 0, s, g);                                                \
+    else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false, KT_, CP_>), grid, dim3(256), \
+                    0, s, g);                                                                   \
+  } while (0)
   if (kt == 32) {
-    if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true, 32>), grid, dim3(256), 0, s, g);
-    else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false, 32>), grid, dim3(256), 0, s, g);
-    else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true, 32>), grid, dim3(256), 0, s, g);
-    else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false, 32>), grid, dim3(256), 0, s, g);
+    if (cp) DVR_GEMM_LAUNCH(32, true);
+    else DVR_GEMM_LAUNCH(32, false);
   } else {
-    if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true, 16>), grid, dim3(256), 0, s, g);
-    else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false, 16>), grid, dim3(256), 0, s, g);
-    else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true, 16>), grid, dim3(256), 0, s, g);
-    else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false, 16>), grid, dim3(256), 0, s, g);
+    if (cp) DVR_GEMM_LAUNCH(16, true);
+    else DVR_GEMM_LAUNCH(16, false);
   }
+#undef DVR_GEMM_LAUNCH
   if (ksplit > 1) {
     const long n = (long)M * N;
     const int blocks = (int)std::min<long>((n + 255) / 256, 4096);
@@ -217,6 +301,13 @@ static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, in
                0, s, g);
   }
   DVR_LAUNCH_CHECK();
+}
+
+void launch_gemm_var(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                     double alpha, const double* A, long lda, const double* B, long ldb,
+                     double beta, double* C, long ldc, bool lower, int kt, bool cp) {
+  launch_gemm_kt(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, nullptr, lower,
+                 kt, cp);
 }
 
 void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,

@@ -135,7 +135,7 @@ void LdlFactor::ensure(int n_) {
   d.ensure(sizeof(double) * (size_t)n);
   linv64.ensure(sizeof(double) * (size_t)((n + NB - 1) / NB) * NB * NB);
   linvsb.ensure(sizeof(double) * (size_t)nsb * SB * SB);
-  w.ensure(sizeof(double) * (size_t)n * 4 * NB);
+  w.ensure(sizeof(double) * (size_t)n * 8 * NB);   // two panels of up to 4 NB columns
   hd.resize(n);
 }
 
@@ -149,55 +149,63 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
   // two n x 128 panel buffers (row index = global row), alternating between
   // panels: the trailing update of panel p still reads its own while the
   // panel operations of panel p+1 fill the other
-  double* Wbuf[2] = {w.as<double>(), w.as<double>() + (size_t)n * 2 * NB};
+  double* Wbuf[2] = {w.as<double>(), w.as<double>() + (size_t)n * 4 * NB};
   auto linv_of = [&](int k0) { return l64 + (size_t)(k0 / NB) * NB * NB; };
   if (!aux) {
     DVR_HIP(hipStreamCreate(&aux));
     DVR_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
     DVR_HIP(hipEventCreateWithFlags(&ev_trail, hipEventDisableTiming));
   }
-  // Panels of 2 NB columns: the second half is updated by the first half
-  // only.  The trailing update is split in two: the next panel's columns (on
+  // Panels of NS sub-panels of NB columns (PW = NS NB): each sub-panel is
+  // factored and then updates the sub-panels after it, rows below.  The
+  // trailing update (K = PW) is split in two: the next panel's columns (on
   // the main stream, so that the next panel can start at once) and the rest
   // (on the auxiliary stream, overlapping the next panel's operations).
+  const int NS = std::max(1, std::min(panel_sub, 4));
+  const int PW = NS * NB;
   bool trail_pending = false;
   int ip = 0;
-  for (int p0 = 0; p0 < n; p0 += 2 * NB, ip ^= 1) {
-    double* W = Wbuf[ip];
-    const int nba = std::min(NB, n - p0);
-    DVR_LAUNCH("ldl_diag", 16.0 * nba * nba, nba * (double)nba * nba / 3.0, k_ldl_diag, dim3(1),
-               dim3(256), 0, s, a, lda, p0, nba, dd, linv_of(p0));
-    const int ma = n - p0 - nba;
-    if (ma <= 0) break;
-    double* L21a = a + (p0 + nba) + (long)p0 * lda;
-    double* Wa = W + (p0 + nba);
-    launch_gemm(s, "ldl_panel", false, true, ma, nba, nba, 1.0, L21a, lda, linv_of(p0), NB, 0.0, Wa,
-                n, 1, nullptr, false);
-    DVR_LAUNCH("ldl_scale", 24.0 * ma * nba, 0.0, k_ldl_scale_cols, dim3(grid_for((long)ma * nba)),
-               dim3(256), 0, s, L21a, lda, Wa, (long)n, ma, nba, dd + p0);
-    const int k1 = p0 + nba;
-    const int nbb = std::min(NB, ma);
-    // second half of the panel (all rows below the first half)
-    launch_gemm(s, "ldl_panel_upd", false, true, ma, nbb, nba, -1.0, L21a, lda, Wa, n, 1.0,
-                a + k1 + (long)k1 * lda, lda, 1, nullptr, false);
-    DVR_LAUNCH("ldl_diag", 16.0 * nbb * nbb, nbb * (double)nbb * nbb / 3.0, k_ldl_diag, dim3(1),
-               dim3(256), 0, s, a, lda, k1, nbb, dd, linv_of(k1));
-    const int mb = ma - nbb;
+  for (int p0 = 0; p0 < n; p0 += PW, ip ^= 1) {
+    double* W = Wbuf[ip];   // column j of the panel's W at W + j n
+    const int cend = std::min(p0 + PW, n);
+    bool last = false;
+    for (int t = 0; t < NS; ++t) {
+      const int k0 = p0 + t * NB;
+      if (k0 >= n) {
+        last = true;
+        break;
+      }
+      const int nb = std::min(NB, n - k0);
+      DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
+                 dim3(256), 0, s, a, lda, k0, nb, dd, linv_of(k0));
+      const int mr = n - k0 - nb;
+      if (mr <= 0) {
+        last = true;
+        break;
+      }
+      double* L21 = a + (k0 + nb) + (long)k0 * lda;
+      double* Wt = W + (k0 + nb) + (long)t * NB * n;
+      launch_gemm(s, "ldl_panel", false, true, mr, nb, nb, 1.0, L21, lda, linv_of(k0), NB, 0.0, Wt,
+                  n, 1, nullptr, false);
+      DVR_LAUNCH("ldl_scale", 24.0 * mr * nb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)mr * nb)),
+                 dim3(256), 0, s, L21, lda, Wt, (long)n, mr, nb, dd + k0);
+      // the later sub-panels of this panel, all rows below this one
+      const int nrem = cend - (k0 + nb);
+      if (nrem > 0)
+        launch_gemm(s, "ldl_panel_upd", false, true, mr, nrem, nb, -1.0, L21, lda, Wt, n, 1.0,
+                    a + (k0 + nb) + (long)(k0 + nb) * lda, lda, 1, nullptr, false);
+    }
+    if (last) break;
+    const int r2 = cend;
+    const int mb = n - r2;
     if (mb <= 0) break;
-    const int r2 = k1 + nbb;
-    double* L21b = a + r2 + (long)k1 * lda;
-    double* Wb = W + r2 + (long)NB * n;
-    launch_gemm(s, "ldl_panel", false, true, mb, nbb, nbb, 1.0, L21b, lda, linv_of(k1), NB, 0.0, Wb,
-                n, 1, nullptr, false);
-    DVR_LAUNCH("ldl_scale", 24.0 * mb * nbb, 0.0, k_ldl_scale_cols, dim3(grid_for((long)mb * nbb)),
-               dim3(256), 0, s, L21b, lda, Wb, (long)n, mb, nbb, dd + k1);
-    // A22 -= [L21a L21b] [W21a W21b]^T over the rows below the panel:
+    // A22 -= L21 W21^T over the rows below the panel (K = PW):
     // first the next panel's columns ...
-    const int K = nba + nbb;
-    const int n1 = std::min(2 * NB, mb);
+    const int K = PW;
+    const int n1 = std::min(PW, mb);
     if (trail_pending) DVR_HIP(hipStreamWaitEvent(s, ev_trail, 0));   // rest of panel p-1
-    launch_gemm(s, "ldl_update_next", false, true, mb, n1, K, -1.0, a + r2 + (long)p0 * lda, lda,
-                W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, 1, nullptr, false);
+    launch_gemm_var(s, "ldl_update_next", false, true, mb, n1, K, -1.0, a + r2 + (long)p0 * lda,
+                    lda, W + r2, n, 1.0, a + r2 + (long)r2 * lda, lda, false, upd_kt, upd_cp);
     // ... then the rest, on the auxiliary stream
     const int m2 = mb - n1;
     trail_pending = false;
@@ -205,8 +213,8 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
       const int r3 = r2 + n1;
       DVR_HIP(hipEventRecord(ev_panel, s));
       DVR_HIP(hipStreamWaitEvent(aux, ev_panel, 0));
-      launch_gemm(aux, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda, lda,
-                  W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, 1, nullptr, true);
+      launch_gemm_var(aux, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda,
+                      lda, W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, true, upd_kt, upd_cp);
       DVR_HIP(hipEventRecord(ev_trail, aux));
       trail_pending = true;
     }

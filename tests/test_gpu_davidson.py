@@ -25,8 +25,9 @@ def _spd_like(n, rng):
     return a
 
 
-@pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 700])
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 200, 700, 1500, 2203])
 def test_ldl_inertia_and_solve(n):
+    """Inertia and solves of the unpivoted LDL^T (ragged last panel at 2203)."""
     from dvr3d_amd import _lib
     rng = np.random.default_rng(n)
     a = _spd_like(n, rng)
@@ -39,6 +40,27 @@ def test_ldl_inertia_and_solve(n):
     res = (a - sigma * np.eye(n)) @ t - r
     cond = np.abs(w - sigma).max() / np.abs(w - sigma).min()
     assert np.abs(res).max() <= 1e-13 * cond * np.abs(r).max() * max(n, 1) ** 0.5
+
+
+def test_ldl_panel_widths_agree(monkeypatch):
+    """Panels of 2 and 4 sub-panels of 64 columns (trailing updates with
+    K = 128 and 256) give the same factorisation: same inertia, solves equal
+    to rounding."""
+    from dvr3d_amd import _lib
+    n = 1700
+    rng = np.random.default_rng(5)
+    a = _spd_like(n, rng)
+    w = np.linalg.eigvalsh(a)
+    sigma = 0.5 * (w[600] + w[601])
+    r = rng.standard_normal((n, 3))
+    out = {}
+    for sub in ("1", "2", "3", "4"):
+        monkeypatch.setenv("DVR_LDL_SUB", sub)
+        out[sub] = _lib.ldl_solve(a, sigma, r)
+    t0 = out["2"][1]
+    for sub, (nu, t) in out.items():
+        assert nu == 601, sub
+        assert np.abs(t - t0).max() <= 1e-9 * np.abs(t0).max(), sub
 
 
 def test_ldl_inertia_on_c0_hamiltonian():
