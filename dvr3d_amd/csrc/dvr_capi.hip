@@ -403,6 +403,7 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.rr_single_wg = env_int("DVR_DAV_RRWG", 0) != 0;
     C->dav.use_graph = env_int("DVR_GRAPH", 1) != 0;
     C->dav.ldl_sub = env_int("DVR_LDL_SUB", 4);
+    C->dav.ldl_xcd = env_int("DVR_LDL_XCD", 1) != 0;
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.max_iter = env_int("DVR_DAV_MAXIT", 2000);
@@ -874,6 +875,7 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
       DVR_HIP(hipMemcpyAsync(H.p, a, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, s));
       LdlFactor f;
       f.panel_sub = env_int("DVR_LDL_SUB", 4);
+      f.upd_xcd = env_int("DVR_LDL_XCD", 1) != 0;
       f.ensure(n);
       const int nu = f.factor(H.as<double>(), sigma, s);
       if (inertia) *inertia = nu;

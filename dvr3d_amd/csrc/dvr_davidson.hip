@@ -154,6 +154,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
   ldl.ensure(N);
   ldl.use_graph = use_graph;
   ldl.panel_sub = ldl_sub;
+  ldl.upd_xcd = ldl_xcd;
   ldl.tuner = &tuner;
   thL.clear();
   nlocked = 0;

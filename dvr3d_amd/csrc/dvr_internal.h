@@ -477,12 +477,15 @@ void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N
                  double alpha, const double* A, long lda, const double* B, long ldb,
                  double beta, double* C, long ldc, int ksplit, double* work, bool lower);
 size_t gemm_work_doubles(int M, int N, int ksplit);
-// Without K split, with the k-slice depth kt (16, 32) and, for beta != 0, C
-// read at the start of each tile (cp) so that its latency hides behind the
-// k loop.
+// Without K split, with the k-slice depth kt (16, 32), for beta != 0 C read
+// at the start of each tile (cp) so that its latency hides behind the k loop,
+// and for lower outputs optionally the XCD-aware tile order (xcd_lower:
+// 8 x 8-tile super-tiles, each on one XCD, so that panels are read into each
+// L2 once per super-tile).
 void launch_gemm_var(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
                      double alpha, const double* A, long lda, const double* B, long ldb,
-                     double beta, double* C, long ldc, bool lower, int kt, bool cp);
+                     double beta, double* C, long ldc, bool lower, int kt, bool cp,
+                     bool xcd_lower = false);
 // split of K that gives about a thousand work-groups for small outputs
 int gemm_auto_ksplit(int M, int N, int K);
 
@@ -528,8 +531,9 @@ struct LdlFactor {
   void solve_graph(double* R, double* T, int nrhs, hipStream_t s);
   bool use_graph = true;
   int panel_sub = 4;            // sub-panels of 64 columns per panel (trailing K = 64 panel_sub)
-  int upd_kt = 16;              // trailing-update GEMM: k-slice depth, C read at the start
-  bool upd_cp = true;
+  int upd_kt = 16;              // trailing-update GEMM: k-slice depth, C read at the start,
+  bool upd_cp = true;           // XCD-aware tile order
+  bool upd_xcd = true;
   GemmTuner* tuner = nullptr;   // per-shape GEMM choices (nullptr: fixed rule)
   hipStream_t aux = nullptr;    // trailing updates overlapped with the next panel
   hipEvent_t ev_panel = nullptr, ev_trail = nullptr;
@@ -590,6 +594,7 @@ class DavidsonSolver {
   bool rr_single_wg = false;   // Rayleigh-Ritz through the one-work-group solver
   bool use_graph = true;       // shift-and-invert solves replayed from HIP graphs
   int ldl_sub = 4;             // LDL^T panel width in 64-column sub-panels
+  bool ldl_xcd = true;         // XCD-aware tile order of the trailing updates
 
  private:
   void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,

@@ -63,6 +63,29 @@ __device__ inline void lower_tile(long L, int nt, int& ti, int& tj) {
   }
 }
 
+// XCD-aware enumeration of the lower tiles (lower == 2).  Work-group L runs on
+// XCD L % 8 (round-robin dispatch); the lower triangle is cut into super-tiles
+// of 8 x 8 tiles, super-tile S goes to XCD S % 8, and the 64 work-groups of a
+// super-tile follow one another on their XCD, so that its 8 row panels and 8
+// column panels are read from HBM once into that XCD's L2 while the
+// super-tiles stay spread evenly over the XCDs.  Returns false for the empty
+// slots (above the diagonal of a diagonal super-tile, beyond the last tile).
+__device__ inline bool lower_tile_xcd(long L, int nt, int& ti, int& tj) {
+  const int xcd = (int)(L & 7);
+  const long j = L >> 3;
+  const long S = xcd + 8 * (j >> 6);
+  const int slot = (int)(j & 63);
+  const int ns = (nt + 7) >> 3;
+  if (S >= (long)ns * (ns + 1) / 2) return false;
+  int si = (int)((sqrt(8.0 * (double)S + 1.0) - 1.0) * 0.5);
+  while ((long)si * (si + 1) / 2 > S) --si;
+  while ((long)(si + 1) * (si + 2) / 2 <= S) ++si;
+  const int sj = (int)(S - (long)si * (si + 1) / 2);
+  ti = si * 8 + (slot & 7);
+  tj = sj * 8 + (slot >> 3);
+  return ti < nt && tj <= ti;
+}
+
 template <bool TA, bool TB, int KT, bool CP>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   constexpr int NQ = KT / 4;   // elements of each operand per thread and slice
@@ -73,7 +96,11 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   // 1D grid over the lower tiles: neighbouring work-groups (dealt round-robin
   // to the XCDs) share their column panel, and each XCD keeps one row panel
   // per group of 8 row-tiles in L2
-  if (g.lower) lower_tile(blockIdx.x, g.ntile, ti, tj);
+  if (g.lower == 2) {
+    if (!lower_tile_xcd(blockIdx.x, g.ntile, ti, tj)) return;
+  } else if (g.lower) {
+    lower_tile(blockIdx.x, g.ntile, ti, tj);
+  }
   const int row0 = ti * TM, col0 = tj * TM;
   const int kbeg = z * g.kchunk;
   const int kend = min(g.K, kbeg + g.kchunk);
@@ -254,7 +281,7 @@ size_t gemm_work_doubles(int M, int N, int ksplit) {
 static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
                            double alpha, const double* A, long lda, const double* B, long ldb,
                            double beta, double* C, long ldc, int ksplit, double* work, bool lower,
-                           int kt, bool cp = false) {
+                           int kt, bool cp = false, bool xcd_lower = false) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0) ksplit = 1;
   if (ksplit < 1) ksplit = 1;
@@ -270,7 +297,13 @@ static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, in
   g.ksplit = ksplit; g.kchunk = kchunk; g.work = work;
   if (lower && M != N) throw HipError("gemm: lower-tile product needs a square output");
   g.ntile = (M + TM - 1) / TM;
-  const dim3 grid(lower ? (unsigned)((long)g.ntile * (g.ntile + 1) / 2) : (M + TM - 1) / TM,
+  if (lower && xcd_lower && ksplit == 1) g.lower = 2;
+  long lower_wgs = (long)g.ntile * (g.ntile + 1) / 2;
+  if (g.lower == 2) {
+    const long ns = (g.ntile + 7) / 8;
+    lower_wgs = 8L * 64 * ((ns * (ns + 1) / 2 + 7) / 8);
+  }
+  const dim3 grid(lower ? (unsigned)lower_wgs : (M + TM - 1) / TM,
                   lower ? 1u : (unsigned)((N + TM - 1) / TM), ksplit);
   const double flops = 2.0 * M * (double)N * K * (lower ? 0.5 : 1.0);
   const double bytes = 8.0 * ((double)M * K + (double)K * N +
@@ -305,9 +338,10 @@ static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, in
 
 void launch_gemm_var(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
                      double alpha, const double* A, long lda, const double* B, long ldb,
-                     double beta, double* C, long ldc, bool lower, int kt, bool cp) {
+                     double beta, double* C, long ldc, bool lower, int kt, bool cp,
+                     bool xcd_lower) {
   launch_gemm_kt(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 1, nullptr, lower,
-                 kt, cp);
+                 kt, cp, xcd_lower);
 }
 
 void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,

@@ -214,7 +214,8 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
       DVR_HIP(hipEventRecord(ev_panel, s));
       DVR_HIP(hipStreamWaitEvent(aux, ev_panel, 0));
       launch_gemm_var(aux, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda,
-                      lda, W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, true, upd_kt, upd_cp);
+                      lda, W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, true, upd_kt, upd_cp,
+                      upd_xcd);
       DVR_HIP(hipEventRecord(ev_trail, aux));
       trail_pending = true;
     }

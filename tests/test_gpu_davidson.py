@@ -61,6 +61,14 @@ def test_ldl_panel_widths_agree(monkeypatch):
     for sub, (nu, t) in out.items():
         assert nu == 601, sub
         assert np.abs(t - t0).max() <= 1e-9 * np.abs(t0).max(), sub
+    # the XCD-aware order of the trailing-update tiles changes only which
+    # work-group computes which tile: bit-identical results
+    monkeypatch.setenv("DVR_LDL_SUB", "4")
+    monkeypatch.setenv("DVR_LDL_XCD", "0")
+    nu0, tx0 = _lib.ldl_solve(a, sigma, r)
+    monkeypatch.setenv("DVR_LDL_XCD", "1")
+    nu1, tx1 = _lib.ldl_solve(a, sigma, r)
+    assert nu0 == nu1 == 601 and np.array_equal(tx0, tx1)
 
 
 def test_ldl_inertia_on_c0_hamiltonian():
