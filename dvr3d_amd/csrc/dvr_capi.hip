@@ -406,6 +406,7 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.ldl_xcd = env_int("DVR_LDL_XCD", 1) != 0;
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
+    P.keep = env_int("DVR_DAV_KEEP", P.keep);
     P.max_iter = env_int("DVR_DAV_MAXIT", 2000);
     P.prec = C->prec_kind;
     if (P.prec == 2) {

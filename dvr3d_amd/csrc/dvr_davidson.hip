@@ -133,6 +133,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
   const int b = std::max(1, std::min(std::min(P.block, n), 64));   // one LDL block per Gram matrix
   const int K = std::max(b, P.slice);
   const int mmax = std::min(n, std::max(P.mmax, 2 * b));
+  const int keep = P.keep > 0 ? std::max(P.keep, b) : 0;
   const int nlmax = std::min(n, nev + 1);
   const double tol = P.tol;
   V.ensure(sizeof(double) * (size_t)N * mmax);
@@ -252,7 +253,9 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
       for (int j = nl; j < m && (int)act.size() < b; ++j)
         if (!(nrm[j] <= tol)) act.push_back(j);
       int mk = m - nl;
-      if (mk + (int)act.size() > mmax) mk = mmax - (int)act.size();
+      // full space: thick restart on the lowest Ritz vectors
+      if (mk + (int)act.size() > mmax) mk = keep > 0 ? std::min(keep, mmax - (int)act.size())
+                                                     : mmax - (int)act.size();
       const int na = (int)act.size();
       launch_resid_gather(s, X.as<double>(), AX.as<double>(), N, thd.as<double>(),
                           upload(idxd, act, s), na, Rb.as<double>(), N, N);

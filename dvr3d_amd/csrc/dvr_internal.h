@@ -574,6 +574,7 @@ struct DavParams {
   int block = 64;       // correction block
   int slice = 256;      // levels locked per certified slice
   int mmax = 448;       // largest search space
+  int keep = 320;       // Ritz vectors kept when the space is full (0: mmax - block)
   double tol = 1e-7;    // residual norm (Eh) for locking
   int max_iter = 2000;
 };
