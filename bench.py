@@ -14,8 +14,13 @@ within 1e-3 cm^-1 of the FP64 reference value (tests/golden/<cfg>_ref.json
 from the reference executable, or <cfg>_oracle.json from the FP64 oracle
 where the reference as shipped cannot finish).
 
+--config c2 runs the J = 2 ABC workload instead: the three k blocks of the
+rotational loop (every level of each block), solved concurrently on one GPU
+(one context and stream per block) and, for comparison, one after another as
+the reference's ccmain loop does.
+
 Prints ONE JSON line.  --gpus N with N > 1 is refused: this build runs on one
-GPU (see DESIGN.md for how the path shards).
+GPU.
 """
 from __future__ import annotations
 
@@ -35,6 +40,7 @@ CONFIGS = {
     "c1": ("c1.job", "ab2_c0.par"),
     "c3": ("c3.job", "ab2_c3.par"),
     "c4": ("c1.job", "ab2_c0.par"),   # 256 seeded perturbations of c1, 300 levels each
+    "c2": ("c2.job", "abc_c2.par"),   # J = 2 ABC, k = 0, 1, 2, all levels of every block
 }
 C4_LEVELS = 300
 TOL_CM = 1e-3
@@ -197,6 +203,71 @@ def sweep(args):
         "checked_instances": len(errs), "max_err_cm_per_checked_instance": errs}))
 
 
+def krot(args):
+    """c2: the k blocks of a J > 0 run.  Each block gets its basis (associated-
+    Legendre quadrature of its k, rotational term) from the Fortran host and
+    its potential from the example POTV through the host's grid loop; the
+    timed step runs all blocks with dvr_run_blocks (inputs resident in HBM).
+    Levels are checked per block against the FP64 oracle's."""
+    from dvr3d_amd import _lib, host
+    jobs = os.path.join(REPO, "oracle", "jobs")
+    os.environ["DVR_PES"] = os.path.join(jobs, CONFIGS["c2"][1])
+    job = host.read_job(os.path.join(jobs, CONFIGS["c2"][0]))
+    sz = host.sizes(job)
+    ks = list(range(sz.kd, sz.ku + 1))
+    gold = json.load(open(os.path.join(REPO, "tests", "golden", "c2_oracle.json")))["k"]
+    ctxs, n3 = [], []
+    for k in ks:
+        b = host.Basis(job, k)
+        na = b.sz.npnta
+        ctx = _lib.Context(b.dims(nvec=0, solver=_lib.DVR_SOLVER_DENSE))
+        ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, b.wterm,
+                      job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
+        ctx.set_potential(b.potential_grid())
+        ctxs.append(ctx)
+
+    def concurrent():
+        _lib.run_blocks(ctxs)
+
+    def sequential():
+        for c in ctxs:
+            c.run()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t0) / steps
+
+    dt = timed(concurrent, args.steps, args.warmup)
+    levels = conv = 0
+    errs = {}
+    for k, c in zip(ks, ctxs):
+        ev = c.eigenvalues()
+        ref = np.array(gold[str(k)]["eval_au"])
+        n3.append(int(c.sizes()[2]))
+        if len(ev) == len(ref):
+            err = np.abs(ev - ref) * host.AUTOCM
+            conv += int((err <= TOL_CM).sum())
+            errs[str(k)] = float(err.max())
+        levels += len(ev)
+    dts = timed(sequential, max(1, args.plain_steps), 0)
+    print(json.dumps({
+        "metric": "converged levels/s at <=1e-3 cm^-1 vs FP64 oracle (c2, k blocks concurrent)",
+        "value": round(conv / dt, 3), "unit": "levels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"c2: ABC Jacobi J={sz.jrot}, k={ks}, npnt1={job.npnt1}, "
+                               f"npnt2={job.npnt2}, nalf={job.nalf}, all levels per block",
+                   "nham3": n3, "levels": levels, "converged": conv,
+                   "max_err_cm_per_k": errs,
+                   "sequential_ms_per_step": round(dts * 1e3, 3),
+                   "stage_ms_per_k": [c.stage_times() for c in ctxs]},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,11 +309,14 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+                                   "(multi-GPU sharding is designed, not built)"}))
         sys.exit(2)
 
     if args.config == "c4":
         sweep(args)
+        return
+    if args.config == "c2":
+        krot(args)
         return
     from dvr3d_amd import _lib, host
     cfg = args.config

@@ -69,8 +69,13 @@ def build_host(verbose=False):
     _run(["gcc", "-O2", "-fPIC", "-c", os.path.join(HOST, "potv_noise.c"), "-o", noise_o],
          verbose=verbose)
     _run(fc + ["-c", os.path.join(HOST, "dvr3drjz_amd.f90"), "-o", main_o], verbose=verbose)
+    grid_o = os.path.join(BUILD, "dvrh_potgrid.o")
+    _run(fc + ["-c", os.path.join(HOST, "dvrh_potgrid.f90"), "-o", grid_o], verbose=verbose)
     lib = os.path.join(HERE, "libdvr3d_host.so")
-    _run(["amdflang", "-shared", setup_o, "-o", lib + ".tmp"], verbose=verbose)
+    # the host library carries the example POTV, so that callers of
+    # dvrh_potential_grid see the surface the executable is linked with
+    _run(["amdflang", "-shared", setup_o, grid_o, potv_o, noise_o, "-lm", "-o", lib + ".tmp"],
+         verbose=verbose)
     os.replace(lib + ".tmp", lib)
     exe = os.path.join(HOST, "dvr3drjz_amd")
     rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")

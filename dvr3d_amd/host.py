@@ -65,6 +65,9 @@ def load() -> C.CDLL:
     lib.dvrh_wsum.argtypes = [C.POINTER(Job), C.POINTER(Sizes), C.POINTER(Consts), C.c_int32,
                               D, D, D, D]
     lib.dvrh_wsum.restype = C.c_int32
+    lib.dvrh_potential_grid.argtypes = [C.POINTER(Job), C.POINTER(Sizes), C.POINTER(Consts),
+                                        D, D, D, D]
+    lib.dvrh_potential_grid.restype = C.c_int32
     _lib = lib
     return lib
 
@@ -135,6 +138,18 @@ class Basis:
     @property
     def xkc(self):
         return self.xk1 if self.job.zr2r1 else self.xk2
+
+    def potential_grid(self) -> np.ndarray:
+        """The linked example POTV (potv_example.f90; surface chosen by the
+        parameter file named in DVR_PES, default pes.par) on this block's
+        grid, V[c, b, a]."""
+        sz = self.sz
+        v = np.zeros((sz.npntc, sz.npntb, sz.npnta))
+        rc = load().dvrh_potential_grid(C.byref(self.job), C.byref(sz), C.byref(self.consts),
+                                        _dp(self.r1), _dp(self.r2), _dp(self.xalf), _dp(v))
+        if rc != 0:
+            raise RuntimeError(f"dvrh_potential_grid failed (code {rc})")
+        return v
 
     def dims(self, nvec: int = 0, solver: int = 0) -> dict:
         sz, job = self.sz, self.job
