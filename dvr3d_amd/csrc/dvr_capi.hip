@@ -877,6 +877,7 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
       LdlFactor f;
       f.panel_sub = env_int("DVR_LDL_SUB", 4);
       f.upd_xcd = env_int("DVR_LDL_XCD", 1) != 0;
+      f.lookahead = env_int("DVR_LDL_LOOKAHEAD", 1) != 0;
       f.ensure(n);
       const int nu = f.factor(H.as<double>(), sigma, s);
       if (inertia) *inertia = nu;

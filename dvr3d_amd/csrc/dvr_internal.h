@@ -531,6 +531,7 @@ struct LdlFactor {
   void solve_graph(double* R, double* T, int nrhs, hipStream_t s);
   bool use_graph = true;
   int panel_sub = 4;            // sub-panels of 64 columns per panel (trailing K = 64 panel_sub)
+  bool lookahead = true;        // trailing updates overlapped with the next panel
   int upd_kt = 16;              // trailing-update GEMM: k-slice depth, C read at the start,
   bool upd_cp = true;           // XCD-aware tile order
   bool upd_xcd = true;

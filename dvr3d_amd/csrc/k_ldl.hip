@@ -28,46 +28,159 @@ constexpr int LD = NB + 1;
 }  // namespace
 
 // One work-group of 256 threads.  Block [k0, k0+nb) of A (lda) in place:
-// strict lower part <- L11, d[k0 + j] <- D_j, Linv (nb x nb, ld nb) <- L11^-1.
+// strict lower part <- L11, d[j] <- D_j, Linv (nb x nb, ld nb) <- L11^-1.
+// Blocked in 16-column blocks so that the work-group meets ~20 barriers
+// instead of four per column:
+//   factorisation, per column block: (1) its diagonal block by wavefront 0
+//   (lane = row, kept in registers; column entries of other rows by lane
+//   shuffles), (2) the rows below by one thread each, the row in registers
+//   (L21 = A21 L11^-T D11^-1), (3) the rank-16 update of the trailing lower
+//   triangle, entries spread over the work-group;
+//   inverse X = L^-1: the diagonal blocks' inverses by one wavefront each
+//   (lane = column, in registers), then the blocks below, diagonal by
+//   diagonal: X_IJ = -X_II sum_{J <= K < I} L_IK X_KJ.
+// Every dependent chain runs through registers: chains through LDS (a read
+// waiting for the previous step's write) cost ~100 cycles per link.
+constexpr int BS = 16;
+
 __global__ void __launch_bounds__(256)
 k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb, double* __restrict__ d,
            double* __restrict__ Linv) {
   __shared__ double S[NB * LD];   // lower: L and D; strict upper: L^-1 transposed
-  __shared__ double red[4 * NB];
+  __shared__ double T[3 * BS * BS];
   const int tid = threadIdx.x;
-  const int r = tid & 63, cg = tid >> 6;   // row, column group (4 groups)
-  for (int c = cg; c < nb; c += 4)
-    if (r < nb) S[r + c * LD] = (r >= c) ? A[(k0 + r) + (long)(k0 + c) * lda] : 0.0;
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int c = wave; c < nb; c += 4)
+    if (lane < nb) S[lane + c * LD] = (lane >= c) ? A[(k0 + lane) + (long)(k0 + c) * lda] : 0.0;
   __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const double rj = 1.0 / S[j + j * LD];
-    // trailing update of the lower part: S(r,c) -= S(r,j) S(c,j) / dj
-    if (r < nb)
-      for (int c = j + 1 + cg; c <= r; c += 4) S[r + c * LD] -= S[r + j * LD] * (S[c + j * LD] * rj);
-    __syncthreads();
-    if (tid > j && tid < nb) S[tid + j * LD] *= rj;
-    __syncthreads();
-  }
-  // L^-1 row by row: X(i,c) = -(L(i,c) + sum_{c<k<i} L(i,k) X(k,c)), the sum
-  // split over the four wavefronts; X(i,c), i > c, is kept at S[c + i*LD]
-  for (int i = 1; i < nb; ++i) {
-    const int c = r;
-    if (c < i) {
-      double s = 0.0;
-      for (int k = c + 1 + cg; k < i; k += 4) s += S[i + k * LD] * S[c + k * LD];
-      red[cg * NB + c] = s;
+  for (int c0 = 0; c0 < nb; c0 += BS) {
+    const int bw = min(BS, nb - c0);
+    // (1) diagonal block: lane r keeps row r of the block in registers and
+    // reads the other rows' column-j entries by lane shuffles
+    if (wave == 0) {
+      const int r = lane;
+      double row[BS];
+#pragma unroll
+      for (int c = 0; c < BS; ++c) row[c] = (r < bw && c <= r) ? S[(c0 + r) + (c0 + c) * LD] : 0.0;
+#pragma unroll
+      for (int j = 0; j < BS; ++j) {
+        if (j < bw) {
+          const double rj = 1.0 / __shfl(row[j], j, 64);
+          const double lrj = row[j] * rj;
+#pragma unroll
+          for (int c = j + 1; c < BS; ++c) {
+            const double scj = __shfl(row[j], c, 64);
+            if (r > j && c <= r) row[c] -= lrj * scj;
+          }
+          if (r > j) row[j] = lrj;
+        }
+      }
+      if (r < bw) {
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+          if (c <= r) S[(c0 + r) + (c0 + c) * LD] = row[c];
+      }
     }
     __syncthreads();
-    if (cg == 0 && c < i)
-      S[c + i * LD] = -(S[i + c * LD] + ((red[c] + red[NB + c]) + (red[2 * NB + c] + red[3 * NB + c])));
+    const int r0 = c0 + bw;
+    const int mrow = nb - r0;
+    if (mrow <= 0) break;
+    // (2) rows below the block, one thread each, the row in registers
+    if (tid < mrow) {
+      const int r = r0 + tid;
+      double v[BS];
+#pragma unroll
+      for (int c = 0; c < BS; ++c) v[c] = c < bw ? S[r + (c0 + c) * LD] : 0.0;
+#pragma unroll
+      for (int c = 0; c < BS; ++c) {
+        if (c < bw) {
+          double t = v[c];
+#pragma unroll
+          for (int k = 0; k < c; ++k)
+            t -= v[k] * S[(c0 + k) + (c0 + k) * LD] * S[(c0 + c) + (c0 + k) * LD];
+          v[c] = t / S[(c0 + c) + (c0 + c) * LD];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < BS; ++c)
+        if (c < bw) S[r + (c0 + c) * LD] = v[c];
+    }
+    __syncthreads();
+    // (3) trailing lower triangle, rows and columns r0..nb-1
+    const int ne = mrow * (mrow + 1) / 2;
+    for (int e = tid; e < ne; e += 256) {
+      int rl = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+      while (rl * (rl + 1) / 2 > e) --rl;
+      while ((rl + 1) * (rl + 2) / 2 <= e) ++rl;
+      const int r = r0 + rl, c = r0 + (e - rl * (rl + 1) / 2);
+      double acc = 0.0;
+      for (int k = 0; k < bw; ++k)
+        acc += S[r + (c0 + k) * LD] * S[(c0 + k) + (c0 + k) * LD] * S[c + (c0 + k) * LD];
+      S[r + c * LD] -= acc;
+    }
     __syncthreads();
   }
-  for (int c = cg; c < nb; c += 4)
-    if (r < nb) {
-      if (r > c) A[(k0 + r) + (long)(k0 + c) * lda] = S[r + c * LD];
-      Linv[r + (long)c * nb] = r == c ? 1.0 : (r > c ? S[c + r * LD] : 0.0);
+  // inverse: (a) diagonal blocks, wavefront I for block I, lane = column
+  const int nbk = (nb + BS - 1) / BS;
+  if (wave < nbk) {
+    const int c0 = wave * BS, bw = min(BS, nb - c0);
+    const int c = lane;
+    double x[BS];   // column c of the block's inverse
+#pragma unroll
+    for (int i = 0; i < BS; ++i) x[i] = i == c ? 1.0 : 0.0;
+#pragma unroll
+    for (int i = 1; i < BS; ++i) {
+      if (i < bw && i > c) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k)
+          if (k >= c) v -= S[(c0 + i) + (c0 + k) * LD] * x[k];
+        x[i] = v;
+      }
     }
-  if (tid < nb) d[k0 + tid] = S[tid + tid * LD];
+    if (c < bw) {
+#pragma unroll
+      for (int i = 0; i < BS; ++i)
+        if (i < bw && i > c) S[(c0 + c) + (c0 + i) * LD] = x[i];
+    }
+  }
+  __syncthreads();
+  // (b) blocks below the diagonal, one block diagonal (offset o) at a time
+  for (int o = 1; o < nbk; ++o) {
+    const int npair = nbk - o;
+    for (int e = tid; e < npair * BS * BS; e += 256) {
+      const int J = e / (BS * BS), I = J + o;
+      const int ii = (e % (BS * BS)) / BS, cc = e % BS;
+      const int i0 = I * BS, j0 = J * BS;
+      if (ii < min(BS, nb - i0) && cc < min(BS, nb - j0)) {
+        const int cg = j0 + cc;
+        double acc = 0.0;
+        for (int k = j0; k < i0; ++k) {
+          const double x = k == cg ? 1.0 : (k < cg ? 0.0 : S[cg + k * LD]);
+          acc += S[(i0 + ii) + k * LD] * x;
+        }
+        T[e] = acc;
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < npair * BS * BS; e += 256) {
+      const int J = e / (BS * BS), I = J + o;
+      const int ii = (e % (BS * BS)) / BS, cc = e % BS;
+      const int i0 = I * BS, j0 = J * BS;
+      if (ii < min(BS, nb - i0) && cc < min(BS, nb - j0)) {
+        double acc = T[J * BS * BS + ii * BS + cc];
+        for (int kk = 0; kk < ii; ++kk) acc += S[(i0 + kk) + (i0 + ii) * LD] * T[J * BS * BS + kk * BS + cc];
+        S[(j0 + cc) + (i0 + ii) * LD] = -acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int c = wave; c < nb; c += 4)
+    if (lane < nb) {
+      if (lane > c) A[(k0 + lane) + (long)(k0 + c) * lda] = S[lane + c * LD];
+      Linv[lane + (long)c * nb] = lane == c ? 1.0 : (lane > c ? S[c + lane * LD] : 0.0);
+    }
+  if (tid < nb) d[tid] = S[tid + tid * LD];
 }
 
 // L21 (m x nb at A, lda) = W (m x nb, ldw) / d[j]
@@ -124,8 +237,8 @@ static int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 8192);
 
 void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, double* Linv) {
   if (nb <= 0 || nb > NB) throw HipError("ldl_block: size out of range");
-  DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1), dim3(256),
-             0, s, A, lda, 0, nb, d, Linv);
+  DVR_LAUNCH("block_ldl", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
+             dim3(256), 0, s, A, lda, 0, nb, d, Linv);
 }
 
 void LdlFactor::ensure(int n_) {
@@ -177,7 +290,7 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
       }
       const int nb = std::min(NB, n - k0);
       DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
-                 dim3(256), 0, s, a, lda, k0, nb, dd, linv_of(k0));
+                 dim3(256), 0, s, a, lda, k0, nb, dd + k0, linv_of(k0));
       const int mr = n - k0 - nb;
       if (mr <= 0) {
         last = true;
@@ -211,9 +324,10 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
     trail_pending = false;
     if (m2 > 0) {
       const int r3 = r2 + n1;
+      hipStream_t ts = lookahead ? aux : s;
       DVR_HIP(hipEventRecord(ev_panel, s));
       DVR_HIP(hipStreamWaitEvent(aux, ev_panel, 0));
-      launch_gemm_var(aux, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda,
+      launch_gemm_var(ts, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda,
                       lda, W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, true, upd_kt, upd_cp,
                       upd_xcd);
       DVR_HIP(hipEventRecord(ev_trail, aux));
