@@ -61,11 +61,25 @@ void DavidsonSolver::small_eig(int m, double* S, double* U, std::vector<double>&
   }
 }
 
-// Orthonormalise the q0 columns at W (N x q0, ld N, destroyed) against the
-// locked vectors and the current basis, append the survivors to V; returns
-// their number.
+// Orthonormalise the q0 columns at Tb (N x q0, ld N, destroyed) against the
+// locked vectors and the current basis, append the survivors to V (with
+// their products H V); returns their number.  Blocks wider than one Gram
+// factorisation (64) go through in chunks of 64, each orthogonalised
+// against the basis grown by the chunks before it.
 int DavidsonSolver::expand(const double* H, int q0, hipStream_t s) {
-  double* Wt = Tb.as<double>();
+  if (q0 <= 64) return expand_chunk(H, Tb.as<double>(), q0, s);
+  const int m0 = m;
+  int qt = 0;
+  for (int c0 = 0; c0 < q0; c0 += 64) {
+    const int q = expand_chunk(H, Tb.as<double>() + (long)c0 * N, std::min(64, q0 - c0), s);
+    m += q;
+    qt += q;
+  }
+  m = m0;
+  return qt;
+}
+
+int DavidsonSolver::expand_chunk(const double* H, double* Wt, int q0, hipStream_t s) {
   double* Vp = V.as<double>();
   double* XLp = XL.as<double>();
   double* Cp = C1.as<double>();
@@ -130,7 +144,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
   stats = DavStats{};
   N = n;
   const int nev = std::min(P.nev, n);
-  const int b = std::max(1, std::min(std::min(P.block, n), 64));   // one LDL block per Gram matrix
+  const int b = std::max(1, std::min(std::min(P.block, n), 128));   // Gram matrices of <= 64 columns
   const int K = std::max(b, P.slice);
   const int mmax = std::min(n, std::max(P.mmax, 2 * b));
   const int keep = P.keep > 0 ? std::max(P.keep, b) : 0;

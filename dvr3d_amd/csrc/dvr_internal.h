@@ -603,6 +603,7 @@ class DavidsonSolver {
             const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc);
   void small_eig(int m, double* S, double* U, std::vector<double>& w, hipStream_t s);
   int expand(const double* H, int q0, hipStream_t s);
+  int expand_chunk(const double* H, double* W, int q0, hipStream_t s);
   int cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s);
   int N = 0, m = 0, nlocked = 0;
   DevBuf V, AV, X, AX, XL, Rb, Tb, Tm, Y, Y2, G, G2, U, C1, thd, nrmd, idxd, scal, work;

@@ -98,6 +98,21 @@ def test_davidson_levels_match_reference(cfg):
     assert np.abs(ev - gold).max() <= 1e-4
 
 
+def test_davidson_block_wider_than_one_gram_factorisation(monkeypatch):
+    """Correction blocks of 128 (orthonormalised in two chunks of 64): c1
+    levels unchanged against the reference."""
+    monkeypatch.setenv("DVR_DAV_BLOCK", "128")
+    monkeypatch.setenv("DVR_DAV_MMAX", "512")
+    monkeypatch.setenv("DVR_DAV_KEEP", "320")
+    job, pes = H.job_and_pes("c1")
+    ctx = H.run_device(H.device_inputs(job, pes, solver=DAVIDSON))
+    st = ctx.solver_stats()
+    assert st["status"] == 0 and st["dense_fallback"] == 0, st
+    ev = ctx.eigenvalues() * O.AUTOCM
+    gold = np.array(H.golden("c1")["eval_cm"])
+    assert len(ev) == len(gold) and np.abs(ev - gold).max() <= 1e-4
+
+
 def test_davidson_wavefunctions_match_dense(monkeypatch):
     monkeypatch.setenv("DVR_DAV_TOL", "1e-10")
     job, pes = H.job_and_pes("c0")
