@@ -658,8 +658,11 @@ int dvr_run(dvr_ctx_t* c) {
 // larger one's.
 int dvr_run_blocks(dvr_ctx_t* const* ctxs, int n) {
   if (!ctxs || n < 0) return DVR_ERR_ARG;
-  for (int i = 0; i < n; ++i)
+  for (int i = 0; i < n; ++i) {
     if (!ctxs[i]) return DVR_ERR_ARG;
+    for (int j = 0; j < i; ++j)
+      if (ctxs[j] == ctxs[i]) return DVR_ERR_ARG;   // one thread per context
+  }
   if (n == 1) return dvr_run(ctxs[0]);
   std::vector<int> rc(n, DVR_OK);
   std::vector<std::thread> th;

@@ -107,10 +107,11 @@ int dvr_run(dvr_ctx_t* ctx);
 
 /* dvr_run on n contexts at once (the k blocks of a J > 0 run with zrot:
  * the ccmain k loop, source/dvr3drjz_segmented.f90:857-1007, which calls
- * jhmain once per k).  Every context must be distinct and ready to run; they
- * execute concurrently on their own streams.  Returns DVR_OK, or the status
- * of the first context (in array order) that failed; each context keeps its
- * own error text. */
+ * jhmain once per k).  Every context must be distinct (DVR_ERR_ARG
+ * otherwise, before anything runs) and ready to run; they execute
+ * concurrently on their own streams.  Returns DVR_OK, or the status of the
+ * first context (in array order) that failed; each context keeps its own
+ * error text. */
 int dvr_run_blocks(dvr_ctx_t* const* ctxs, int n);
 
 /* Truncation results: iv1 as the reference's iv1(ndimc,ndimb) (column-major,

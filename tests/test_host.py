@@ -183,3 +183,18 @@ def test_host_potential_grid_equals_oracle_abc_surface(monkeypatch):
         ref = O.potential_grid(ob, pes)
         assert v.shape == ref.shape
         assert np.abs(v - ref).max() <= 1e-14 * np.abs(ref).max(), k
+
+
+def test_run_blocks_rejects_bad_arguments_without_a_gpu():
+    """dvr_run_blocks checks its arguments before touching the device: a null
+    array, a null context or the same context twice."""
+    import ctypes
+    from dvr3d_amd import _lib
+    lib = _lib.load()
+    assert lib.dvr_run_blocks(None, 1) == _lib.DVR_ERR_ARG
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert lib.dvr_run_blocks(arr, 2) == _lib.DVR_ERR_ARG
+    fake = ctypes.c_void_p(0x1000)
+    arr = (ctypes.c_void_p * 2)(fake, fake)
+    assert lib.dvr_run_blocks(arr, 2) == _lib.DVR_ERR_ARG
+    assert lib.dvr_run_blocks(arr, -1) == _lib.DVR_ERR_ARG
