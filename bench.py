@@ -43,6 +43,7 @@ CONFIGS = {
     "c2": ("c2.job", "abc_c2.par"),   # J = 2 ABC, k = 0, 1, 2, all levels of every block
 }
 C4_LEVELS = 300
+C4_TRAIN = 32        # c4 instances solved densely and used to train the learned preconditioner
 TOL_CM = 1e-3
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
@@ -149,13 +150,18 @@ def perturbed_par(base, seed):
 
 
 def sweep(args):
-    """c4: seeded perturbations of the c1 potential solved one after another
-    in one context (the plain dense path: at N3 = 4000 it is the faster
-    form), lowest 300 levels each.  Levels of the first instances are checked
-    against the FP64 oracle run on the same perturbed potential."""
+    """c4: seeded perturbations of the c1 potential (every PES parameter
+    scaled by 1 + N(0, 0.01)), lowest 300 levels each, solved one after
+    another in two contexts.  The first C4_TRAIN instances run the plain
+    dense path with eigenvectors, and each one's Hamiltonian and eigenpairs
+    train the learned preconditioner on the GPU (dvr_mlp_train, continuing
+    from the previous instance's parameters); the remaining instances run
+    block-Davidson with that learned preconditioner (with the dense path as
+    fallback when it stalls).  Levels of two instances of each part are
+    checked against the FP64 oracle's (tests/golden/c4_oracle.json)."""
     import math
     from dvr3d_amd import _lib, host
-    from oracle import dvr3d_oracle as O
+    os.environ.setdefault("DVR_DAV_MAXIT", "100")
     job = host.read_job(os.path.join(REPO, "oracle", "jobs", "c1.job"))
     b = host.Basis(job)
     lines = [l for l in open(os.path.join(REPO, "oracle", "jobs", "ab2_c0.par")).read().split("\n")
@@ -163,44 +169,75 @@ def sweep(args):
     de, a, re_ = (float(v) for v in lines[1].split()[:3])
     kb, te, c = (float(v) for v in lines[2].split()[:3])
     base = np.array([de, a, re_, kb, te, c])
-    dims = b.dims(nvec=0, solver=_lib.DVR_SOLVER_DENSE)
-    dims["neval"] = C4_LEVELS
-    ctx = _lib.Context(dims)
     na = b.sz.npnta
-    ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, None,
-                  job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
     xa = b.xalf[:na].copy()
+
+    def context(solver, nvec):
+        dims = b.dims(nvec=nvec, solver=solver)
+        dims["neval"] = C4_LEVELS
+        ctx = _lib.Context(dims)
+        ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, None,
+                      job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
+        return ctx
 
     def params(i):
         p = base * perturbed_par(base, 1000 + i)
-        return p, np.array([p[0], p[1], p[2], p[3],
-                            math.cos(p[4] * (4.0 * math.atan(1.0)) / 180.0), p[5]])
+        return np.array([p[0], p[1], p[2], p[3],
+                         math.cos(p[4] * (4.0 * math.atan(1.0)) / 180.0), p[5]])
 
     n = args.instances
-    ctx.eval_potential_ab2(b.rc, b.rb, xa, params(0)[1])
-    ctx.run()                                   # warm-up (allocations)
-    levels = []
-    t0 = time.perf_counter()
-    for i in range(n):
-        ctx.eval_potential_ab2(b.rc, b.rb, xa, params(i)[1])
-        ctx.run()
-        if i < args.check:
-            levels.append(ctx.eigenvalues().copy())
-    dt = time.perf_counter() - t0
-    ojob = O.load_job(os.path.join(REPO, "oracle", "jobs", "c1.job"))
-    ojob["neval"] = C4_LEVELS
-    errs = []
-    for i in range(min(args.check, n)):
-        p = params(i)[0]
-        pes = O.PES(1, *p)
-        ref = O.run(ojob, pes)["eval"][:C4_LEVELS]
-        errs.append(float(np.abs(levels[i] - ref).max() * O.AUTOCM))
+    ntrain = min(C4_TRAIN, n)
+    gold = json.load(open(os.path.join(REPO, "tests", "golden", "c4_oracle.json")))["eval_au"]
+    dense = context(_lib.DVR_SOLVER_DENSE, C4_LEVELS)
+    dense.keep_hamiltonian(True)
+    dense.eval_potential_ab2(b.rc, b.rb, xa, params(0))
+    dense.run()                                   # warm-up (allocations)
+    mlp = _lib.mlp_init(0)
+    errs, t_solve, t_train, losses = {}, 0.0, 0.0, []
+    for i in range(ntrain):
+        t0 = time.perf_counter()
+        dense.eval_potential_ab2(b.rc, b.rb, xa, params(i))
+        dense.run()
+        t1 = time.perf_counter()
+        mlp, hist = dense.mlp_train(mlp, steps=args.train_steps, batch=32, seed=i, lr=2e-3)
+        t2 = time.perf_counter()
+        t_solve += t1 - t0
+        t_train += t2 - t1
+        losses.append(float(hist[-1]))
+        if str(i) in gold:
+            errs[str(i)] = float(np.abs(dense.eigenvalues() - np.array(gold[str(i)])).max()
+                                 * host.AUTOCM)
+    learned = context(_lib.DVR_SOLVER_DAVIDSON, 0)
+    learned.set_preconditioner(_lib.PREC_LEARNED, mlp)
+    t_dav, stats = 0.0, []
+    for i in range(ntrain, n):
+        t0 = time.perf_counter()
+        learned.eval_potential_ab2(b.rc, b.rb, xa, params(i))
+        learned.run()
+        t_dav += time.perf_counter() - t0
+        stats.append(learned.solver_stats())
+        if str(i) in gold:
+            errs[str(i)] = float(np.abs(learned.eigenvalues() - np.array(gold[str(i)])).max()
+                                 * host.AUTOCM)
+    napp = n - ntrain
     print(json.dumps({
-        "metric": "solves/s (c4 sweep)", "value": round(n / dt, 3), "unit": "solves/s",
-        "n_gpus": 1, "instances": n, "ms_per_solve": round(dt / n * 1e3, 3),
-        "levels_per_instance": C4_LEVELS, "higher_is_better": True, "dtype": "f64",
-        "data": "synthetic", "solver": "dense",
-        "checked_instances": len(errs), "max_err_cm_per_checked_instance": errs}))
+        "metric": "solves/s (c4 sweep)",
+        "value": round(n / (t_solve + t_train + t_dav), 3), "unit": "solves/s",
+        "n_gpus": 1, "instances": n, "levels_per_instance": C4_LEVELS,
+        "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+        "dense": {"instances": ntrain, "ms_per_solve": round(t_solve / max(ntrain, 1) * 1e3, 3),
+                  "training_ms_per_instance": round(t_train / max(ntrain, 1) * 1e3, 3),
+                  "train_steps_per_instance": args.train_steps,
+                  "final_training_loss": losses[-1] if losses else None},
+        "learned_davidson": {
+            "instances": napp,
+            "ms_per_solve": round(t_dav / napp * 1e3, 3) if napp else None,
+            "converged_without_fallback": sum(1 for st in stats if st["dense_fallback"] == 0),
+            "dense_fallbacks": sum(1 for st in stats if st["dense_fallback"] != 0),
+            "mean_iterations": (round(float(np.mean([st["iterations"] for st in stats])), 1)
+                                if stats else None),
+            "iteration_cap": int(os.environ["DVR_DAV_MAXIT"])},
+        "max_err_cm_per_checked_instance": errs}))
 
 
 def krot(args):
@@ -274,10 +311,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--train-steps", type=int, default=40,
+                    help="c4: training steps of the learned preconditioner per dense instance")
     ap.add_argument("--solver", default="davidson", choices=["dense", "davidson"])
     ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
-    ap.add_argument("--check", type=int, default=2,
-                    help="c4: instances checked against the oracle on the host")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain FP64 dense path run beside the chosen form")
     ap.add_argument("--no-cpu-baseline", action="store_true")

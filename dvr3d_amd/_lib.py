@@ -294,6 +294,23 @@ def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+MLP_NF, MLP_WIDTH = 6, 128
+
+
+def mlp_init(seed=0):
+    """Initial parameters of the learned preconditioner in the device layout
+    (float32: W1 [NF x 128] row-major, b1, W2 [128 x 128] row-major, b2, W3,
+    b3): W1, W2 with N(0, 1/fan-in) entries, biases zero, W3 = 0 and b3 = 1,
+    so that the untrained network is the classical Davidson preconditioner."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.standard_normal((MLP_NF, MLP_WIDTH)) / np.sqrt(MLP_NF)
+    w2 = rng.standard_normal((MLP_WIDTH, MLP_WIDTH)) / np.sqrt(MLP_WIDTH)
+    out = np.concatenate([w1.ravel(), np.zeros(MLP_WIDTH), w2.ravel(), np.zeros(MLP_WIDTH),
+                          np.zeros(MLP_WIDTH), np.ones(1)]).astype(np.float32)
+    assert out.size == MLP_NPARAM
+    return out
+
+
 def mlp_forward(params, feats, bf16=False):
     lib = load()
     p = np.ascontiguousarray(params, dtype=np.float32)

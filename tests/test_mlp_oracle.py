@@ -71,3 +71,17 @@ def test_flatten_roundtrip():
     q = M.unflatten(v)
     for k in p:
         assert np.allclose(q[k], p[k], atol=1e-6)
+
+
+def test_device_layout_initialiser_matches_oracle_layout():
+    """_lib.mlp_init lays the parameters out as the oracle's flatten() does
+    (same shapes, W3 = 0, b3 = 1, zero biases) and with the oracle's fan-in
+    scaling."""
+    from dvr3d_amd import _lib
+    p = _lib.mlp_init(3)
+    assert p.dtype == np.float32 and p.size == M.NPARAM
+    u = M.unflatten(p)
+    assert np.all(u["b1"] == 0) and np.all(u["b2"] == 0) and np.all(u["W3"] == 0)
+    assert u["b3"][0] == 1.0
+    assert abs(u["W1"].std() * np.sqrt(M.NF) - 1.0) < 0.15
+    assert abs(u["W2"].std() * np.sqrt(M.WIDTH) - 1.0) < 0.05
