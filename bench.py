@@ -67,8 +67,8 @@ def cpu_baseline(cfg: str, budget_s: float):
     at a reduced dimension on the assembled slice, each scaled to the full
     size (assembly ~ N3^2, eigensolve ~ N3^3)."""
     from oracle import dvr3d_oracle as O
-    job = O.load_job(os.path.join(REPO, "oracle", "jobs", CONFIGS[cfg][0]))
-    pes = O.PES.from_par(open(os.path.join(REPO, "oracle", "jobs", CONFIGS[cfg][1])).read())
+    job = O.load_job(os.path.join(REPO, "dvr3d_amd", "jobs", CONFIGS[cfg][0]))
+    pes = O.PES.from_par(open(os.path.join(REPO, "dvr3d_amd", "jobs", CONFIGS[cfg][1])).read())
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if cfg != "c3":
         t0 = time.perf_counter()
@@ -162,9 +162,9 @@ def sweep(args):
     import math
     from dvr3d_amd import _lib, host
     os.environ.setdefault("DVR_DAV_MAXIT", "100")
-    job = host.read_job(os.path.join(REPO, "oracle", "jobs", "c1.job"))
+    job = host.read_job(os.path.join(REPO, "dvr3d_amd", "jobs", "c1.job"))
     b = host.Basis(job)
-    lines = [l for l in open(os.path.join(REPO, "oracle", "jobs", "ab2_c0.par")).read().split("\n")
+    lines = [l for l in open(os.path.join(REPO, "dvr3d_amd", "jobs", "ab2_c0.par")).read().split("\n")
              if l.strip()]
     de, a, re_ = (float(v) for v in lines[1].split()[:3])
     kb, te, c = (float(v) for v in lines[2].split()[:3])
@@ -247,7 +247,7 @@ def krot(args):
     timed step runs all blocks with dvr_run_blocks (inputs resident in HBM).
     Levels are checked per block against the FP64 oracle's."""
     from dvr3d_amd import _lib, host
-    jobs = os.path.join(REPO, "oracle", "jobs")
+    jobs = os.path.join(REPO, "dvr3d_amd", "jobs")
     os.environ["DVR_PES"] = os.path.join(jobs, CONFIGS["c2"][1])
     job = host.read_job(os.path.join(jobs, CONFIGS["c2"][0]))
     sz = host.sizes(job)
@@ -357,9 +357,9 @@ def main():
         return
     from dvr3d_amd import _lib, host
     cfg = args.config
-    job = host.read_job(os.path.join(REPO, "oracle", "jobs", CONFIGS[cfg][0]))
+    job = host.read_job(os.path.join(REPO, "dvr3d_amd", "jobs", CONFIGS[cfg][0]))
     b = host.Basis(job)
-    par = read_pes(os.path.join(REPO, "oracle", "jobs", CONFIGS[cfg][1]))
+    par = read_pes(os.path.join(REPO, "dvr3d_amd", "jobs", CONFIGS[cfg][1]))
     na = b.sz.npnta
     xa = b.xalf[:na].copy()
 

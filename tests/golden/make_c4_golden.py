@@ -22,12 +22,12 @@ CHECKED = (0, 1, bench.C4_TRAIN, bench.C4_TRAIN + 1)
 
 
 def main():
-    lines = [l for l in open(os.path.join(REPO, "oracle", "jobs", "ab2_c0.par")).read().split("\n")
+    lines = [l for l in open(os.path.join(REPO, "dvr3d_amd", "jobs", "ab2_c0.par")).read().split("\n")
              if l.strip()]
     de, a, re_ = (float(v) for v in lines[1].split()[:3])
     kb, te, c = (float(v) for v in lines[2].split()[:3])
     base = np.array([de, a, re_, kb, te, c])
-    job = O.load_job(os.path.join(REPO, "oracle", "jobs", "c1.job"))
+    job = O.load_job(os.path.join(REPO, "dvr3d_amd", "jobs", "c1.job"))
     job["neval"] = bench.C4_LEVELS
     out = {}
     for i in CHECKED:
@@ -36,7 +36,7 @@ def main():
         out[str(i)] = r["eval"][: bench.C4_LEVELS].tolist()
         print("instance", i, "nham3", r["nham3"], flush=True)
     with open(os.path.join(REPO, "tests", "golden", "c4_oracle.json"), "w") as f:
-        json.dump({"source": "oracle/dvr3d_oracle.py on oracle/jobs/c1.job, perturbed PES",
+        json.dump({"source": "oracle/dvr3d_oracle.py on dvr3d_amd/jobs/c1.job, perturbed PES",
                    "levels": bench.C4_LEVELS, "eval_au": out}, f)
 
 

@@ -2,7 +2,7 @@
 
 Runs the reference DVR3DRJZ executable built by oracle/Makefile
 (oracle/_ref/dvr3drjz_ref, compiled from the reference sources in this
-container) on the .job decks in oracle/jobs/ and stores, per config, the
+container) on the .job decks in dvr3d_amd/jobs/ and stores, per config, the
 numbers the reference prints: the 2D block sizes (nham2 per r1 block), the
 selection outcome iv2, the quadrature points and the lowest eigenvalues in
 cm^-1 (D:2727-2730, format 5d24.12).  The output is plain JSON data.
@@ -32,7 +32,7 @@ import time
 
 REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 REF = os.path.join(REPO, "oracle", "_ref", "dvr3drjz_ref")
-JOBS = os.path.join(REPO, "oracle", "jobs")
+JOBS = os.path.join(REPO, "dvr3d_amd", "jobs")
 PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par", "k2s": "abc_k2s.par"}
 
 
@@ -174,7 +174,7 @@ def main(argv):
         blocks = parse_levels_per_k(text)
         jrot = int(open(os.path.join(JOBS, cfg + ".job")).read().split("\n")[2].split()[1])
         k0 = jrot + 1 - len(blocks)
-        res = dict(source="oracle/_ref/dvr3drjz_ref on oracle/jobs/%s.job with oracle/jobs/%s"
+        res = dict(source="oracle/_ref/dvr3drjz_ref on dvr3d_amd/jobs/%s.job with dvr3d_amd/jobs/%s"
                    % (cfg, PES_OF[cfg]), jrot=jrot,
                    eval_au={str(k0 + i): b for i, b in enumerate(blocks)})
         json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"))
@@ -187,14 +187,14 @@ def main(argv):
     if argv and argv[0] == "--parse":
         cfg, path = argv[1], argv[2]
         res = parse_output(open(path).read())
-        res["source"] = "oracle/_ref/dvr3drjz_ref on oracle/jobs/%s.job" % cfg
+        res["source"] = "oracle/_ref/dvr3drjz_ref on dvr3d_amd/jobs/%s.job" % cfg
         json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"), indent=1)
         return
     rundir = os.path.join(REPO, "_work", "runs")
     cfgs = [a for a in argv if not a.startswith("--")]
     for cfg in cfgs:
         res = run_config(cfg, os.path.join(rundir, cfg))
-        res["source"] = "oracle/_ref/dvr3drjz_ref on oracle/jobs/%s.job" % cfg
+        res["source"] = "oracle/_ref/dvr3drjz_ref on dvr3d_amd/jobs/%s.job" % cfg
         json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"), indent=1)
         print(cfg, "levels", len(res["eval_cm"]), "wall", res["ref_wall_s"])
 

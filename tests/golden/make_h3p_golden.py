@@ -1,6 +1,6 @@
 """Golden fixtures from the reference's own H3+ Jacobi jobs
 (H3+Jacobi/jobs/dvrJ0.job, dvrJ1.job, copied as data to
-oracle/jobs/h3p_J0.job, h3p_J1.job).
+dvr3d_amd/jobs/h3p_J0.job, h3p_J1.job).
 
 The reference DVR3DRJZ linked with the reference's own H3+ POTV
 (H3+Jacobi/source/pot_h3p.f90) is built by `make -C oracle h3p` from the
@@ -81,7 +81,7 @@ def main():
                    stdout=subprocess.DEVNULL)
     gold, grids = {}, {}
     for name in JOBS:
-        jobfile = os.path.join(REPO, "oracle", "jobs", name + ".job")
+        jobfile = os.path.join(REPO, "dvr3d_amd", "jobs", name + ".job")
         job = O.load_job(jobfile)
         rundir = os.path.join(REPO, "_work", "runs", name)
         os.makedirs(rundir, exist_ok=True)
@@ -109,7 +109,7 @@ def main():
             vals += _floats(line)
     gold["shipped_J0_cm"] = vals
     gold["source"] = ("oracle/_ref/dvr3drjz_h3p (reference DVR3DRJZ + H3+Jacobi/source/"
-                      "pot_h3p.f90) on oracle/jobs/h3p_J0.job, h3p_J1.job")
+                      "pot_h3p.f90) on dvr3d_amd/jobs/h3p_J0.job, h3p_J1.job")
     with open(os.path.join(REPO, "tests", "golden", "h3p_ref.json"), "w") as f:
         json.dump(gold, f)
     np.savez_compressed(os.path.join(REPO, "tests", "golden", "h3p_vgrid.npz"), **grids)

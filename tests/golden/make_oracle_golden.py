@@ -20,8 +20,8 @@ PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par", "c2": "abc
 
 
 def main(cfg):
-    job = O.load_job(os.path.join(REPO, "oracle", "jobs", cfg + ".job"))
-    pes = O.PES.from_par(open(os.path.join(REPO, "oracle", "jobs", PES_OF[cfg])).read())
+    job = O.load_job(os.path.join(REPO, "dvr3d_amd", "jobs", cfg + ".job"))
+    pes = O.PES.from_par(open(os.path.join(REPO, "dvr3d_amd", "jobs", PES_OF[cfg])).read())
     t0 = time.time()
     ks = O.k_blocks(job)
     if len(ks) > 1:
@@ -31,13 +31,13 @@ def main(cfg):
             blocks[str(k)] = dict(nham2=r["ndim2d"].tolist(), iv2=r["iv2"].tolist(),
                                   nham3=int(r["nham3"]), eval_au=r["eval"].tolist())
             print(cfg, "k", k, "nham3", r["nham3"], "levels", len(r["eval"]), flush=True)
-        out = dict(source=f"oracle/dvr3d_oracle.py on oracle/jobs/{cfg}.job", jrot=job["jrot"],
+        out = dict(source=f"oracle/dvr3d_oracle.py on dvr3d_amd/jobs/{cfg}.job", jrot=job["jrot"],
                    k=blocks, oracle_wall_s=time.time() - t0)
         with open(os.path.join(REPO, "tests", "golden", cfg + "_oracle.json"), "w") as f:
             json.dump(out, f)
         return
     r = O.run(job, pes)
-    out = dict(source=f"oracle/dvr3d_oracle.py on oracle/jobs/{cfg}.job",
+    out = dict(source=f"oracle/dvr3d_oracle.py on dvr3d_amd/jobs/{cfg}.job",
                nham2=r["ndim2d"].tolist(), iv2=r["iv2"].tolist(), nham3=int(r["nham3"]),
                eval_au=r["eval"].tolist(), eval_cm=(r["eval"] * O.AUTOCM).tolist(),
                oracle_wall_s=time.time() - t0)

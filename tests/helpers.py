@@ -7,7 +7,7 @@ import numpy as np
 from oracle import dvr3d_oracle as O
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-JOBS = os.path.join(REPO, "oracle", "jobs")
+JOBS = os.path.join(REPO, "dvr3d_amd", "jobs")
 GOLD = os.path.join(REPO, "tests", "golden")
 PES_OF = {"c0": "ab2_c0.par", "c1": "ab2_c0.par", "c3": "ab2_c3.par", "c2": "abc_c2.par",
           "k2s": "abc_k2s.par"}
