@@ -12,6 +12,10 @@
 !     distance (g1 from COMMON /MASS/, the pot_water.f convention for g2 = 0);
 !     N is a standard normal number keyed by the IEEE bit patterns of the
 !     three arguments (dvr_synth_normal, potv_noise.c).
+!   kind 3 (tabulated):  line 2: the path of a table file, raw float64
+!     (stream access): n, then n rows r1 r2 cos(theta) V -- e.g. another
+!     program's POTV evaluated on the DVR grid of a job; POTV returns the V of
+!     the row at the point (a point missing from the table stops the run).
 ! Users link their own SUBROUTINE POTV(V,R1,R2,XCOS) instead, exactly as for
 ! the reference program (docum.tex:31-45).
 subroutine potv(v, r1, r2, xcos)
@@ -29,6 +33,8 @@ subroutine potv(v, r1, r2, xcos)
   common /mass/ xmass, g1, g2, xmassr
   double precision, save :: p(6)
   integer, save :: kind = 0
+  double precision, allocatable, save :: tab(:, :)
+  integer, save :: ntab = 0, last = 1
   character(len=512) :: fname
   integer :: n, ios
   double precision :: t1, t2, te, yy, d
@@ -41,10 +47,18 @@ subroutine potv(v, r1, r2, xcos)
       stop 1
     end if
     read(92, *) kind
-    read(92, *) p(1), p(2), p(3)
-    read(92, *) p(4), te, p(6)
-    close(92)
-    if (kind == 1) then
+    if (kind == 3) then
+      read(92, '(a)') fname
+      close(92)
+      call load_table(trim(adjustl(fname)))
+    else
+      read(92, *) p(1), p(2), p(3)
+      read(92, *) p(4), te, p(6)
+      close(92)
+    end if
+    if (kind == 3) then
+      continue
+    else if (kind == 1) then
       p(5) = cos(te * (4.0d0 * atan(1.0d0)) / 180.0d0)
     else if (kind == 2) then
       p(5) = te
@@ -53,7 +67,9 @@ subroutine potv(v, r1, r2, xcos)
       stop 1
     end if
   end if
-  if (kind == 1) then
+  if (kind == 3) then
+    v = lookup(r1, r2, xcos)
+  else if (kind == 1) then
     t1 = 1.0d0 - exp(-p(2) * (r1 - p(3)))
     t2 = 1.0d0 - exp(-p(2) * (r2 - p(3)))
     v = p(1) * (t1 * t1 + t2 * t2) + p(4) * ((xcos - p(5)) * (xcos - p(5))) &
@@ -66,4 +82,40 @@ subroutine potv(v, r1, r2, xcos)
     v = p(1) * (t1 * t1 + t2 * t2) + p(4) * ((xcos - 1.0d0) * (xcos - 1.0d0))
     if (p(6) /= 0.0d0) v = v + p(6) * dvr_synth_normal(r1, r2, xcos)
   end if
+contains
+
+  subroutine load_table(path)
+    character(len=*), intent(in) :: path
+    double precision :: dn
+    integer :: u, ios
+    open(newunit=u, file=path, access='stream', form='unformatted', status='old', &
+         action='read', iostat=ios)
+    if (ios /= 0) then
+      write(6, *) 'potv: cannot open table ', path
+      stop 1
+    end if
+    read(u) dn
+    ntab = nint(dn)
+    allocate(tab(4, ntab))
+    read(u) tab
+    close(u)
+  end subroutine
+
+  ! the row at (r1, r2, x), searched from the row after the previous hit
+  ! (POTV is called in grid order, so the search is usually one step)
+  double precision function lookup(r1, r2, x)
+    double precision, intent(in) :: r1, r2, x
+    integer :: i, j
+    do j = 0, ntab - 1
+      i = mod(last + j - 1, ntab) + 1
+      if (abs(tab(1, i) - r1) + abs(tab(2, i) - r2) + abs(tab(3, i) - x) < 1.0d-12) then
+        last = i + 1
+        lookup = tab(4, i)
+        return
+      end if
+    end do
+    write(6, *) 'potv: point missing from the table', r1, r2, x
+    stop 1
+  end function
+
 end subroutine potv

@@ -111,6 +111,7 @@ def write_iwave_fixture_k(cfg: str, path: str, nk: int):
     recs = F.read_records(path)
     out = dict(rec1=F.ints(recs[0]), rec2=np.frombuffer(recs[1], dtype=np.uint8),
                rec3=F.doubles(recs[2]), rec4_r2m2=F.doubles(recs[3]),
+               rec4_raw=np.frombuffer(recs[3], dtype=np.uint8),
                rec5_r1=F.doubles(recs[4]), rec6_r2=F.doubles(recs[5]),
                nrec=np.array([len(recs)]))
     pos = 6

@@ -11,6 +11,10 @@ sources where they lie, and run here on both jobs.  Written:
                               H3+Jacobi/results/result.H3p.J0.dvr
   tests/golden/h3p_vgrid.npz  the same POTV on every k block's DVR grid,
                               V[c, b, a] (data: the GPU tests need no PES code)
+  tests/golden/h3p_J0_iwave_ref.npz, h3p_J1_iwave_ref.npz
+                              the set-up records of the reference's fort.26
+                              and, per k, the angular records, the
+                              eigenvalues and the first wavefunction
 The grid points come from the oracle's set-up, which reproduces the
 reference's quadratures bit for bit (tests/test_oracle.py).
 
@@ -29,7 +33,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 sys.path.insert(0, REPO)
 
 from oracle import dvr3d_oracle as O  # noqa: E402
-from tests.golden.make_goldens import _floats  # noqa: E402
+from tests.golden.make_goldens import _floats, write_iwave_fixture_k  # noqa: E402
 
 REF = os.path.join(REPO, "oracle", "_ref", "dvr3drjz_h3p")
 GRID = os.path.join(REPO, "oracle", "_ref", "h3p_potgrid")
@@ -92,6 +96,7 @@ def main():
         blocks = parse_blocks(open(os.path.join(rundir, name + ".out")).read())
         ks = O.k_blocks(job)
         assert len(blocks) == len(ks), (name, len(blocks), ks)
+        write_iwave_fixture_k(name, os.path.join(rundir, "fort.26"), len(ks))
         gold[name] = {}
         for k, b in zip(ks, blocks):
             basis = O.setup_basis(job, k)
