@@ -69,6 +69,13 @@ __device__ inline double block_sum_n(double v, double* red, int nthreads) {
   return s;
 }
 
+// The last TAIL columns of every problem (all of a problem of n <= TAIL) are
+// reduced by one work-group with the trailing TAIL x TAIL block in LDS
+// (k_tri_tail): one launch instead of two per column where the columns are
+// short and every launch is latency.
+constexpr int TAIL = 128;
+__device__ __host__ inline int tail_start(int n) { return n > TAIL ? n - TAIL : 0; }
+
 // pbuf holds s = A v (unscaled); p = tau s.
 __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
   __shared__ double red[TC / 64];
@@ -77,11 +84,7 @@ __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
   const int n = b.n[p];
   const long vo = b.voff[p];
   double* A = b.A + b.aoff[p];
-  if (n == 1) {
-    if (j == 0 && tid == 0) b.d[vo] = A[0];
-    return;
-  }
-  if (j > n - 2) return;
+  if (j >= tail_start(n)) return;   // k_tri_tail's part
   double* vprev = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
   double* vcur = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
   double* w = b.wbuf + vo;
@@ -162,7 +165,7 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   int p = 0, base = 0;
   for (; p < b.nprob; ++p) {
     const int n = b.n[p];
-    const int cnt = (j <= n - 3) ? (n - j - 1 + UC - 1) / UC : 0;
+    const int cnt = (j <= n - 3 && j < tail_start(n)) ? (n - j - 1 + UC - 1) / UC : 0;
     if (bid < base + cnt) break;
     base += cnt;
   }
@@ -249,30 +252,168 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   }
 }
 
+// Columns tail_start(n) .. n-2 of problem blockIdx.x: the trailing block
+// A(j0:n, j0:n) (both triangles) is copied to LDS together with the state the
+// column launches leave behind -- the reflector v_{j0-1}, s_{j0-1} = A v_{j0-1}
+// and tau_{j0-1}, the rank-2 update of step j0-1 still pending -- and the
+// column and trailing phases of every remaining step run on it exactly as in
+// k_tri_col / k_tri_upd.  The lower triangle (reflectors) is written back.
+constexpr int TT = 1024;
+constexpr int LDB = TAIL + 1;
+constexpr size_t TAIL_LDS = sizeof(double) * ((size_t)TAIL * LDB + 4 * TAIL);
+
+__global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b) {
+  extern __shared__ double sh[];
+  double* B = sh;
+  double* vp = B + TAIL * LDB;   // v_{j-1}
+  double* vc = vp + TAIL;        // v_j (the two swap roles every step)
+  double* w = vc + TAIL;         // w_{j-1}
+  double* ss = w + TAIL;         // s_{j-1}, then s_j
+  __shared__ double red[TT / 64];
+  const int p = blockIdx.x, tid = threadIdx.x;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  double* A = b.A + b.aoff[p];
+  if (n == 1) {
+    if (tid == 0) b.d[vo] = A[0];
+    return;
+  }
+  const int j0 = tail_start(n), t = n - j0;
+  for (int e = tid; e < t * t; e += TT) {
+    const int i = e % t, k = e / t;
+    B[i + k * LDB] = A[(j0 + i) + (long)(j0 + k) * n];
+  }
+  double taup = 0.0;
+  if (j0 > 0) {
+    const double* vprev = ((j0 - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
+    for (int i = tid; i < t; i += TT) {
+      vp[i] = vprev[j0 + i];
+      ss[i] = b.pbuf[vo + j0 + i];
+    }
+    taup = b.tau[vo + j0 - 1];
+  }
+  __syncthreads();
+  for (int j = j0; j <= n - 2; ++j) {
+    const int jl = j - j0;
+    double part = 0.0;
+    if (j > 0) {
+      double kp = 0.0;
+      for (int i = jl + tid; i < t; i += TT) kp += ss[i] * vp[i];
+      const double K = taup * block_sum_n(kp, red, TT);
+      const double alpha = -0.5 * taup * K;
+      const double vj = vp[jl];
+      const double wj = taup * ss[jl] + alpha * vj;
+      for (int i = jl + tid; i < t; i += TT) {
+        const double wi = taup * ss[i] + alpha * vp[i];
+        w[i] = wi;
+        const double ci = B[i + jl * LDB] - (vp[i] * wj + wi * vj);
+        B[i + jl * LDB] = ci;
+        if (i >= jl + 2) part += ci * ci;
+      }
+    } else {
+      for (int i = jl + 2 + tid; i < t; i += TT) part += B[i + jl * LDB] * B[i + jl * LDB];
+    }
+    const double sigma = block_sum_n(part, red, TT);   // w and column jl now visible
+    if (j == n - 2) {
+      if (tid == 0) {
+        b.d[vo + j] = B[jl + jl * LDB];
+        b.e[vo + j] = B[(t - 1) + jl * LDB];
+        b.tau[vo + j] = 0.0;
+        double dl = B[(t - 1) + (t - 1) * LDB];
+        if (j > 0) dl -= (vp[t - 1] * w[t - 1] + w[t - 1] * vp[t - 1]);
+        b.d[vo + n - 1] = dl;
+      }
+      break;
+    }
+    const double alpha = B[(jl + 1) + jl * LDB];
+    double tj, beta, scal;
+    if (sigma == 0.0) {
+      tj = 0.0;
+      beta = alpha;
+      scal = 0.0;
+    } else {
+      const double mu = sqrt(alpha * alpha + sigma);
+      beta = (alpha <= 0.0) ? mu : -mu;
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    for (int i = jl + 1 + tid; i < t; i += TT) {
+      const double vi = (i == jl + 1) ? 1.0 : B[i + jl * LDB] * scal;
+      vc[i] = vi;
+      if (i > jl + 1) B[i + jl * LDB] = vi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      b.d[vo + j] = B[jl + jl * LDB];
+      b.e[vo + j] = beta;
+      b.tau[vo + j] = tj;
+      B[(jl + 1) + jl * LDB] = beta;
+    }
+    // trailing phase: pending update of step j-1 on columns jl+1.. (rows
+    // jl+1..) and s_j = A v_j, eight threads per column
+    {
+      const int grp = tid >> 3, sub = tid & 7;
+      for (int k = jl + 1 + grp; k < t; k += TT / 8) {
+        const double wk = j > 0 ? w[k] : 0.0, pk = j > 0 ? vp[k] : 0.0;
+        double acc = 0.0;
+        for (int i = jl + 1 + sub; i < t; i += 8) {
+          double u = B[i + k * LDB];
+          if (j > 0) {
+            u -= vp[i] * wk + w[i] * pk;
+            B[i + k * LDB] = u;
+          }
+          acc += u * vc[i];
+        }
+        for (int off = 4; off > 0; off >>= 1) acc += __shfl_down(acc, off, 8);
+        if (sub == 0) ss[k] = acc;
+      }
+    }
+    __syncthreads();
+    double* const vt = vp;
+    vp = vc;
+    vc = vt;
+    taup = tj;
+  }
+  __syncthreads();
+  for (int e = tid; e < t * t; e += TT) {
+    const int i = e % t, k = e / t;
+    if (i >= k) A[(j0 + i) + (long)(j0 + k) * n] = B[i + k * LDB];
+  }
+}
+
 void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
-  int nmax = 0;
-  for (int p = 0; p < b.nprob; ++p) nmax = h_n[p] > nmax ? h_n[p] : nmax;
+  int jend = 0, nmax = 0;
+  for (int p = 0; p < b.nprob; ++p) {
+    jend = std::max(jend, tail_start(h_n[p]));
+    nmax = std::max(nmax, h_n[p]);
+  }
   if (nmax <= 0) return;
-  const int jmax = nmax >= 2 ? nmax - 2 : 0;
-  for (int j = 0; j <= jmax; ++j) {
+  for (int j = 0; j < jend; ++j) {
     double cbytes = 0.0, ubytes = 0.0, uflops = 0.0;
     int grid = 0;
     for (int p = 0; p < b.nprob; ++p) {
+      if (j >= tail_start(h_n[p])) continue;
       const double m = (double)(h_n[p] - j);
-      if (m > 0) cbytes += 8.0 * 6.0 * m;
-      if (j <= h_n[p] - 3) {
-        grid += (h_n[p] - j - 1 + UC - 1) / UC;
-        const double t = m - 1.0;
-        // trailing square read (+ written back when an update is pending)
-        ubytes += (j > 0 ? 16.0 : 8.0) * t * t + 8.0 * 4.0 * t;
-        uflops += (j > 0 ? 6.0 : 2.0) * t * t;
-      }
+      cbytes += 8.0 * 6.0 * m;
+      grid += (h_n[p] - j - 1 + UC - 1) / UC;
+      const double t = m - 1.0;
+      // trailing square read (+ written back when an update is pending)
+      ubytes += (j > 0 ? 16.0 : 8.0) * t * t + 8.0 * 4.0 * t;
+      uflops += (j > 0 ? 6.0 : 2.0) * t * t;
     }
     DVR_LAUNCH("tri_col", cbytes, 0.0, k_tri_col, dim3(b.nprob), dim3(TC), 0, s, b, j);
     if (grid > 0) {
       DVR_LAUNCH("tri_upd", ubytes, uflops, k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
     }
   }
+  DVR_HIP(hipFuncSetAttribute((const void*)k_tri_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)TAIL_LDS));
+  double tbytes = 0.0;
+  for (int p = 0; p < b.nprob; ++p) {
+    const double t = (double)(h_n[p] - tail_start(h_n[p]));
+    tbytes += 8.0 * 2.0 * t * t;
+  }
+  DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail, dim3(b.nprob), dim3(TT), TAIL_LDS, s, b);
   DVR_LAUNCH_CHECK();
 }
 

@@ -51,9 +51,19 @@ def test_dvr_diag_closed_form():
     assert np.abs(e - (2 - 2 * np.cos(k * np.pi / (n + 1)))).max() < 1e-13
     full = a + np.tril(a, -1).T
     assert np.abs(full @ v - v * e).max() < 1e-12
+    # already tridiagonal past the LDS-resident tail (every reflector is the
+    # identity, tau = 0, in both the column launches and the tail)
+    n = 300
+    a = np.diag([2.0] * n) + np.diag([-1.0] * (n - 1), -1)
+    e, v = _lib.diag(a)
+    k = np.arange(1, n + 1)
+    assert np.abs(e - (2 - 2 * np.cos(k * np.pi / (n + 1)))).max() < 1e-13
+    full = a + np.tril(a, -1).T
+    assert np.abs(full @ v - v * e).max() < 1e-12
 
 
-@pytest.mark.parametrize("sizes", [[1], [2], [3, 5], [17, 64, 1, 130], [257, 31]])
+@pytest.mark.parametrize("sizes", [[1], [2], [3, 5], [17, 64, 1, 130], [257, 31],
+                                   [128, 129, 127, 2], [700, 128, 129, 300]])
 def test_syev_batch_random(sizes):
     from dvr3d_amd import _lib
     rng = np.random.default_rng(sum(sizes))
