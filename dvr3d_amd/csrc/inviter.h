@@ -26,7 +26,7 @@ __device__ inline double ii_hash(unsigned long long x) {
 // s0: this thread's scratch; array a (0..5) starts at s0 + a*astride and its
 // element i is at + i*stride.  Vector k is written to Zp + k*n.
 template <int CH>
-__device__ inline void inverse_iteration_cluster(int n, const double* __restrict__ d,
+__device__ __forceinline__ void inverse_iteration_cluster(int n, const double* __restrict__ d,
                                                  const double* __restrict__ e,
                                                  const double* __restrict__ lams, int kb, int ke,
                                                  double tn, double* s0, long astride, long stride,

@@ -564,10 +564,72 @@ k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
                             scratch + t, nm * T, T, Z + zoff[p], (unsigned long long)p);
 }
 
+// The same inverse iteration with the per-thread scratch in LDS, for
+// problems small enough that several threads' scratch fits in one CU's LDS
+// (the Rayleigh-Ritz problems of the iterative solver): with few clusters
+// the global-memory version above runs a handful of wavefronts whose every
+// burst of loads waits on memory.  One wavefront per work-group; its first TW
+// lanes take one cluster each, their arrays interleaved so that a step
+// touches TW consecutive doubles.  When all of a work-group's clusters belong
+// to one problem its tridiagonal is staged in LDS as well.
+constexpr int LDS_BYTES_MAX = 160 * 1024;
+
+// DVR_STEIN=global keeps every problem on the global-memory version
+bool stein_use_lds() {
+  const char* v = getenv("DVR_STEIN");
+  return !(v && std::string(v) == "global");
+}
+
+int stein_lds_lanes(int nmax) {
+  if (nmax <= 0) return 0;
+  const long avail = LDS_BYTES_MAX / 8 - 2L * nmax;
+  return (int)std::min<long>(64, std::max<long>(0, avail / (6L * nmax)));
+}
+
+__global__ void __launch_bounds__(64)
+k_stein_lds(EigBatch b, const Cluster* __restrict__ cl, int ncl,
+            const double* __restrict__ eval, const long* __restrict__ eoff,
+            double* __restrict__ Z, const long* __restrict__ zoff, int TW) {
+  extern __shared__ double sh[];
+  const int nm = b.nmax;
+  const int c0 = blockIdx.x * TW;
+  const int cnt = min(TW, ncl - c0);
+  const int lane = threadIdx.x;
+  double* dl = sh + (long)6 * nm * TW;
+  double* el = dl + nm;
+  const int p0 = cl[c0].p;
+  const bool staged = cl[c0 + cnt - 1].p == p0;
+  if (staged) {
+    const long vo = b.voff[p0];
+    for (int i = lane; i < b.n[p0]; i += 64) {
+      dl[i] = b.d[vo + i];
+      el[i] = b.e[vo + i];
+    }
+  }
+  __syncthreads();
+  if (lane >= cnt) return;
+  const Cluster C = cl[c0 + lane];
+  const int p = C.p;
+  const long vo = b.voff[p];
+  inverse_iteration_cluster<8>(b.n[p], staged ? dl : b.d + vo, staged ? el : b.e + vo,
+                                eval + eoff[p], C.kb, C.ke, b.tnrm[p], sh + lane,
+                                (long)nm * TW, TW, Z + zoff[p], (unsigned long long)p);
+}
+
 void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
                   int ncl, const double* eval, const long* eoff, double* Z,
                   const long* zoff, double* scratch, int nthreads) {
   if (ncl <= 0) return;
+  const int tw = stein_lds_lanes(b.nmax);
+  if (tw >= 4 && stein_use_lds()) {
+    const size_t lds = sizeof(double) * ((size_t)6 * b.nmax * tw + 2 * (size_t)b.nmax);
+    DVR_HIP(hipFuncSetAttribute((const void*)k_stein_lds,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DVR_LAUNCH("stein", 0.0, 0.0, k_stein_lds, dim3((ncl + tw - 1) / tw), dim3(64), lds, s, b, cl,
+               ncl, eval, eoff, Z, zoff, tw);
+    DVR_LAUNCH_CHECK();
+    return;
+  }
   const int grid = (ncl + TB - 1) / TB;
   DVR_LAUNCH("stein", 0.0, 0.0, k_stein, dim3(grid), dim3(TB), 0, s, b, cl, ncl, eval,
                    eoff, Z, zoff, scratch, nthreads);
