@@ -15,7 +15,7 @@
 //   W21 = A21 L11^-T                      (k_gemm, NT)
 //   L21 = W21 D1^-1                       (k_ldl_scale_cols, in place)
 //   A22 -= L21 W21^T                      (k_gemm, NT, lower tiles only).
-// The triangular solves run over super-blocks of SB = 512 rows whose inverse
+// The triangular solves run over super-blocks of SB = 1024 rows whose inverse
 // diagonal blocks are formed once per factorisation (forward substitution on
 // the identity), so that one application costs a few GEMMs per super-block.
 #include "dvr_internal.h"
@@ -29,113 +29,128 @@ constexpr int LD = NB + 1;
 
 // One work-group of 256 threads.  Block [k0, k0+nb) of A (lda) in place:
 // strict lower part <- L11, d[j] <- D_j, Linv (nb x nb, ld nb) <- L11^-1.
-// Blocked in 16-column blocks so that the work-group meets ~20 barriers
+// Blocked in 16-column panels so that the work-group meets ~16 barriers
 // instead of four per column:
-//   factorisation, per column block: (1) its diagonal block by wavefront 0
-//   (lane = row, kept in registers; column entries of other rows by lane
-//   shuffles), (2) the rows below by one thread each, the row in registers
-//   (L21 = A21 L11^-T D11^-1), (3) the rank-16 update of the trailing lower
-//   triangle, entries spread over the work-group;
+//   factorisation, per panel: (P) wavefront 0 factors the panel (all rows
+//   from the panel's first column down, lane = row, the panel's 16 columns
+//   in registers); pivots and multipliers of other rows are read with
+//   v_readlane (uniform lane index), so the only dependent chain is the
+//   register arithmetic; (U) the rank-16 update of the trailing lower
+//   triangle, entries spread over the work-group, with the panel's
+//   unscaled columns W = L D kept in LDS;
 //   inverse X = L^-1: the diagonal blocks' inverses by one wavefront each
-//   (lane = column, in registers), then the blocks below, diagonal by
-//   diagonal: X_IJ = -X_II sum_{J <= K < I} L_IK X_KJ.
-// Every dependent chain runs through registers: chains through LDS (a read
-// waiting for the previous step's write) cost ~100 cycles per link.
+//   (lane = column, right-looking substitution in registers), then the
+//   blocks below, diagonal by diagonal: X_IJ = -X_II sum_{J <= K < I} L_IK X_KJ.
+// Every inner loop has a compile-time trip count, so its LDS reads issue
+// together instead of one round trip per step.
 constexpr int BS = 16;
 
+__device__ inline double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/x from the hardware estimate and two Newton steps
+__device__ inline double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// FULL: nb == NB, every panel 16 wide (all guards resolved at compile time)
+template <bool FULL>
 __global__ void __launch_bounds__(256)
-k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb, double* __restrict__ d,
+k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb_, double* __restrict__ d,
            double* __restrict__ Linv) {
   __shared__ double S[NB * LD];   // lower: L and D; strict upper: L^-1 transposed
+  __shared__ double Wb[NB * BS];  // the current panel's W = L D, by column
   __shared__ double T[3 * BS * BS];
+  const int nb = FULL ? NB : nb_;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  for (int c = wave; c < nb; c += 4)
-    if (lane < nb) S[lane + c * LD] = (lane >= c) ? A[(k0 + lane) + (long)(k0 + c) * lda] : 0.0;
+  {
+    double v[NB / 4];
+#pragma unroll
+    for (int q = 0; q < NB / 4; ++q) {
+      const int c = wave + 4 * q;
+      v[q] = (c < nb && lane < nb && lane >= c) ? A[(k0 + lane) + (long)(k0 + c) * lda] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < NB / 4; ++q) S[lane + (wave + 4 * q) * LD] = v[q];
+  }
   __syncthreads();
   for (int c0 = 0; c0 < nb; c0 += BS) {
-    const int bw = min(BS, nb - c0);
-    // (1) diagonal block: lane r keeps row r of the block in registers and
-    // reads the other rows' column-j entries by lane shuffles
+    const int bw = FULL ? BS : min(BS, nb - c0);
+    // (P) the panel: lane r holds row r of columns c0 .. c0+bw; entries
+    // above the diagonal are zero (upper part of S) and only pass through
     if (wave == 0) {
       const int r = lane;
-      double row[BS];
+      double col[BS], wr[BS];
 #pragma unroll
-      for (int c = 0; c < BS; ++c) row[c] = (r < bw && c <= r) ? S[(c0 + r) + (c0 + c) * LD] : 0.0;
+      for (int c = 0; c < BS; ++c) col[c] = (c < bw) ? S[r + (c0 + c) * LD] : 0.0;
 #pragma unroll
       for (int j = 0; j < BS; ++j) {
         if (j < bw) {
-          const double rj = 1.0 / __shfl(row[j], j, 64);
-          const double lrj = row[j] * rj;
+          const double dj = readlane_d(col[j], c0 + j);
+          const double w = col[j];
+          const double l = w * recip_nr(dj);
 #pragma unroll
-          for (int c = j + 1; c < BS; ++c) {
-            const double scj = __shfl(row[j], c, 64);
-            if (r > j && c <= r) row[c] -= lrj * scj;
+          for (int c = j + 1; c < BS; ++c)
+            if (c < bw) col[c] -= w * readlane_d(l, c0 + c);
+          wr[j] = w;
+          col[j] = r > c0 + j ? l : (r == c0 + j ? dj : 0.0);
+        }
+      }
+      if (r < nb) {
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+          if (j < bw) {
+            S[r + (c0 + j) * LD] = col[j];
+            Wb[r + j * NB] = wr[j];
           }
-          if (r > j) row[j] = lrj;
-        }
-      }
-      if (r < bw) {
-#pragma unroll
-        for (int c = 0; c < BS; ++c)
-          if (c <= r) S[(c0 + r) + (c0 + c) * LD] = row[c];
       }
     }
     __syncthreads();
-    const int r0 = c0 + bw;
-    const int mrow = nb - r0;
-    if (mrow <= 0) break;
-    // (2) rows below the block, one thread each, the row in registers
-    if (tid < mrow) {
-      const int r = r0 + tid;
-      double v[BS];
+    const int c1 = c0 + bw;
+    const int m = nb - c1;
+    if (m <= 0) break;
+    // (U) trailing lower triangle, rows and columns c1 .. nb-1: lane = row,
+    // wavefront w takes the columns c1 + w, c1 + w + 4, ...
+    if (lane < m) {
+      const int r = c1 + lane;
+      double wv[BS];
 #pragma unroll
-      for (int c = 0; c < BS; ++c) v[c] = c < bw ? S[r + (c0 + c) * LD] : 0.0;
+      for (int k = 0; k < BS; ++k) wv[k] = (k < bw) ? Wb[r + k * NB] : 0.0;
+      for (int cl = wave; cl <= lane; cl += 4) {
+        const int c = c1 + cl;
+        double acc = 0.0;
 #pragma unroll
-      for (int c = 0; c < BS; ++c) {
-        if (c < bw) {
-          double t = v[c];
-#pragma unroll
-          for (int k = 0; k < c; ++k)
-            t -= v[k] * S[(c0 + k) + (c0 + k) * LD] * S[(c0 + c) + (c0 + k) * LD];
-          v[c] = t / S[(c0 + c) + (c0 + c) * LD];
-        }
+        for (int k = 0; k < BS; ++k)
+          if (k < bw) acc += wv[k] * S[c + (c0 + k) * LD];
+        S[r + c * LD] -= acc;
       }
-#pragma unroll
-      for (int c = 0; c < BS; ++c)
-        if (c < bw) S[r + (c0 + c) * LD] = v[c];
-    }
-    __syncthreads();
-    // (3) trailing lower triangle, rows and columns r0..nb-1
-    const int ne = mrow * (mrow + 1) / 2;
-    for (int e = tid; e < ne; e += 256) {
-      int rl = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-      while (rl * (rl + 1) / 2 > e) --rl;
-      while ((rl + 1) * (rl + 2) / 2 <= e) ++rl;
-      const int r = r0 + rl, c = r0 + (e - rl * (rl + 1) / 2);
-      double acc = 0.0;
-      for (int k = 0; k < bw; ++k)
-        acc += S[r + (c0 + k) * LD] * S[(c0 + k) + (c0 + k) * LD] * S[c + (c0 + k) * LD];
-      S[r + c * LD] -= acc;
     }
     __syncthreads();
   }
   // inverse: (a) diagonal blocks, wavefront I for block I, lane = column
   const int nbk = (nb + BS - 1) / BS;
   if (wave < nbk) {
-    const int c0 = wave * BS, bw = min(BS, nb - c0);
+    const int c0 = wave * BS, bw = FULL ? BS : min(BS, nb - c0);
     const int c = lane;
     double x[BS];   // column c of the block's inverse
 #pragma unroll
     for (int i = 0; i < BS; ++i) x[i] = i == c ? 1.0 : 0.0;
 #pragma unroll
-    for (int i = 1; i < BS; ++i) {
-      if (i < bw && i > c) {
-        double v = 0.0;
+    for (int k = 0; k < BS - 1; ++k) {
+      if (k < bw) {
+        const double xk = x[k];
 #pragma unroll
-        for (int k = 0; k < i; ++k)
-          if (k >= c) v -= S[(c0 + i) + (c0 + k) * LD] * x[k];
-        x[i] = v;
+        for (int i = k + 1; i < BS; ++i)
+          if (i < bw) x[i] -= S[(c0 + i) + (c0 + k) * LD] * xk;
       }
     }
     if (c < bw) {
@@ -152,12 +167,23 @@ k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb, double* __restrict_
       const int J = e / (BS * BS), I = J + o;
       const int ii = (e % (BS * BS)) / BS, cc = e % BS;
       const int i0 = I * BS, j0 = J * BS;
-      if (ii < min(BS, nb - i0) && cc < min(BS, nb - j0)) {
-        const int cg = j0 + cc;
+      const int bwj = FULL ? BS : min(BS, nb - j0);
+      if (FULL || (ii < min(BS, nb - i0) && cc < bwj)) {
+        const double* Lrow = S + (i0 + ii);   // L(i0+ii, k) at Lrow[k * LD]
+        const double* Xcol = S + (j0 + cc);   // X(k, j0+cc) at Xcol[k * LD], k > j0+cc
+        // K = J: X_JJ is unit lower
         double acc = 0.0;
-        for (int k = j0; k < i0; ++k) {
-          const double x = k == cg ? 1.0 : (k < cg ? 0.0 : S[cg + k * LD]);
-          acc += S[(i0 + ii) + k * LD] * x;
+#pragma unroll
+        for (int k = 0; k < BS; ++k) {
+          if (k < bwj) {
+            const double x = k == cc ? 1.0 : (k > cc ? Xcol[(j0 + k) * LD] : 0.0);
+            acc += Lrow[(j0 + k) * LD] * x;
+          }
+        }
+        // J < K < I: full blocks
+        for (int kb = j0 + BS; kb < i0; kb += BS) {
+#pragma unroll
+          for (int k = 0; k < BS; ++k) acc += Lrow[(kb + k) * LD] * Xcol[(kb + k) * LD];
         }
         T[e] = acc;
       }
@@ -167,9 +193,11 @@ k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb, double* __restrict_
       const int J = e / (BS * BS), I = J + o;
       const int ii = (e % (BS * BS)) / BS, cc = e % BS;
       const int i0 = I * BS, j0 = J * BS;
-      if (ii < min(BS, nb - i0) && cc < min(BS, nb - j0)) {
-        double acc = T[J * BS * BS + ii * BS + cc];
-        for (int kk = 0; kk < ii; ++kk) acc += S[(i0 + kk) + (i0 + ii) * LD] * T[J * BS * BS + kk * BS + cc];
+      if (FULL || (ii < min(BS, nb - i0) && cc < min(BS, nb - j0))) {
+        double acc = T[e];
+#pragma unroll
+        for (int kk = 0; kk < BS; ++kk)
+          if (kk < ii) acc += S[(i0 + kk) + (i0 + ii) * LD] * T[J * BS * BS + kk * BS + cc];
         S[(j0 + cc) + (i0 + ii) * LD] = -acc;
       }
     }
@@ -181,6 +209,16 @@ k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb, double* __restrict_
       Linv[lane + (long)c * nb] = lane == c ? 1.0 : (lane > c ? S[c + lane * LD] : 0.0);
     }
   if (tid < nb) d[tid] = S[tid + tid * LD];
+}
+
+static void launch_ldl_diag(hipStream_t s, const char* name, double* A, long lda, int k0, int nb,
+                            double* d, double* Linv) {
+  if (nb == NB)
+    DVR_LAUNCH(name, 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag<true>, dim3(1),
+               dim3(256), 0, s, A, lda, k0, nb, d, Linv);
+  else
+    DVR_LAUNCH(name, 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag<false>, dim3(1),
+               dim3(256), 0, s, A, lda, k0, nb, d, Linv);
 }
 
 // L21 (m x nb at A, lda) = W (m x nb, ldw) / d[j]
@@ -237,8 +275,7 @@ static int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 8192);
 
 void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, double* Linv) {
   if (nb <= 0 || nb > NB) throw HipError("ldl_block: size out of range");
-  DVR_LAUNCH("block_ldl", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
-             dim3(256), 0, s, A, lda, 0, nb, d, Linv);
+  launch_ldl_diag(s, "block_ldl", A, lda, 0, nb, d, Linv);
 }
 
 void LdlFactor::ensure(int n_) {
@@ -289,8 +326,7 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
         break;
       }
       const int nb = std::min(NB, n - k0);
-      DVR_LAUNCH("ldl_diag", 16.0 * nb * nb, nb * (double)nb * nb / 3.0, k_ldl_diag, dim3(1),
-                 dim3(256), 0, s, a, lda, k0, nb, dd + k0, linv_of(k0));
+      launch_ldl_diag(s, "ldl_diag", a, lda, k0, nb, dd + k0, linv_of(k0));
       const int mr = n - k0 - nb;
       if (mr <= 0) {
         last = true;
