@@ -404,6 +404,7 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.use_graph = env_int("DVR_GRAPH", 1) != 0;
     C->dav.ldl_sub = env_int("DVR_LDL_SUB", 4);
     C->dav.ldl_xcd = env_int("DVR_LDL_XCD", 1) != 0;
+    C->dav.ldl_sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.keep = env_int("DVR_DAV_KEEP", P.keep);

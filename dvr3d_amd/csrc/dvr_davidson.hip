@@ -166,6 +166,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
   C1.ensure(sizeof(double) * (size_t)std::max(nlmax, mmax) * b);
   thd.ensure(sizeof(double) * mmax);
   nrmd.ensure(sizeof(double) * mmax);
+  ldl.sb = ldl_sb;
   ldl.ensure(N);
   ldl.use_graph = use_graph;
   ldl.panel_sub = ldl_sub;

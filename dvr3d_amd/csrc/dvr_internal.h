@@ -516,7 +516,7 @@ struct GemmTuner {
 // LDL^T of H - sigma I without pivoting (k_ldl.hip)
 // ---------------------------------------------------------------------------
 struct LdlFactor {
-  static constexpr int SB = 1024;  // super-block of the triangular solves
+  int sb = 2048;  // super-block of the triangular solves (set before ensure)
   int n = 0, nsb = 0;
   DevBuf A, d, linv64, linvsb, w, work;
   std::vector<double> hd;
@@ -597,6 +597,7 @@ class DavidsonSolver {
   bool use_graph = true;       // shift-and-invert solves replayed from HIP graphs
   int ldl_sub = 4;             // LDL^T panel width in 64-column sub-panels
   bool ldl_xcd = true;         // XCD-aware tile order of the trailing updates
+  int ldl_sb = 2048;           // super-block of the shift-and-invert triangular solves
 
  private:
   void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,

@@ -15,7 +15,7 @@
 //   W21 = A21 L11^-T                      (k_gemm, NT)
 //   L21 = W21 D1^-1                       (k_ldl_scale_cols, in place)
 //   A22 -= L21 W21^T                      (k_gemm, NT, lower tiles only).
-// The triangular solves run over super-blocks of SB = 1024 rows whose inverse
+// The triangular solves run over super-blocks of sb rows (2048 by default) whose inverse
 // diagonal blocks are formed once per factorisation (forward substitution on
 // the identity), so that one application costs a few GEMMs per super-block.
 #include "dvr_internal.h"
@@ -280,11 +280,11 @@ void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, dou
 
 void LdlFactor::ensure(int n_) {
   n = n_;
-  nsb = (n + SB - 1) / SB;
+  nsb = (n + sb - 1) / sb;
   A.ensure(sizeof(double) * (size_t)n * n);
   d.ensure(sizeof(double) * (size_t)n);
   linv64.ensure(sizeof(double) * (size_t)((n + NB - 1) / NB) * NB * NB);
-  linvsb.ensure(sizeof(double) * (size_t)nsb * SB * SB);
+  linvsb.ensure(sizeof(double) * (size_t)nsb * sb * sb);
   w.ensure(sizeof(double) * (size_t)n * 8 * NB);   // two panels of up to 4 NB columns
   hd.resize(n);
 }
@@ -376,25 +376,25 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
   // NB-level factors: X = L_sb^-1 by forward substitution on the identity
   double* lsb = linvsb.as<double>();
   for (int b = 0; b < nsb; ++b) {
-    const int r0 = b * SB, ns = std::min(SB, n - r0);
-    double* X = lsb + (size_t)b * SB * SB;   // ns x ns, ld SB
+    const int r0 = b * sb, ns = std::min(sb, n - r0);
+    double* X = lsb + (size_t)b * sb * sb;   // ns x ns, ld sb
     DVR_LAUNCH("ldl_ident", 8.0 * ns * ns, 0.0, k_identity, dim3(grid_for((long)ns * ns)),
-               dim3(256), 0, s, X, (long)SB, ns);
+               dim3(256), 0, s, X, (long)sb, ns);
     for (int j0 = 0; j0 < ns; j0 += NB) {
       const int nb = std::min(NB, ns - j0);
       const double* li = l64 + (size_t)((r0 + j0) / NB) * NB * NB;
       // rows j0..j0+nb of X <- Linv64 * rows j0..j0+nb (only columns <= j0+nb
       // are non-zero; columns 0..j0+nb are processed)
       const int ncol = j0 + nb;
-      launch_gemm(s, "ldl_inv", false, false, nb, ncol, nb, 1.0, li, nb, X + j0, SB, 0.0, W, nb, 1,
+      launch_gemm(s, "ldl_inv", false, false, nb, ncol, nb, 1.0, li, nb, X + j0, sb, 0.0, W, nb, 1,
                   nullptr, false);
       DVR_LAUNCH("ldl_copy", 16.0 * nb * ncol, 0.0, k_copy2d, dim3(grid_for((long)nb * ncol)),
-                 dim3(256), 0, s, W, (long)nb, X + j0, (long)SB, nb, ncol);
+                 dim3(256), 0, s, W, (long)nb, X + j0, (long)sb, nb, ncol);
       const int mr = ns - j0 - nb;
       if (mr > 0) {
         const double* L = a + (r0 + j0 + nb) + (long)(r0 + j0) * lda;
-        launch_gemm(s, "ldl_inv", false, false, mr, ncol, nb, -1.0, L, lda, X + j0, SB, 1.0,
-                    X + j0 + nb, SB, 1, nullptr, false);
+        launch_gemm(s, "ldl_inv", false, false, mr, ncol, nb, -1.0, L, lda, X + j0, sb, 1.0,
+                    X + j0 + nb, sb, 1, nullptr, false);
       }
     }
   }
@@ -427,9 +427,9 @@ void LdlFactor::solve(double* R, double* T, int nrhs, hipStream_t s) {
   const double* lsb = linvsb.as<double>();
   // forward: Y = L^-1 R, Y stored in T
   for (int b = 0; b < nsb; ++b) {
-    const int r0 = b * SB, ns = std::min(SB, n - r0);
-    const double* X = lsb + (size_t)b * SB * SB;
-    gemm_ks(s, work, tuner, "si_fwd_diag", false, false, ns, nrhs, ns, 1.0, X, SB, R + r0, n, 0.0,
+    const int r0 = b * sb, ns = std::min(sb, n - r0);
+    const double* X = lsb + (size_t)b * sb * sb;
+    gemm_ks(s, work, tuner, "si_fwd_diag", false, false, ns, nrhs, ns, 1.0, X, sb, R + r0, n, 0.0,
             T + r0, n);
     const int m = n - r0 - ns;
     if (m > 0)
@@ -441,9 +441,9 @@ void LdlFactor::solve(double* R, double* T, int nrhs, hipStream_t s) {
   // backward: Z = L^-T Y.  The result of super-block b goes to R (free now),
   // and Y above it is updated in T.
   for (int b = nsb - 1; b >= 0; --b) {
-    const int r0 = b * SB, ns = std::min(SB, n - r0);
-    const double* X = lsb + (size_t)b * SB * SB;
-    gemm_ks(s, work, tuner, "si_bwd_diag", true, false, ns, nrhs, ns, 1.0, X, SB, T + r0, n, 0.0, R + r0,
+    const int r0 = b * sb, ns = std::min(sb, n - r0);
+    const double* X = lsb + (size_t)b * sb * sb;
+    gemm_ks(s, work, tuner, "si_bwd_diag", true, false, ns, nrhs, ns, 1.0, X, sb, T + r0, n, 0.0, R + r0,
             n);
     if (r0 > 0)
       gemm_ks(s, work, tuner, "si_bwd_upd", true, false, r0, nrhs, ns, -1.0, a + r0, lda, R + r0, n, 1.0,
