@@ -734,10 +734,14 @@ int dvr_get_vectors(const dvr_ctx_t* c, double* vec) {
 int dvr_get_solver_stats(const dvr_ctx_t* c, int32_t* st, int32_t n) {
   if (!c || !c->ran || !st || n < 1) return DVR_ERR_ARG;
   const DavStats& d = c->dav.stats;
-  const int32_t v[9] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
-                        d.slices, d.certified, d.missing,
-                        c->dims.solver == DVR_SOLVER_DAVIDSON && c->dav_status != 0};
-  for (int i = 0; i < n && i < 9; ++i) st[i] = v[i];
+  const int32_t v[13] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
+                         d.slices, d.certified, d.missing,
+                         c->dims.solver == DVR_SOLVER_DAVIDSON && c->dav_status != 0,
+                         (int32_t)std::lround(d.phase_ms[0] * 1e3),
+                         (int32_t)std::lround(d.phase_ms[1] * 1e3),
+                         (int32_t)std::lround(d.phase_ms[2] * 1e3),
+                         (int32_t)std::lround(d.phase_ms[3] * 1e3)};
+  for (int i = 0; i < n && i < 13; ++i) st[i] = v[i];
   return DVR_OK;
 }
 

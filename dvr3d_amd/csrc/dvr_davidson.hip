@@ -140,7 +140,40 @@ int DavidsonSolver::cholqr(const double* Z, int q, double* tmp, double* out, hip
   return (int)keep.size();
 }
 
+void PhaseClock::mark(int label, hipStream_t s) {
+  if (used == (int)ev.size()) {
+    hipEvent_t e;
+    DVR_HIP(hipEventCreate(&e));
+    ev.push_back(e);
+    lab.push_back(-1);
+  }
+  lab[used] = label;
+  DVR_HIP(hipEventRecord(ev[used], s));
+  ++used;
+}
+
+void PhaseClock::collect(double* out) {
+  for (int i = 0; i + 1 < used; ++i) {
+    if (lab[i] < 0) continue;
+    float ms = 0.0f;
+    DVR_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+    out[lab[i]] += ms;
+  }
+  used = 0;
+}
+
+enum { PH_RR = 0, PH_SI = 1, PH_EXPAND = 2, PH_FACTOR = 3 };
+
 int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_t s) {
+  clock.used = 0;
+  const int rc = solve_impl(H, n, P, s);
+  clock.mark(-1, s);
+  DVR_HIP(hipStreamSynchronize(s));
+  clock.collect(stats.phase_ms);
+  return rc;
+}
+
+int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipStream_t s) {
   stats = DavStats{};
   N = n;
   const int nev = std::min(P.nev, n);
@@ -229,6 +262,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
     gemm(s, "proj", true, false, m, m, N, 1.0, V.as<double>(), N, AV.as<double>(), N, 0.0, T, m);
     small_eig(m, T, nullptr, w0, s);
     sigma = w0[0];
+    clock.mark(PH_FACTOR, s);
     ldl.factor(H, sigma, s);
     stats.factorizations++;
   }
@@ -241,6 +275,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
     for (;;) {
       if (stats.iterations >= P.max_iter) return 2;   // stalled
       stats.iterations++;
+      clock.mark(PH_RR, s);
       rayleigh_ritz();
       int nl = 0;
       while (nl < m && nrm[nl] <= tol && locked_slice + nl < K && nlocked + nl < nlmax) ++nl;
@@ -272,6 +307,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
       if (mk + (int)act.size() > mmax) mk = keep > 0 ? std::min(keep, mmax - (int)act.size())
                                                      : mmax - (int)act.size();
       const int na = (int)act.size();
+      clock.mark(PH_SI, s);
       launch_resid_gather(s, X.as<double>(), AX.as<double>(), N, thd.as<double>(),
                           upload(idxd, act, s), na, Rb.as<double>(), N, N);
       if (si) {
@@ -290,6 +326,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
       }
       stats.solves += na;
       // restart on the kept Ritz vectors (contiguous columns nl .. nl+mk)
+      clock.mark(PH_EXPAND, s);
       if (mk > 0) {
         DVR_HIP(hipMemcpyAsync(V.p, X.as<double>() + (long)nl * N, sizeof(double) * (size_t)N * mk,
                                hipMemcpyDeviceToDevice, s));
@@ -305,6 +342,7 @@ int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_
     const double top = *std::max_element(thL.begin(), thL.end());
     const double sig_hi = have_next ? 0.5 * (top + th_next)
                                     : top + 1e-6 * (std::fabs(top) + 1.0);
+    clock.mark(PH_FACTOR, s);
     const int nu = ldl.factor(H, sig_hi, s);
     stats.factorizations++;
     sigma = sig_hi;

@@ -582,11 +582,30 @@ struct DavParams {
 struct DavStats {
   int iterations = 0, matvecs = 0, solves = 0, factorizations = 0, slices = 0;
   int certified = 0, missing = 0;
+  // device time per phase (HIP events on the solver's stream, ms):
+  // Rayleigh-Ritz, shift-and-invert solves, basis expansion, factorisations
+  double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
+};
+// HIP timing events reused across solves (created on demand)
+struct PhaseClock {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> lab;
+  int used = 0;
+  PhaseClock() = default;
+  PhaseClock(const PhaseClock&) = delete;
+  PhaseClock& operator=(const PhaseClock&) = delete;
+  ~PhaseClock() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  void mark(int label, hipStream_t s);
+  // adds the time from each mark to the next one to out[label]
+  void collect(double* out);
 };
 class DavidsonSolver {
  public:
   // 0: certified; 2: stalled; 3: a level below the last shift was missed
   int solve(const double* H, int n, const DavParams& P, hipStream_t s);
+  int solve_impl(const double* H, int n, const DavParams& P, hipStream_t s);
   // the lowest nvec eigenvectors (ascending) into Z (n x nvec)
   void vectors(int nvec, double* Z, hipStream_t s);
   std::vector<double> eval;
@@ -611,6 +630,7 @@ class DavidsonSolver {
   DevBuf gd, gsc, gr, gm, Dd, Fd, Wd, tha;
   EigWork rr;
   LdlFactor ldl;
+  PhaseClock clock;
   std::vector<double> thL, gw;
   std::vector<int> perm;
 };

@@ -13,7 +13,18 @@
 // chain is arithmetic.
 #pragma once
 
+#include <type_traits>
+
 namespace dvr {
+
+// 1/x from the hardware reciprocal estimate and two Newton steps: within a
+// few ulps for normal x, and about half the dependent instructions of the
+// correctly rounded division on the recurrences below
+__device__ inline double rcp_refined(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
 
 __device__ inline double ii_hash(unsigned long long x) {
   x += 0x9E3779B97F4A7C15ull;
@@ -25,131 +36,105 @@ __device__ inline double ii_hash(unsigned long long x) {
 
 // s0: this thread's scratch; array a (0..5) starts at s0 + a*astride and its
 // element i is at + i*stride.  Vector k is written to Zp + k*n.
+//
+// The three recurrences (factorisation, forward and backward sweeps) move
+// CH elements between memory and registers per burst.  Full bursts carry no
+// per-element guards, and the pivoting choice is a select rather than a
+// branch, so a burst is straight-line code whose only dependent chain is the
+// arithmetic of the recurrence.
 template <int CH>
 __device__ __forceinline__ void inverse_iteration_cluster(int n, const double* __restrict__ d,
-                                                 const double* __restrict__ e,
-                                                 const double* __restrict__ lams, int kb, int ke,
-                                                 double tn, double* s0, long astride, long stride,
-                                                 double* Zp, unsigned long long seed) {
+                                                          const double* __restrict__ e,
+                                                          const double* __restrict__ lams,
+                                                          int kb, int ke, double tn, double* s0,
+                                                          long astride, long stride,
+                                                          double* Zp, unsigned long long seed) {
   double* FL = s0;
   double* FD = s0 + astride;
   double* FU = s0 + 2 * astride;
   double* FU2 = s0 + 3 * astride;
   double* X = s0 + 4 * astride;
-  double* PV = s0 + 5 * astride;   // pivot flags of rows >= 128
+  double* PV = s0 + 5 * astride;   // 1 where rows i and i+1 were interchanged
 #define II(arr, i) (arr)[(long)(i) * stride]
   constexpr double eps = 2.220446049250313e-16;
   const double tnn = tn > 0.0 ? tn : 1.0;
   const double pertol = 10.0 * eps * tnn, tiny = eps * tnn;
+  const int nm1 = n - 1;                 // elimination steps 0 .. n-2
+  const int nfull = nm1 / CH * CH;       // steps covered by full bursts
   double lprev = 0.0;
-  // CH: elements moved between memory and registers per burst
   for (int k = kb; k < ke; ++k) {
     double lam = lams[k];
     if (k > kb && lam - lprev < pertol) lam = lprev + pertol;
     lprev = lam;
-    // factorisation; pivot flags of rows < 128 in two register masks, the
-    // others in PV
-    unsigned long long pv0 = 0ull, pv1 = 0ull;
+    // factorisation T - lam I = P L U (the dgttrf recurrence): step i
+    // eliminates row i+1 with row i or, when |T(i+1,i)| > |pivot|, after
+    // interchanging them
     double dc = d[0] - lam, uc = n > 1 ? e[0] : 0.0;
-    for (int i0 = 0; i0 < n - 1; i0 += CH) {
-      double ev[CH], dv[CH], o_fl[CH], o_fd[CH], o_fu[CH], o_fu2[CH];
-      bool o_sw[CH];
+    auto fact_burst = [&](int i0, auto full) {
+      constexpr bool F = decltype(full)::value;
+      double ev[CH], dv[CH];
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
         const int i = i0 + q;
-        ev[q] = (i < n) ? e[i] : 0.0;          // e[i] and e[i+1] of step i
-        dv[q] = (i + 1 < n) ? d[i + 1] : 0.0;
+        ev[q] = (F || i < nm1) ? e[i] : 0.0;
+        dv[q] = (F || i < nm1) ? d[i + 1] : 0.0;
       }
-      const double etail = (i0 + CH < n) ? e[i0 + CH] : 0.0;
+      const double etail = (i0 + CH < nm1) ? e[i0 + CH] : 0.0;
 #pragma unroll
       for (int q = 0; q < CH; ++q) {
         const int i = i0 + q;
-        o_sw[q] = false;
-        if (i < n - 1) {
+        if (F || i < nm1) {
           const double li = ev[q];
-          double dn = dv[q] - lam;
+          const double dn = dv[q] - lam;
           const double enext = (q + 1 < CH) ? ev[q + 1] : etail;
-          double un = (i < n - 2) ? enext : 0.0;
-          double piv, fl, fu = uc, fu2 = 0.0;
-          if (fabs(dc) >= fabs(li)) {
-            if (dc != 0.0) {
-              fl = li / dc;
-              dn -= fl * uc;
-              piv = dc;
-            } else {
-              fl = 0.0;
-              piv = tiny;
-            }
-          } else {
-            fl = dc / li;
-            piv = li;
-            fu = dn;
-            dn = uc - fl * dn;
-            fu2 = un;
-            un = -fl * un;
-            o_sw[q] = true;
-          }
-          o_fl[q] = fl;
-          o_fd[q] = 1.0 / (piv == 0.0 ? tiny : piv);
-          o_fu[q] = fu;
-          o_fu2[q] = fu2;
-          dc = dn;
-          uc = un;
+          const double un = (i < n - 2) ? enext : 0.0;
+          const bool sw = fabs(dc) < fabs(li);
+          const double den = sw ? li : dc, num = sw ? dc : li;
+          const double rden = rcp_refined(den == 0.0 ? tiny : den);
+          const double fl = num * rden;   // den == 0 only with li == 0
+          II(FL, i) = fl;
+          II(FD, i) = rden;
+          II(FU, i) = sw ? dn : uc;
+          II(FU2, i) = sw ? un : 0.0;
+          II(PV, i) = sw ? 1.0 : 0.0;
+          dc = sw ? uc - fl * dn : dn - fl * uc;
+          uc = sw ? -fl * un : un;
         }
       }
-#pragma unroll
-      for (int q = 0; q < CH; ++q) {
-        const int i = i0 + q;
-        if (i < n - 1) {
-          II(FL, i) = o_fl[q];
-          II(FD, i) = o_fd[q];
-          II(FU, i) = o_fu[q];
-          II(FU2, i) = o_fu2[q];
-          if (i < 64) pv0 |= (unsigned long long)o_sw[q] << i;
-          else if (i < 128) pv1 |= (unsigned long long)o_sw[q] << (i - 64);
-          else II(PV, i) = o_sw[q] ? 1.0 : 0.0;
-        }
-      }
-    }
+    };
+    for (int i0 = 0; i0 < nfull; i0 += CH) fact_burst(i0, std::true_type{});
+    if (nfull < nm1) fact_burst(nfull, std::false_type{});
     II(FD, n - 1) = 1.0 / (dc == 0.0 ? tiny : dc);
     double scale = 1.0;
     for (int it = 0; it < 3; ++it) {
       const unsigned long long base = (seed << 40) ^ ((unsigned long long)k << 20);
       // forward sweep: L^-1 with the row interchanges
       double xc = it == 0 ? ii_hash(base) : II(X, 0) * scale;
-      for (int i0 = 0; i0 < n - 1; i0 += CH) {
-        double fl[CH], xn[CH], out[CH];
-        bool sw[CH];
+      auto fwd_burst = [&](int i0, auto full) {
+        constexpr bool F = decltype(full)::value;
+        double fl[CH], xn[CH], sw[CH];
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
           const int i = i0 + q;
-          if (i < n - 1) {
+          if (F || i < nm1) {
             fl[q] = II(FL, i);
+            sw[q] = II(PV, i);
             xn[q] = it == 0 ? ii_hash(base ^ (unsigned long long)(i + 1)) : II(X, i + 1) * scale;
-            if (i < 64) sw[q] = (pv0 >> i) & 1ull;
-            else if (i < 128) sw[q] = (pv1 >> (i - 64)) & 1ull;
-            else sw[q] = II(PV, i) != 0.0;
           }
         }
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
           const int i = i0 + q;
-          if (i < n - 1) {
-            double x = xn[q];
-            if (!sw[q]) {
-              x -= fl[q] * xc;
-              out[q] = xc;
-            } else {
-              out[q] = x;
-              x = xc - fl[q] * x;
-            }
-            xc = x;
+          if (F || i < nm1) {
+            const bool s = sw[q] != 0.0;
+            const double a = s ? xn[q] : xc, bq = s ? xc : xn[q];
+            xc = bq - fl[q] * a;
+            II(X, i) = a;
           }
         }
-#pragma unroll
-        for (int q = 0; q < CH; ++q)
-          if (i0 + q < n - 1) II(X, i0 + q) = out[q];
-      }
+      };
+      for (int i0 = 0; i0 < nfull; i0 += CH) fwd_burst(i0, std::true_type{});
+      if (nfull < nm1) fwd_burst(nfull, std::false_type{});
       // backward sweep: U^-1, tracking max |x|
       double x1 = xc * II(FD, n - 1);
       II(X, n - 1) = x1;
@@ -160,12 +145,14 @@ __device__ __forceinline__ void inverse_iteration_cluster(int n, const double* _
         II(X, n - 2) = x0;
         mxv = fmax(mxv, fabs(x0));
       }
-      for (int i1 = n - 3; i1 >= 0; i1 -= CH) {
+      // rows n-3 .. 0, CH at a time from the top end
+      auto bwd_burst = [&](int i1, auto full) {
+        constexpr bool F = decltype(full)::value;
         double xv[CH], fu[CH], fu2[CH], fd[CH];
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
           const int i = i1 - q;
-          if (i >= 0) {
+          if (F || i >= 0) {
             xv[q] = II(X, i);
             fu[q] = II(FU, i);
             fu2[q] = II(FU2, i);
@@ -174,18 +161,18 @@ __device__ __forceinline__ void inverse_iteration_cluster(int n, const double* _
         }
 #pragma unroll
         for (int q = 0; q < CH; ++q) {
-          if (i1 - q >= 0) {
+          if (F || i1 - q >= 0) {
             const double xi = (xv[q] - fu[q] * x0 - fu2[q] * x1) * fd[q];
-            xv[q] = xi;
             mxv = fmax(mxv, fabs(xi));
             x1 = x0;
             x0 = xi;
+            II(X, i1 - q) = xi;
           }
         }
-#pragma unroll
-        for (int q = 0; q < CH; ++q)
-          if (i1 - q >= 0) II(X, i1 - q) = xv[q];
-      }
+      };
+      int i1 = n - 3;
+      for (; i1 - (CH - 1) >= 0; i1 -= CH) bwd_burst(i1, std::true_type{});
+      if (i1 >= 0) bwd_burst(i1, std::false_type{});
       const double sc = mxv > 0.0 ? 1.0 / mxv : 1.0;
       if (k > kb) {
         for (int i = 0; i < n; ++i) II(X, i) *= sc;

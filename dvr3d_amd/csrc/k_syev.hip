@@ -474,14 +474,34 @@ void launch_tridiag_bounds(hipStream_t s, const EigBatch& b) {
   DVR_LAUNCH_CHECK();
 }
 
+// Sturm count: the number of negative pivots of T - x I = L D L^T.  The
+// quotient e2/q uses rcp_refined (the recurrence's critical path);
+// |q| >= pivmin keeps it away from denormals.  d and e2 are read eight at a
+// time.
+
 __device__ inline int sturm_count(const double* __restrict__ d,
                                   const double* __restrict__ e2, int n,
                                   double x, double pivmin) {
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   int c = q <= 0.0;
-  for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
+  int i = 1;
+  for (; i + 7 < n; i += 8) {
+    double dv[8], ev[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      dv[u] = d[i + u];
+      ev[u] = e2[i + u - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      q = (dv[u] - x) - ev[u] * rcp_refined(q);
+      if (fabs(q) < pivmin) q = -pivmin;
+      c += q <= 0.0;
+    }
+  }
+  for (; i < n; ++i) {
+    q = (d[i] - x) - e2[i - 1] * rcp_refined(q);
     if (fabs(q) < pivmin) q = -pivmin;
     c += q <= 0.0;
   }
@@ -560,7 +580,7 @@ k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
   const int p = C.p;
   const long vo = b.voff[p];
   const long nm = b.nmax;
-  inverse_iteration_cluster<16>(b.n[p], b.d + vo, b.e + vo, eval + eoff[p], C.kb, C.ke, b.tnrm[p],
+  inverse_iteration_cluster<8>(b.n[p], b.d + vo, b.e + vo, eval + eoff[p], C.kb, C.ke, b.tnrm[p],
                             scratch + t, nm * T, T, Z + zoff[p], (unsigned long long)p);
 }
 
@@ -573,6 +593,12 @@ k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
 // touches TW consecutive doubles.  When all of a work-group's clusters belong
 // to one problem its tridiagonal is staged in LDS as well.
 constexpr int LDS_BYTES_MAX = 160 * 1024;
+
+// DVR_BT=global keeps the back-transformation on the global-memory kernels
+bool bt_use_lds() {
+  const char* v = getenv("DVR_BT");
+  return !(v && std::string(v) == "global");
+}
 
 // DVR_STEIN=global keeps every problem on the global-memory version
 bool stein_use_lds() {
@@ -780,6 +806,155 @@ k_ormtr(EigBatch b, double* __restrict__ Z, const long* __restrict__ zoff,
   }
 }
 
+// LDS-resident versions for problems of at most BT_LDS rows (the
+// Rayleigh-Ritz problems of the iterative solver): the global-memory kernels
+// above reload the reflectors and the eigenvector tile from memory for every
+// 64-row chunk of every block, a round trip per chunk.  Here a work-group
+// stages the whole reflector block (and, in the back-transform, its
+// eigenvector tile) in LDS once per block.
+constexpr int BT_LDS = 448;
+constexpr int VLD = NB + 1;   // row stride of a staged reflector block
+
+// T factor of block blk: G = V^T V over the block's rows, then the forward
+// recurrence T(0:c, c) = -tau_c T(0:c, 0:c) G(0:c, c), T(c, c) = tau_c.
+__global__ void __launch_bounds__(TB)
+k_larft_lds(EigBatch b, const int* __restrict__ nblk_ofs, double* __restrict__ tbuf,
+            const long* __restrict__ toff) {
+  extern __shared__ double sh[];
+  const int bid = blockIdx.x;
+  int lo_p = 0, hi_p = b.nprob - 1;
+  while (lo_p < hi_p) {
+    const int mid = (lo_p + hi_p + 1) >> 1;
+    if (nblk_ofs[mid] <= bid) lo_p = mid; else hi_p = mid - 1;
+  }
+  const int p = lo_p;
+  const int blk = bid - nblk_ofs[p];
+  const int n = b.n[p];
+  const int nr = n - 2;
+  const int j0 = blk * NB;
+  if (nr <= 0 || j0 >= nr) return;
+  const int nbk = min(NB, nr - j0);
+  const int i0 = j0 + 1, nrow = n - i0;
+  const double* A = b.A + b.aoff[p];
+  const double* tau = b.tau + b.voff[p];
+  double* Vl = sh;                       // nrow x VLD
+  double* G = Vl + (long)nrow * VLD;     // NB x NB
+  double* Ts = G + NB * NB;              // NB x NB
+  const int tid = threadIdx.x;
+  for (int r = 0; r < NB; ++r)
+    for (int il = tid; il < nrow; il += TB)
+      Vl[il * VLD + r] = r < nbk ? refl(A, n, j0 + r, i0 + il) : 0.0;
+  for (int e = tid; e < NB * NB; e += TB) Ts[e] = 0.0;
+  __syncthreads();
+  for (int e = tid; e < NB * NB; e += TB) {
+    const int r = e % NB, c = e / NB;
+    double g = 0.0;
+    if (r < c && c < nbk) {
+      double g0 = 0.0, g1 = 0.0;
+      int il = 0;
+      for (; il + 1 < nrow; il += 2) {
+        g0 += Vl[il * VLD + r] * Vl[il * VLD + c];
+        g1 += Vl[(il + 1) * VLD + r] * Vl[(il + 1) * VLD + c];
+      }
+      if (il < nrow) g0 += Vl[il * VLD + r] * Vl[il * VLD + c];
+      g = g0 + g1;
+    }
+    G[e] = g;
+  }
+  __syncthreads();
+  for (int c = 0; c < nbk; ++c) {
+    const double tc = tau[j0 + c];
+    if (tid < c) {
+      const int r = tid;
+      double sacc = 0.0;
+      for (int l = r; l < c; ++l) sacc += Ts[r + NB * l] * G[l + NB * c];
+      Ts[r + NB * c] = -tc * sacc;
+    }
+    if (tid == 0) Ts[c + NB * c] = tc;
+    __syncthreads();
+  }
+  double* Tout = tbuf + toff[p] + (long)blk * NB * NB;
+  for (int e = tid; e < NB * NB; e += TB) Tout[e] = Ts[e];
+}
+
+// Z tile (all n rows, CT columns) in LDS for the whole back-transform; per
+// block: stage V and T, W = V^T Z, W2 = T W, Z -= V W2.
+__global__ void __launch_bounds__(TB)
+k_ormtr_lds(EigBatch b, double* __restrict__ Z, const long* __restrict__ zoff,
+            const int* __restrict__ m, const int* __restrict__ tile_ofs,
+            const double* __restrict__ tbuf, const long* __restrict__ toff) {
+  extern __shared__ double sh[];
+  const int bid = blockIdx.x;
+  int lo_p = 0, hi_p = b.nprob - 1;
+  while (lo_p < hi_p) {
+    const int mid = (lo_p + hi_p + 1) >> 1;
+    if (tile_ofs[mid] <= bid) lo_p = mid; else hi_p = mid - 1;
+  }
+  const int p = lo_p;
+  const int tile = bid - tile_ofs[p];
+  const int n = b.n[p];
+  const int mp = m[p];
+  const int c0 = tile * CT;
+  if (c0 >= mp) return;
+  const int ct = min(CT, mp - c0);
+  const int nr = n - 2;
+  if (nr <= 0) return;
+  const double* A = b.A + b.aoff[p];
+  double* Zp = Z + zoff[p] + (long)c0 * n;
+  const double* Tp = tbuf + toff[p];
+  const int nmx = b.nmax;
+  double* Vl = sh;                        // (nmx - 1) x VLD
+  double* Zl = Vl + (long)(nmx - 1) * VLD;  // n x CT, row-major
+  double* W = Zl + (long)nmx * CT;        // NB x CT
+  double* W2 = W + NB * CT;               // NB x CT
+  double* Tl = W2 + NB * CT;              // NB x NB
+  const int tid = threadIdx.x;
+  for (int c = 0; c < CT; ++c)
+    for (int i = tid; i < n; i += TB) Zl[i * CT + c] = c < ct ? Zp[i + (long)c * n] : 0.0;
+  const int nblk = (nr + NB - 1) / NB;
+  const int tr = tid / CT, tc = tid % CT;   // owner of W(tr, tc)
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int j0 = blk * NB;
+    const int nbk = min(NB, nr - j0);
+    const int i0 = j0 + 1, nrow = n - i0;
+    __syncthreads();   // previous block's update of Zl done
+    for (int r = 0; r < NB; ++r)
+      for (int il = tid; il < nrow; il += TB)
+        Vl[il * VLD + r] = r < nbk ? refl(A, n, j0 + r, i0 + il) : 0.0;
+    const double* Tb = Tp + (long)blk * NB * NB;
+    for (int e = tid; e < NB * NB; e += TB) Tl[e] = Tb[e];
+    __syncthreads();
+    {
+      double w0 = 0.0, w1 = 0.0;
+      const double* zc = Zl + (long)i0 * CT + tc;
+      int il = 0;
+      for (; il + 1 < nrow; il += 2) {
+        w0 += Vl[il * VLD + tr] * zc[il * CT];
+        w1 += Vl[(il + 1) * VLD + tr] * zc[(il + 1) * CT];
+      }
+      if (il < nrow) w0 += Vl[il * VLD + tr] * zc[il * CT];
+      W[tr * CT + tc] = w0 + w1;
+    }
+    __syncthreads();
+    {
+      double sacc = 0.0;
+      for (int l = tr; l < nbk; ++l) sacc += Tl[tr + NB * l] * W[l * CT + tc];
+      W2[tr * CT + tc] = tr < nbk ? sacc : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < nrow * CT; e += TB) {
+      const int il = e / CT, c = e % CT;
+      double sacc = 0.0;
+#pragma unroll 8
+      for (int r = 0; r < NB; ++r) sacc += Vl[il * VLD + r] * W2[r * CT + c];
+      Zl[(i0 + il) * CT + c] -= sacc;
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < ct; ++c)
+    for (int i = tid; i < n; i += TB) Zp[i + (long)c * n] = Zl[i * CT + c];
+}
+
 void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
                           const long* zoff, const int* m, const int* h_n,
                           const int* h_m, double* tbuf, const long* toff,
@@ -807,12 +982,29 @@ void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
     obytes += 8.0 * (n * n * 0.5 + 2.0 * n * mm * ((n - 2.0) / NB > 1.0 ? (n - 2.0) / NB : 1.0));
     oflops += 4.0 * n * n * mm;
   }
+  const bool lds = b.nmax <= BT_LDS && bt_use_lds();
   if (nbt > 0) {
-    DVR_LAUNCH("larft", 0.0, 0.0, k_larft, dim3(nbt), dim3(TB), 0, s, b, d, tbuf, toff);
+    if (lds) {
+      const size_t sh = sizeof(double) * ((size_t)(b.nmax - 1) * VLD + 2 * NB * NB);
+      DVR_HIP(hipFuncSetAttribute((const void*)k_larft_lds,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+      DVR_LAUNCH("larft", 0.0, 0.0, k_larft_lds, dim3(nbt), dim3(TB), sh, s, b, d, tbuf, toff);
+    } else {
+      DVR_LAUNCH("larft", 0.0, 0.0, k_larft, dim3(nbt), dim3(TB), 0, s, b, d, tbuf, toff);
+    }
   }
   if (ntt > 0) {
-    DVR_LAUNCH("ormtr", obytes, oflops, k_ormtr, dim3(ntt), dim3(TB), 0, s, b, Z, zoff, m,
-                   d + np + 1, tbuf, toff);
+    if (lds) {
+      const size_t sh = sizeof(double) * ((size_t)(b.nmax - 1) * VLD + (size_t)b.nmax * CT +
+                                          2 * NB * CT + NB * NB);
+      DVR_HIP(hipFuncSetAttribute((const void*)k_ormtr_lds,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+      DVR_LAUNCH("ormtr", obytes, oflops, k_ormtr_lds, dim3(ntt), dim3(TB), sh, s, b, Z, zoff, m,
+                 d + np + 1, tbuf, toff);
+    } else {
+      DVR_LAUNCH("ormtr", obytes, oflops, k_ormtr, dim3(ntt), dim3(TB), 0, s, b, Z, zoff, m,
+                 d + np + 1, tbuf, toff);
+    }
   }
   DVR_LAUNCH_CHECK();
   DVR_HIP(hipFreeAsync(d, s));

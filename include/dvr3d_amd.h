@@ -144,7 +144,10 @@ int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
  * [2] H-vector products, [3] shift-and-invert solves, [4] LDL^T
  * factorisations, [5] slices, [6] inertia count at the last shift,
  * [7] levels missing at the failed certification, [8] 1 when the dense
- * solver was used as fallback. */
+ * solver was used as fallback; device time (microseconds, HIP events on the
+ * solver's stream) of [9] the Rayleigh-Ritz steps, [10] the shift-and-invert
+ * solves, [11] the basis expansions (orthogonalisation and H-vector
+ * products), [12] the LDL^T factorisations. */
 int dvr_get_solver_stats(const dvr_ctx_t* ctx, int32_t* stats, int32_t n);
 
 /* Shift-and-invert kernel of the Davidson solver, exposed for testing and
