@@ -380,7 +380,15 @@ void run_pipeline(dvr_ctx_t* C) {
       launch_syrk_split32(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile, C->xkc.as<double>(),
                           nc, 8.0 * ((double)G * n3 + (double)n3 * n3),
                           (double)G * n3 * (n3 + 1.0));
-    else
+    else if (env_int("DVR_SYRK3_GEMM", 1)) {
+      // the lower tiles of cint^T cint on the tuned GEMM (XCD-aware tile
+      // order), then the block scale, the diagonal and the mirror
+      launch_gemm_var(s, "syrk", true, false, n3, n3, (int)G, 1.0, C->cint.as<double>(), G,
+                      C->cint.as<double>(), G, 0.0, C->H3.as<double>(), n3, true,
+                      env_int("DVR_SYRK3_KT", 16) == 32 ? 32 : 16, false, true);
+      launch_scale_mirror(s, C->H3.as<double>(), n3, C->xkc.as<double>(), nc, d_blk3,
+                          desc[0].diag);
+    } else
       launch_syrk_scaled(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile,
                          C->xkc.as<double>(), nc,
                          8.0 * ((double)G * n3 + (double)n3 * n3),

@@ -179,6 +179,10 @@ struct SyrkDesc {
   int tile0;  // first tile index of this group in the flat launch
   int ntile;  // number of lower-triangular tiles
 };
+// After a lower-tile C = X^T X: C(i,j) *= xk(blk(i), blk(j)), + dg(i) on
+// the diagonal, mirrored to the upper triangle (C: n x n, ld n).
+void launch_scale_mirror(hipStream_t s, double* C, int n, const double* xk, int ldxk,
+                         const int* blk, const double* dg);
 // bytes/flops: algorithmic traffic and work of the launch (profiling only).
 void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
                         int total_tiles, const double* xk, int ldxk,

@@ -206,6 +206,51 @@ void launch_syrk_split32(hipStream_t s, const SyrkDesc* d_desc, int ngroup, int 
   DVR_LAUNCH_CHECK();
 }
 
+// Scale and mirror after a lower-tile product C = X^T X (one group): for
+// i >= j, C(i,j) <- C(i,j) * xk(blk(i), blk(j)) (+ dg(i) on the diagonal),
+// rounded one by one as in mkham2/loadh, and C(j,i) <- C(i,j).  One
+// work-group per lower 64x64 tile; the mirrored tile is stored through LDS
+// so that both stores run along columns.
+__global__ void __launch_bounds__(256)
+k_scale_mirror(double* __restrict__ C, int n, const double* __restrict__ xk, int ldxk,
+               const int* __restrict__ blk, const double* __restrict__ dg) {
+  __shared__ double T[TM * LDT];
+  const int t = blockIdx.x;
+  int ti = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  while (ti * (ti + 1) / 2 > t) --ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  const int row0 = ti * TM, col0 = tj * TM;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < TM * TM; e += 256) {
+    const int r = e & (TM - 1), c = e / TM;
+    const int i = row0 + r, j = col0 + c;
+    if (i < n && j < n && i >= j) {
+#pragma clang fp contract(off)
+      double v = C[i + (long)n * j] * xk[blk[i] + (long)ldxk * blk[j]];
+      if (i == j && dg) v = v + dg[i];
+      C[i + (long)n * j] = v;
+      T[r * LDT + c] = v;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < TM * TM; e += 256) {
+    const int a = e & (TM - 1), b = e / TM;
+    const int row = col0 + a, col = row0 + b;   // C(row, col) = C(col, row)
+    if (row < n && col < n && row < col) C[row + (long)n * col] = T[b * LDT + a];
+  }
+}
+
+void launch_scale_mirror(hipStream_t s, double* C, int n, const double* xk, int ldxk,
+                         const int* blk, const double* dg) {
+  const int nt = (n + TM - 1) / TM;
+  const int tiles = nt * (nt + 1) / 2;
+  if (tiles <= 0) return;
+  DVR_LAUNCH("scale_mirror", 8.0 * 2.0 * (double)n * n, 0.0, k_scale_mirror, dim3(tiles),
+             dim3(256), 0, s, C, n, xk, ldxk, blk, dg);
+  DVR_LAUNCH_CHECK();
+}
+
 void launch_syrk_scaled(hipStream_t s, const SyrkDesc* d_desc, int ngroup,
                         int total_tiles, const double* xk, int ldxk,
                         double bytes, double flops) {
