@@ -742,14 +742,15 @@ int dvr_get_vectors(const dvr_ctx_t* c, double* vec) {
 int dvr_get_solver_stats(const dvr_ctx_t* c, int32_t* st, int32_t n) {
   if (!c || !c->ran || !st || n < 1) return DVR_ERR_ARG;
   const DavStats& d = c->dav.stats;
-  const int32_t v[13] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
+  const int32_t v[14] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
                          d.slices, d.certified, d.missing,
                          c->dims.solver == DVR_SOLVER_DAVIDSON && c->dav_status != 0,
                          (int32_t)std::lround(d.phase_ms[0] * 1e3),
                          (int32_t)std::lround(d.phase_ms[1] * 1e3),
                          (int32_t)std::lround(d.phase_ms[2] * 1e3),
-                         (int32_t)std::lround(d.phase_ms[3] * 1e3)};
-  for (int i = 0; i < n && i < 13; ++i) st[i] = v[i];
+                         (int32_t)std::lround(d.phase_ms[3] * 1e3),
+                         (int32_t)std::lround(d.phase_ms[4] * 1e3)};
+  for (int i = 0; i < n && i < 14; ++i) st[i] = v[i];
   return DVR_OK;
 }
 
@@ -788,11 +789,8 @@ int dvr_mlp_train(dvr_ctx_t* c, int32_t steps, int32_t batch, uint64_t seed, flo
     DevBuf P, Dd;
     P.ensure(sizeof(float) * MLP_NPARAM);
     DVR_HIP(hipMemcpyAsync(P.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, s));
-    std::vector<double> dg(n3);
-    DVR_HIP(hipMemcpy2DAsync(dg.data(), sizeof(double), c->H3keep.p, sizeof(double) * (n3 + 1),
-                             sizeof(double), n3, hipMemcpyDeviceToHost, s));
-    DVR_HIP(hipStreamSynchronize(s));
-    upload(Dd, dg, s);
+    Dd.ensure(sizeof(double) * n3);
+    launch_get_diag(s, c->H3keep.as<double>(), n3, Dd.as<double>());
     std::vector<double> lam(c->h_eval.begin(), c->h_eval.begin() + nz);
     std::vector<double> hist;
     const int rc = mlp_train(s, c->H3keep.as<double>(), n3, c->Z3.as<double>(), nz, lam,

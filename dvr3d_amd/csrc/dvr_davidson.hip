@@ -162,7 +162,7 @@ void PhaseClock::collect(double* out) {
   used = 0;
 }
 
-enum { PH_RR = 0, PH_SI = 1, PH_EXPAND = 2, PH_FACTOR = 3 };
+enum { PH_RR = 0, PH_SI = 1, PH_EXPAND = 2, PH_FACTOR = 3, PH_START = 4 };
 
 int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_t s) {
   clock.used = 0;
@@ -210,8 +210,10 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
 
   // start: unit vectors on the b smallest diagonal elements
   std::vector<double> dg(N);
-  DVR_HIP(hipMemcpy2DAsync(dg.data(), sizeof(double), H, sizeof(double) * (N + 1), sizeof(double),
-                           N, hipMemcpyDeviceToHost, s));
+  clock.mark(PH_START, s);
+  Dd.ensure(sizeof(double) * N);
+  launch_get_diag(s, H, N, Dd.as<double>());
+  DVR_HIP(hipMemcpyAsync(dg.data(), Dd.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
   DVR_HIP(hipStreamSynchronize(s));
   std::vector<int> ord(N);
   std::iota(ord.begin(), ord.end(), 0);
@@ -219,11 +221,9 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
                     [&](int x, int y) { return dg[x] < dg[y] || (dg[x] == dg[y] && x < y); });
   std::vector<int> idx0(ord.begin(), ord.begin() + b);
   {
-    std::vector<double> v0((size_t)N * b, 0.0);
-    for (int c = 0; c < b; ++c) v0[(size_t)c * N + idx0[c]] = 1.0;
-    DVR_HIP(hipMemcpyAsync(V.p, v0.data(), sizeof(double) * v0.size(), hipMemcpyHostToDevice, s));
-    launch_gather_cols(s, H, N, upload(idxd, idx0, s), b, AV.as<double>(), N, N);
-    DVR_HIP(hipStreamSynchronize(s));
+    const int* di = upload(idxd, idx0, s);
+    launch_unit_cols(s, V.as<double>(), N, di, b);
+    launch_gather_cols(s, H, N, di, b, AV.as<double>(), N, N);
   }
   m = b;
   stats.matvecs += b;
@@ -246,8 +246,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
 
   const bool si = P.prec == 0;
   if (!si) {
-    Dd.ensure(sizeof(double) * N);
-    DVR_HIP(hipMemcpyAsync(Dd.p, dg.data(), sizeof(double) * N, hipMemcpyHostToDevice, s));
+    // Dd already holds the diagonal
     if (P.prec == 2) {
       if (!P.mlp || !P.st) throw HipError("davidson: learned preconditioner without parameters");
       Fd.ensure(sizeof(float) * (size_t)N * b * MLP_NF);

@@ -588,7 +588,8 @@ struct DavStats {
   int certified = 0, missing = 0;
   // device time per phase (HIP events on the solver's stream, ms):
   // Rayleigh-Ritz, shift-and-invert solves, basis expansion, factorisations
-  double phase_ms[4] = {0.0, 0.0, 0.0, 0.0};
+  // and the start (diagonal, start block, first projection)
+  double phase_ms[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 };
 // HIP timing events reused across solves (created on demand)
 struct PhaseClock {
@@ -686,6 +687,10 @@ int mlp_train(hipStream_t s, const double* H, int N, const double* Z, int nz,
 // FP64 refinement with the exact operator (dvr_refine.hip)
 // ---------------------------------------------------------------------------
 void launch_add_diag(hipStream_t s, double* Y, const double* X, const double* e, int N, int nv);
+// out[i] = H[i, i]
+void launch_get_diag(hipStream_t s, const double* H, int N, double* out);
+// column c of V (N x b, ld N) <- e_{idx[c]}
+void launch_unit_cols(hipStream_t s, double* V, int N, const int* idx, int b);
 struct RefineWork {
   DevBuf U, V, Y, T, Z, X2, Y2, th, nrm, work;
   EigWork eig;

@@ -161,6 +161,34 @@ k_add_diag(double* __restrict__ Y, const double* __restrict__ X, const double* _
     Y[t] += e[t % N] * X[t];
 }
 
+// out[i] = H[i, i]  (H: N x N, ld N)
+__global__ void __launch_bounds__(256)
+k_get_diag(const double* __restrict__ H, int N, double* __restrict__ out) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < N; i += (long)gridDim.x * 256)
+    out[i] = H[i * (N + 1L)];
+}
+
+// column c of V (N x b, ld N) <- the unit vector e_{idx[c]}
+__global__ void __launch_bounds__(256)
+k_unit_cols(double* __restrict__ V, int N, const int* __restrict__ idx, int b) {
+  const long n = (long)N * b;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % N), c = (int)(t / N);
+    V[t] = r == idx[c] ? 1.0 : 0.0;
+  }
+}
+
+void launch_unit_cols(hipStream_t s, double* V, int N, const int* idx, int b) {
+  const long n = (long)N * b;
+  DVR_LAUNCH("unit_cols", 8.0 * n, 0.0, k_unit_cols,
+             dim3((int)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, s, V, N, idx, b);
+}
+
+void launch_get_diag(hipStream_t s, const double* H, int N, double* out) {
+  DVR_LAUNCH("get_diag", 16.0 * N, 0.0, k_get_diag, dim3((N + 255) / 256), dim3(256), 0, s, H, N,
+             out);
+}
+
 void launch_add_diag(hipStream_t s, double* Y, const double* X, const double* e, int N, int nv) {
   const long n = (long)N * nv;
   DVR_LAUNCH("add_diag", 24.0 * n, 2.0 * n, k_add_diag,
