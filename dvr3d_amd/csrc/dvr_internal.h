@@ -236,10 +236,11 @@ void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
                   double* Z, const long* zoff, double* scratch, int nthreads);
 size_t stein_scratch_doubles(int nmax, int nthreads);
 // Back-transformation Z <- Q Z with the stored reflectors (blocked WY).
+struct DevBuf;
 void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
                           const long* zoff, const int* m, const int* h_n,
                           const int* h_m, double* tbuf, const long* toff,
-                          long tbuf_total);
+                          long tbuf_total, DevBuf& ofs);
 int backtransform_nb();
 // Single-work-group solver (k_syev_wg.hip): phase bit 0 = tridiagonalise and
 // find the lowest cnt[p] eigenvalues; bit 1 = the lowest mvec[p] vectors into
@@ -301,7 +302,7 @@ T* upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
 struct EigWork {
   DevBuf n, aoff, voff, d, e, e2, tau, vb0, vb1, wb, pb, gl, gu, tn, piv;
   DevBuf il, kofs, eoff, eval;
-  DevBuf zoff, m, clus, scr, tbuf, toff;
+  DevBuf zoff, m, clus, scr, tbuf, toff, btofs;
   EigBatch b{};
   std::vector<int> hn;
   std::vector<long> hvoff;
@@ -436,12 +437,11 @@ struct EigWork {
     if (ncl == 0) return;
     const int T = ncl;
     scr.ensure(sizeof(double) * stein_scratch_doubles(b.nmax, T));
-    DevBuf dcl;
-    dcl.ensure(sizeof(Cluster) * ncl);
-    DVR_HIP(hipMemcpyAsync(dcl.p, cl.data(), sizeof(Cluster) * ncl,
+    clus.ensure(sizeof(Cluster) * ncl);
+    DVR_HIP(hipMemcpyAsync(clus.p, cl.data(), sizeof(Cluster) * ncl,
                            hipMemcpyHostToDevice, s));
     const long* dz = upload(zoff, zoff_, s);
-    launch_stein(s, b, dcl.as<Cluster>(), ncl, eval.as<double>(),
+    launch_stein(s, b, clus.as<Cluster>(), ncl, eval.as<double>(),
                  eoff.as<long>(), Z, dz, scr.as<double>(), T);
     // WY factors
     const int NBk = backtransform_nb();
@@ -454,7 +454,7 @@ struct EigWork {
     }
     tbuf.ensure(sizeof(double) * (tt + 1));
     launch_backtransform(s, b, Z, dz, upload(m, mv, s), hn.data(), mv.data(),
-                         tbuf.as<double>(), upload(toff, ht, s), tt);
+                         tbuf.as<double>(), upload(toff, ht, s), tt, btofs);
     DVR_HIP(hipStreamSynchronize(s));
   }
 };

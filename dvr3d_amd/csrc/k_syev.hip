@@ -958,7 +958,7 @@ k_ormtr_lds(EigBatch b, double* __restrict__ Z, const long* __restrict__ zoff,
 void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
                           const long* zoff, const int* m, const int* h_n,
                           const int* h_m, double* tbuf, const long* toff,
-                          long tbuf_total) {
+                          long tbuf_total, DevBuf& ofs) {
   (void)tbuf_total;
   // host-side offsets of blocks and tiles, uploaded through a small buffer
   const int np = b.nprob;
@@ -973,8 +973,8 @@ void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
   }
   h[np] = nbt;
   h[np + 1 + np] = ntt;
-  int* d = nullptr;
-  DVR_HIP(hipMallocAsync((void**)&d, sizeof(int) * 2 * (np + 1), s));
+  ofs.ensure(sizeof(int) * 2 * (np + 1));
+  int* d = ofs.as<int>();
   DVR_HIP(hipMemcpyAsync(d, h, sizeof(int) * 2 * (np + 1), hipMemcpyHostToDevice, s));
   double obytes = 0.0, oflops = 0.0;
   for (int p = 0; p < np; ++p) {
@@ -1007,7 +1007,6 @@ void launch_backtransform(hipStream_t s, const EigBatch& b, double* Z,
     }
   }
   DVR_LAUNCH_CHECK();
-  DVR_HIP(hipFreeAsync(d, s));
   DVR_HIP(hipStreamSynchronize(s));
   free(h);
 }
