@@ -206,42 +206,31 @@ k_syev_lds(const double* __restrict__ Ain, long astride, int n, int kcnt,
   lo -= pad;
   hi += pad;
 
-  // ---- eigenvalues: one wavefront per eigenvalue, 64 shifts per step
+  // ---- eigenvalues: one thread per eigenvalue, bisection on the Sturm
+  // count (all lanes read the same d, e2 entry at each step: LDS broadcast).
+  // A wavefront per eigenvalue with 64 shifts per step cut the steps 8-fold
+  // but ran the n/4 eigenvalues of each wavefront one after another.
   double* ev = eval + (long)p * estride;
-  for (int k = wave; k < kcnt; k += NWV) {
+  for (int k = tid; k < kcnt; k += NT) {
     double l = lo, h = hi;
-    for (int it = 0; it < 48; ++it) {
+    for (int it = 0; it < 80; ++it) {
       const double width = h - l;
       if (width <= 2.0 * kEpsL * fmax(fabs(l), fabs(h)) || width <= 4.0 * piv) break;
-      const double step = width / 65.0;
-      const double x = l + (double)(lane + 1) * step;
+      const double x = 0.5 * (l + h);
+      if (x <= l || x >= h) break;
       double q = d[0] - x;
       if (fabs(q) < piv) q = -piv;
       int c = q <= 0.0;
       for (int i = 1; i < n; ++i) {
-        q = (d[i] - x) - e2[i - 1] / q;
+        q = (d[i] - x) - e2[i - 1] * rcp_refined(q);
         if (fabs(q) < piv) q = -piv;
         c += q <= 0.0;
       }
-      const unsigned long long mask = __ballot(c <= k);
-      double nl, nh;
-      if (mask == 0ull) {
-        nl = l;
-        nh = l + step;
-      } else {
-        const int L = 63 - __clzll((long long)mask);
-        nl = l + (double)(L + 1) * step;
-        nh = (L == 63) ? h : l + (double)(L + 2) * step;
-      }
-      if (nl == l && nh == h) break;
-      l = nl;
-      h = nh;
+      if (c <= k) l = x; else h = x;
     }
-    if (lane == 0) {
-      const double lam = 0.5 * (l + h);
-      ev[k] = lam;
-      sv[k] = lam;     // LDS copy for the vector phase
-    }
+    const double lam = 0.5 * (l + h);
+    ev[k] = lam;
+    sv[k] = lam;     // LDS copy for the vector phase
   }
   __syncthreads();
 
@@ -273,7 +262,7 @@ k_syev_lds(const double* __restrict__ Ain, long astride, int n, int kcnt,
   {
     const int c = tid;
     if (c < ncl)
-      inverse_iteration_cluster<1>(n, d, e, sv, cls[c], cls[c + 1], tn,
+      inverse_iteration_cluster<8>(n, d, e, sv, cls[c], cls[c + 1], tn,
                                 scratch + (long)blockIdx.x * 6 * n * n + c, (long)n * n, n, Zp,
                                 (unsigned long long)p);
   }
