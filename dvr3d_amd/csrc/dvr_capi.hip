@@ -892,6 +892,7 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
       f.panel_sub = env_int("DVR_LDL_SUB", 4);
       f.upd_xcd = env_int("DVR_LDL_XCD", 1) != 0;
       f.lookahead = env_int("DVR_LDL_LOOKAHEAD", 1) != 0;
+      f.sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
       f.ensure(n);
       const int nu = f.factor(H.as<double>(), sigma, s);
       if (inertia) *inertia = nu;

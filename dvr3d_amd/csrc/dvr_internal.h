@@ -473,6 +473,10 @@ struct GemmArgs {
   double alpha, beta;
   int ksplit, kchunk, lower, ntile;
   double* work;
+  // strided batch over blockIdx.z (without K split): product z uses
+  // A + z sa, B + z sb, C + z sc
+  long sa = 0, sb = 0, sc = 0;
+  int nbatch = 1;
 };
 // C = alpha op(A) op(B) + beta C (column-major).  ksplit > 1 splits K over
 // work-groups; work then holds gemm_work_doubles(M, N, ksplit) doubles.
@@ -481,6 +485,11 @@ void launch_gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N
                  double alpha, const double* A, long lda, const double* B, long ldb,
                  double beta, double* C, long ldc, int ksplit, double* work, bool lower);
 size_t gemm_work_doubles(int M, int N, int ksplit);
+// nbatch products of one shape at constant strides (no K split)
+void launch_gemm_batched(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                         double alpha, const double* A, long lda, long sa, const double* B,
+                         long ldb, long sb, double beta, double* C, long ldc, long sc,
+                         int nbatch);
 // Without K split, with the k-slice depth kt (16, 32), for beta != 0 C read
 // at the start of each tile (cp) so that its latency hides behind the k loop,
 // and for lower outputs optionally the XCD-aware tile order (xcd_lower:

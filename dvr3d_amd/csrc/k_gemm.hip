@@ -87,11 +87,18 @@ __device__ inline bool lower_tile_xcd(long L, int nt, int& ti, int& tj) {
 }
 
 template <bool TA, bool TB, int KT, bool CP>
-__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g0) {
   constexpr int NQ = KT / 4;   // elements of each operand per thread and slice
   __shared__ double As[KT * LDT];
   __shared__ double Bs[KT * LDT];
-  const int z = blockIdx.z;
+  GemmArgs g = g0;
+  int z = blockIdx.z;
+  if (g.nbatch > 1) {
+    g.A += z * g.sa;
+    g.B += z * g.sb;
+    g.C += z * g.sc;
+    z = 0;
+  }
   int ti = blockIdx.x, tj = blockIdx.y;
   // 1D grid over the lower tiles: neighbouring work-groups (dealt round-robin
   // to the XCDs) share their column panel, and each XCD keeps one row panel
@@ -333,6 +340,29 @@ static void launch_gemm_kt(hipStream_t s, const char* name, bool ta, bool tb, in
     DVR_LAUNCH("gemm_reduce", 8.0 * n * (ksplit + 1), 0.0, k_gemm_reduce, dim3(blocks), dim3(256),
                0, s, g);
   }
+  DVR_LAUNCH_CHECK();
+}
+
+void launch_gemm_batched(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
+                         double alpha, const double* A, long lda, long sa, const double* B,
+                         long ldb, long sb, double beta, double* C, long ldc, long sc,
+                         int nbatch) {
+  if (M <= 0 || N <= 0 || nbatch <= 0) return;
+  GemmArgs g;
+  g.M = M; g.N = N; g.K = K; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.alpha = alpha; g.beta = beta; g.lower = 0;
+  g.kchunk = K > 0 ? ((K + 31) / 32) * 32 : 32;
+  g.ksplit = 1; g.work = nullptr;
+  g.ntile = (M + TM - 1) / TM;
+  g.sa = sa; g.sb = sb; g.sc = sc; g.nbatch = nbatch;
+  const dim3 grid((M + TM - 1) / TM, (N + TM - 1) / TM, nbatch);
+  const double flops = 2.0 * M * (double)N * K * nbatch;
+  const double bytes = 8.0 * nbatch * ((double)M * K + (double)K * N +
+                                       (beta != 0.0 ? 2.0 : 1.0) * M * N);
+  if (ta && tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, true, 16, false>), grid, dim3(256), 0, s, g);
+  else if (ta) DVR_LAUNCH(name, bytes, flops, (k_gemm<true, false, 16, false>), grid, dim3(256), 0, s, g);
+  else if (tb) DVR_LAUNCH(name, bytes, flops, (k_gemm<false, true, 16, false>), grid, dim3(256), 0, s, g);
+  else DVR_LAUNCH(name, bytes, flops, (k_gemm<false, false, 16, false>), grid, dim3(256), 0, s, g);
   DVR_LAUNCH_CHECK();
 }
 

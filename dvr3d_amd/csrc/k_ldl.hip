@@ -271,6 +271,29 @@ k_identity(double* __restrict__ X, long ld, int n) {
   }
 }
 
+// nb_ identity blocks of n x n, each at X + b n^2 (ld n)
+__global__ void __launch_bounds__(256)
+k_identity_batched(double* __restrict__ X, int n, int nbk) {
+  const long nn = (long)n * n, tot = nn * nbk;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < tot; t += (long)gridDim.x * 256) {
+    const long loc = t % nn;
+    X[t] = (loc % n) == (loc / n) ? 1.0 : 0.0;
+  }
+}
+
+// batch y: dst + y sd (ld ldd) <- src + y ss (ld lds), m x ncol each
+__global__ void __launch_bounds__(256)
+k_copy2d_batched(const double* __restrict__ src, long lds, long ss, double* __restrict__ dst,
+                 long ldd, long sd, int m, int ncol) {
+  const long n = (long)m * ncol;
+  const double* sp = src + blockIdx.y * ss;
+  double* dp = dst + blockIdx.y * sd;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % m), c = (int)(t / m);
+    dp[r + (long)c * ldd] = sp[r + (long)c * lds];
+  }
+}
+
 static int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
 
 void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, double* Linv) {
@@ -372,10 +395,33 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
   }
   if (trail_pending) DVR_HIP(hipStreamWaitEvent(s, ev_trail, 0));
   double* W = Wbuf[0];
-  // inverse of every SB x SB diagonal super-block (unit lower), from the
-  // NB-level factors: X = L_sb^-1 by forward substitution on the identity
+  // inverse of every sb x sb diagonal super-block (unit lower), from the
+  // NB-level factors: X = L_sb^-1 by forward substitution on the identity,
+  // 64 rows at a time.  The full super-blocks go through each step together
+  // (strided-batched products), the last, shorter one on its own.
   double* lsb = linvsb.as<double>();
-  for (int b = 0; b < nsb; ++b) {
+  const int nfull = n / sb;
+  const long sb2 = (long)sb * sb;
+  if (nfull > 0) {
+    DVR_LAUNCH("ldl_ident", 8.0 * sb2 * nfull, 0.0, k_identity_batched,
+               dim3(grid_for(sb2 * nfull)), dim3(256), 0, s, lsb, sb, nfull);
+    for (int j0 = 0; j0 < sb; j0 += NB) {
+      const int ncol = j0 + NB;
+      // rows j0..j0+NB of X <- Linv64 (rows j0..j0+NB of X), per super-block
+      launch_gemm_batched(s, "ldl_inv", false, false, NB, ncol, NB, 1.0, l64 + (size_t)(j0 / NB) * NB * NB,
+                          NB, (long)(sb / NB) * NB * NB, lsb + j0, sb, sb2, 0.0, W, NB,
+                          (long)NB * sb, nfull);
+      DVR_LAUNCH("ldl_copy", 16.0 * NB * ncol * nfull, 0.0, k_copy2d_batched,
+                 dim3(grid_for((long)NB * ncol), nfull), dim3(256), 0, s, W, (long)NB,
+                 (long)NB * sb, lsb + j0, (long)sb, sb2, NB, ncol);
+      const int mr = sb - j0 - NB;
+      if (mr > 0)
+        launch_gemm_batched(s, "ldl_inv", false, false, mr, ncol, NB, -1.0,
+                            a + (j0 + NB) + (long)j0 * lda, lda, (long)sb * (1 + lda), lsb + j0,
+                            sb, sb2, 1.0, lsb + j0 + NB, sb, sb2, nfull);
+    }
+  }
+  for (int b = nfull; b < nsb; ++b) {
     const int r0 = b * sb, ns = std::min(sb, n - r0);
     double* X = lsb + (size_t)b * sb * sb;   // ns x ns, ld sb
     DVR_LAUNCH("ldl_ident", 8.0 * ns * ns, 0.0, k_identity, dim3(grid_for((long)ns * ns)),

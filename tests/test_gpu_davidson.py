@@ -71,6 +71,26 @@ def test_ldl_panel_widths_agree(monkeypatch):
     assert nu0 == nu1 == 601 and np.array_equal(tx0, tx1)
 
 
+@pytest.mark.parametrize("sb", ["512", "1024", "2048"])
+def test_ldl_super_block_sizes(monkeypatch, sb):
+    """Triangular solves over super-blocks of 512 / 1024 / 2048 rows (the
+    full super-blocks' inverses formed together as one strided batch, the
+    ragged last one alone) agree with the dense solve."""
+    from dvr3d_amd import _lib
+    monkeypatch.setenv("DVR_LDL_SB", sb)
+    n = 2203
+    rng = np.random.default_rng(11)
+    a = _spd_like(n, rng)
+    w = np.linalg.eigvalsh(a)
+    sigma = 0.5 * (w[700] + w[701])
+    r = rng.standard_normal((n, 4))
+    nu, t = _lib.ldl_solve(a, sigma, r)
+    assert nu == 701
+    ref = np.linalg.solve(a - sigma * np.eye(n), r)
+    cond = np.abs(w - sigma).max() / np.abs(w - sigma).min()
+    assert np.abs(t - ref).max() <= 1e-13 * cond * np.abs(ref).max() * n ** 0.5
+
+
 def test_ldl_inertia_on_c0_hamiltonian():
     """Eigenvalue counts of the c0 H3 at shifts between its levels."""
     from dvr3d_amd import _lib
