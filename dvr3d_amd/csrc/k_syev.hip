@@ -75,6 +75,7 @@ __device__ inline double block_sum_n(double v, double* red, int nthreads) {
 // short and every launch is latency.
 constexpr int TAIL = 128;
 __device__ __host__ inline int tail_start(int n) { return n > TAIL ? n - TAIL : 0; }
+bool tri_fused_enabled();
 
 // pbuf holds s = A v (unscaled); p = tau s.
 __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
@@ -252,6 +253,129 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   }
 }
 
+// One launch per column for problems of at most FMAX rows (the
+// Rayleigh-Ritz problems of the iterative solver), where every launch is
+// latency: each work-group of the column-j launch repeats the column phase
+// of k_tri_col itself -- K = tau s.v, w, the pending update of column j, its
+// norm and the reflector v_j, all on at most FMAX rows held in LDS -- and then
+// runs the trailing phase of k_tri_upd on its own UC columns.  Storage
+// differs from the two-launch form: s = A v alternates between pbuf (even j)
+// and wbuf (odd j), since the launch reads s_{j-1} while it writes s_j; w is
+// never stored; the reflector v_j goes to vbuf[j&1] and reaches the lower
+// part of column j one launch later (written by the next launch's first
+// work-group, when no work-group reads column j any more; the tail kernel
+// does it for the last one).
+constexpr int FMAX = 512;
+
+__global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
+  __shared__ double vp[FMAX], wp[FMAX], vj[FMAX];
+  __shared__ double red[TB / 64];
+  __shared__ double cj1;
+  const int bid = blockIdx.x;
+  int p = 0, base = 0;
+  for (; p < b.nprob; ++p) {
+    const int n = b.n[p];
+    const int cnt = (j < tail_start(n)) ? (n - j - 1 + UC - 1) / UC : 0;
+    if (bid < base + cnt) break;
+    base += cnt;
+  }
+  if (p >= b.nprob) return;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  double* A = b.A + b.aoff[p];
+  const int tid = threadIdx.x;
+  const bool first = bid == base;   // the problem's first work-group
+  const double* sprev = ((j - 1) & 1 ? b.wbuf : b.pbuf) + vo;
+  double* scur = (j & 1 ? b.wbuf : b.pbuf) + vo;
+  const double* vprev_g = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  double* vcur_g = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  // the previous reflector into the lower part of its column
+  if (first && j > 0)
+    for (int i = j + 1 + tid; i < n; i += TB) A[i + (long)(j - 1) * n] = vprev_g[i];
+  // pending update of step j-1: w = tau (s - (tau/2)(s.v) v)
+  const double taup = j > 0 ? b.tau[vo + j - 1] : 0.0;
+  double kp = 0.0;
+  for (int i = j + tid; i < n; i += TB) {
+    const double v = j > 0 ? vprev_g[i] : 0.0;
+    const double sv = j > 0 ? sprev[i] : 0.0;
+    vp[i] = v;
+    wp[i] = sv;
+    kp += sv * v;
+  }
+  const double K = taup * block_sum(kp, red);   // vp, wp (= s) visible
+  const double alpha = -0.5 * taup * K;
+  for (int i = j + tid; i < n; i += TB) wp[i] = taup * wp[i] + alpha * vp[i];
+  __syncthreads();
+  // column j with the pending update, and the norm below its subdiagonal
+  const double wj = wp[j], vpj = vp[j];
+  const double* col = A + (long)j * n;
+  double c[FMAX / TB];
+  double part = 0.0;
+#pragma unroll
+  for (int q = 0; q < FMAX / TB; ++q) {
+    const int i = j + tid + q * TB;
+    c[q] = 0.0;
+    if (i < n) {
+      c[q] = col[i] - (vp[i] * wj + wp[i] * vpj);
+      if (i == j + 1) cj1 = c[q];
+      if (i >= j + 2) part += c[q] * c[q];
+    }
+  }
+  const double sigma = block_sum(part, red);   // cj1 visible
+  const double alph = cj1;
+  double tj, beta, scal;
+  if (sigma == 0.0) {
+    tj = 0.0;
+    beta = alph;
+    scal = 0.0;
+  } else {
+    const double mu = sqrt(alph * alph + sigma);
+    beta = (alph <= 0.0) ? mu : -mu;
+    tj = (beta - alph) / beta;
+    scal = 1.0 / (alph - beta);
+  }
+#pragma unroll
+  for (int q = 0; q < FMAX / TB; ++q) {
+    const int i = j + tid + q * TB;
+    if (i > j && i < n) {
+      const double v = (i == j + 1) ? 1.0 : c[q] * scal;
+      vj[i] = v;
+      if (first) vcur_g[i] = v;
+    }
+    if (first && i == j) {
+      b.d[vo + j] = c[q];
+      b.e[vo + j] = beta;
+      b.tau[vo + j] = tj;
+    }
+  }
+  __syncthreads();
+  // trailing phase on this work-group's columns (two per wavefront)
+  const int lane = tid & 63, wave = tid >> 6;
+  const int k0 = j + 1 + (bid - base) * UC + 2 * wave;
+  if (k0 >= n) return;
+  const bool two = k0 + 1 < n;
+  double* ca = A + (long)k0 * n;
+  double* cb = two ? A + (long)(k0 + 1) * n : ca;
+  const double wa = wp[k0], pa = vp[k0];
+  const double wb = two ? wp[k0 + 1] : 0.0, pb = two ? vp[k0 + 1] : 0.0;
+  double sa = 0.0, sb = 0.0;
+  for (int i = j + 1 + lane; i < n; i += 64) {
+    const double vpi = vp[i], wi = wp[i], vji = vj[i];
+    const double ua = ca[i] - (vpi * wa + wi * pa);
+    const double ub = cb[i] - (vpi * wb + wi * pb);
+    ca[i] = ua;
+    if (two) cb[i] = ub;
+    sa += ua * vji;
+    sb += ub * vji;
+  }
+  sa = wave_sum(sa);
+  sb = wave_sum(sb);
+  if (lane == 0) {
+    scur[k0] = sa;
+    if (two) scur[k0 + 1] = sb;
+  }
+}
+
 // Columns tail_start(n) .. n-2 of problem blockIdx.x: the trailing block
 // A(j0:n, j0:n) (both triangles) is copied to LDS together with the state the
 // column launches leave behind -- the reflector v_{j0-1}, s_{j0-1} = A v_{j0-1}
@@ -262,7 +386,7 @@ constexpr int TT = 1024;
 constexpr int LDB = TAIL + 1;
 constexpr size_t TAIL_LDS = sizeof(double) * ((size_t)TAIL * LDB + 4 * TAIL);
 
-__global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b) {
+__global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b, int fused) {
   extern __shared__ double sh[];
   double* B = sh;
   double* vp = B + TAIL * LDB;   // v_{j-1}
@@ -286,10 +410,15 @@ __global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b) {
   double taup = 0.0;
   if (j0 > 0) {
     const double* vprev = ((j0 - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
+    // s_{j0-1}: pbuf, or (one-launch columns) the buffer of that parity
+    const double* sprev = (fused && ((j0 - 1) & 1) ? b.wbuf : b.pbuf) + vo;
     for (int i = tid; i < t; i += TT) {
       vp[i] = vprev[j0 + i];
-      ss[i] = b.pbuf[vo + j0 + i];
+      ss[i] = sprev[j0 + i];
     }
+    // the one-launch columns leave the last reflector to be stored
+    if (fused)
+      for (int i = j0 + 1 + tid; i < n; i += TT) A[i + (long)(j0 - 1) * n] = vprev[i];
     taup = b.tau[vo + j0 - 1];
   }
   __syncthreads();
@@ -388,7 +517,20 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
     nmax = std::max(nmax, h_n[p]);
   }
   if (nmax <= 0) return;
-  for (int j = 0; j < jend; ++j) {
+  const bool fused = nmax <= FMAX && tri_fused_enabled();
+  for (int j = 0; j < jend && fused; ++j) {
+    double bytes = 0.0, flops = 0.0;
+    int grid = 0;
+    for (int p = 0; p < b.nprob; ++p) {
+      if (j >= tail_start(h_n[p])) continue;
+      const double t = (double)(h_n[p] - j - 1);
+      grid += (h_n[p] - j - 1 + UC - 1) / UC;
+      bytes += (j > 0 ? 16.0 : 8.0) * t * t + 8.0 * 6.0 * t;
+      flops += (j > 0 ? 6.0 : 2.0) * t * t;
+    }
+    DVR_LAUNCH("tri_fused", bytes, flops, k_tri_fused, dim3(grid), dim3(TB), 0, s, b, j);
+  }
+  for (int j = 0; j < jend && !fused; ++j) {
     double cbytes = 0.0, ubytes = 0.0, uflops = 0.0;
     int grid = 0;
     for (int p = 0; p < b.nprob; ++p) {
@@ -413,7 +555,8 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
     const double t = (double)(h_n[p] - tail_start(h_n[p]));
     tbytes += 8.0 * 2.0 * t * t;
   }
-  DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail, dim3(b.nprob), dim3(TT), TAIL_LDS, s, b);
+  DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail, dim3(b.nprob), dim3(TT), TAIL_LDS, s, b,
+             fused ? 1 : 0);
   DVR_LAUNCH_CHECK();
 }
 
@@ -593,6 +736,12 @@ k_stein(EigBatch b, const Cluster* __restrict__ cl, int ncl,
 // touches TW consecutive doubles.  When all of a work-group's clusters belong
 // to one problem its tridiagonal is staged in LDS as well.
 constexpr int LDS_BYTES_MAX = 160 * 1024;
+
+// DVR_TRI=twolaunch keeps two launches per column for small problems too
+bool tri_fused_enabled() {
+  const char* v = getenv("DVR_TRI");
+  return !(v && std::string(v) == "twolaunch");
+}
 
 // DVR_BT=global keeps the back-transformation on the global-memory kernels
 bool bt_use_lds() {

@@ -73,6 +73,22 @@ def test_syev_batch_random(sizes):
         _check(a, w, z, a.shape[0])
 
 
+@pytest.mark.parametrize("tri", ["fused", "twolaunch"])
+def test_syev_small_problems_column_launches(monkeypatch, tri):
+    """Problems of at most 512 rows run one launch per column (every
+    work-group repeats the column phase); DVR_TRI=twolaunch keeps the
+    column and trailing launches apart.  Both agree with LAPACK, in a batch
+    that mixes sizes below the LDS tail (100), across it (129, 300) and at
+    the one-launch limit (512)."""
+    from dvr3d_amd import _lib
+    monkeypatch.setenv("DVR_TRI", tri)
+    rng = np.random.default_rng(448)
+    mats = [_sym(nn, rng) for nn in (448, 100, 129, 300, 512)]
+    ws, zs = _lib.syev_batch(mats)
+    for a, w, z in zip(mats, ws, zs):
+        _check(a, w, z, a.shape[0])
+
+
 def test_syev_batch_partial_and_clusters():
     from dvr3d_amd import _lib
     rng = np.random.default_rng(7)
