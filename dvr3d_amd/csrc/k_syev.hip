@@ -289,18 +289,47 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
   double* scur = (j & 1 ? b.wbuf : b.pbuf) + vo;
   const double* vprev_g = ((j - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
   double* vcur_g = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  const int lane = tid & 63, wave = tid >> 6;
+  const double* col = A + (long)j * n;
+  // every global read of the launch is issued here, so that they share one
+  // memory round trip: v_{j-1}, s_{j-1} and column j (rows j.., two per
+  // thread) and this wavefront's two trailing columns (rows j+1.., eight per
+  // lane)
+  constexpr int RQ = FMAX / TB, RL = FMAX / 64;
+  double vr[RQ], sr[RQ], cr[RQ];
+#pragma unroll
+  for (int q = 0; q < RQ; ++q) {
+    const int i = j + tid + q * TB;
+    const bool in = i < n;
+    vr[q] = (in && j > 0) ? vprev_g[i] : 0.0;
+    sr[q] = (in && j > 0) ? sprev[i] : 0.0;
+    cr[q] = in ? col[i] : 0.0;
+  }
+  const int k0 = j + 1 + (bid - base) * UC + 2 * wave;
+  const bool have = k0 < n, two = k0 + 1 < n;
+  double* ca = A + (long)(have ? k0 : 0) * n;
+  double* cb = two ? A + (long)(k0 + 1) * n : ca;
+  double xa[RL], xb[RL];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const int i = j + 1 + lane + 64 * r;
+    xa[r] = (have && i < n) ? ca[i] : 0.0;
+    xb[r] = (two && i < n) ? cb[i] : 0.0;
+  }
+  const double taup = j > 0 ? b.tau[vo + j - 1] : 0.0;
   // the previous reflector into the lower part of its column
   if (first && j > 0)
     for (int i = j + 1 + tid; i < n; i += TB) A[i + (long)(j - 1) * n] = vprev_g[i];
   // pending update of step j-1: w = tau (s - (tau/2)(s.v) v)
-  const double taup = j > 0 ? b.tau[vo + j - 1] : 0.0;
   double kp = 0.0;
-  for (int i = j + tid; i < n; i += TB) {
-    const double v = j > 0 ? vprev_g[i] : 0.0;
-    const double sv = j > 0 ? sprev[i] : 0.0;
-    vp[i] = v;
-    wp[i] = sv;
-    kp += sv * v;
+#pragma unroll
+  for (int q = 0; q < RQ; ++q) {
+    const int i = j + tid + q * TB;
+    if (i < n) {
+      vp[i] = vr[q];
+      wp[i] = sr[q];
+      kp += sr[q] * vr[q];
+    }
   }
   const double K = taup * block_sum(kp, red);   // vp, wp (= s) visible
   const double alpha = -0.5 * taup * K;
@@ -308,15 +337,14 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
   __syncthreads();
   // column j with the pending update, and the norm below its subdiagonal
   const double wj = wp[j], vpj = vp[j];
-  const double* col = A + (long)j * n;
-  double c[FMAX / TB];
+  double c[RQ];
   double part = 0.0;
 #pragma unroll
-  for (int q = 0; q < FMAX / TB; ++q) {
+  for (int q = 0; q < RQ; ++q) {
     const int i = j + tid + q * TB;
     c[q] = 0.0;
     if (i < n) {
-      c[q] = col[i] - (vp[i] * wj + wp[i] * vpj);
+      c[q] = cr[q] - (vp[i] * wj + wp[i] * vpj);
       if (i == j + 1) cj1 = c[q];
       if (i >= j + 2) part += c[q] * c[q];
     }
@@ -350,23 +378,22 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
   }
   __syncthreads();
   // trailing phase on this work-group's columns (two per wavefront)
-  const int lane = tid & 63, wave = tid >> 6;
-  const int k0 = j + 1 + (bid - base) * UC + 2 * wave;
-  if (k0 >= n) return;
-  const bool two = k0 + 1 < n;
-  double* ca = A + (long)k0 * n;
-  double* cb = two ? A + (long)(k0 + 1) * n : ca;
+  if (!have) return;
   const double wa = wp[k0], pa = vp[k0];
   const double wb = two ? wp[k0 + 1] : 0.0, pb = two ? vp[k0 + 1] : 0.0;
   double sa = 0.0, sb = 0.0;
-  for (int i = j + 1 + lane; i < n; i += 64) {
-    const double vpi = vp[i], wi = wp[i], vji = vj[i];
-    const double ua = ca[i] - (vpi * wa + wi * pa);
-    const double ub = cb[i] - (vpi * wb + wi * pb);
-    ca[i] = ua;
-    if (two) cb[i] = ub;
-    sa += ua * vji;
-    sb += ub * vji;
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const int i = j + 1 + lane + 64 * r;
+    if (i < n) {
+      const double vpi = vp[i], wi = wp[i], vji = vj[i];
+      const double ua = xa[r] - (vpi * wa + wi * pa);
+      const double ub = xb[r] - (vpi * wb + wi * pb);
+      ca[i] = ua;
+      if (two) cb[i] = ub;
+      sa += ua * vji;
+      sb += ub * vji;
+    }
   }
   sa = wave_sum(sa);
   sb = wave_sum(sb);
