@@ -268,9 +268,11 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
 constexpr int FMAX = 512;
 
 __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
-  __shared__ double vp[FMAX], wp[FMAX], vj[FMAX];
+  // v_{j-1}, s_{j-1} and the updated column j; w_{j-1} and v_j are formed
+  // from them where they are needed, so that the two reductions are the
+  // launch's only barriers
+  __shared__ double vp[FMAX], sp[FMAX], cl[FMAX];
   __shared__ double red[TB / 64];
-  __shared__ double cj1;
   const int bid = blockIdx.x;
   int p = 0, base = 0;
   for (; p < b.nprob; ++p) {
@@ -327,16 +329,15 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
     const int i = j + tid + q * TB;
     if (i < n) {
       vp[i] = vr[q];
-      wp[i] = sr[q];
+      sp[i] = sr[q];
       kp += sr[q] * vr[q];
     }
   }
-  const double K = taup * block_sum(kp, red);   // vp, wp (= s) visible
+  const double K = taup * block_sum(kp, red);   // vp, sp visible
   const double alpha = -0.5 * taup * K;
-  for (int i = j + tid; i < n; i += TB) wp[i] = taup * wp[i] + alpha * vp[i];
-  __syncthreads();
+  auto wof = [&](int i) { return taup * sp[i] + alpha * vp[i]; };   // w_{j-1}(i)
   // column j with the pending update, and the norm below its subdiagonal
-  const double wj = wp[j], vpj = vp[j];
+  const double wj = wof(j), vpj = vp[j];
   double c[RQ];
   double part = 0.0;
 #pragma unroll
@@ -344,13 +345,13 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
     const int i = j + tid + q * TB;
     c[q] = 0.0;
     if (i < n) {
-      c[q] = cr[q] - (vp[i] * wj + wp[i] * vpj);
-      if (i == j + 1) cj1 = c[q];
+      c[q] = cr[q] - (vp[i] * wj + wof(i) * vpj);
+      cl[i] = c[q];
       if (i >= j + 2) part += c[q] * c[q];
     }
   }
-  const double sigma = block_sum(part, red);   // cj1 visible
-  const double alph = cj1;
+  const double sigma = block_sum(part, red);   // cl visible
+  const double alph = cl[j + 1];
   double tj, beta, scal;
   if (sigma == 0.0) {
     tj = 0.0;
@@ -362,31 +363,29 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
     tj = (beta - alph) / beta;
     scal = 1.0 / (alph - beta);
   }
+  if (first) {
 #pragma unroll
-  for (int q = 0; q < FMAX / TB; ++q) {
-    const int i = j + tid + q * TB;
-    if (i > j && i < n) {
-      const double v = (i == j + 1) ? 1.0 : c[q] * scal;
-      vj[i] = v;
-      if (first) vcur_g[i] = v;
-    }
-    if (first && i == j) {
-      b.d[vo + j] = c[q];
-      b.e[vo + j] = beta;
-      b.tau[vo + j] = tj;
+    for (int q = 0; q < RQ; ++q) {
+      const int i = j + tid + q * TB;
+      if (i > j && i < n) vcur_g[i] = (i == j + 1) ? 1.0 : c[q] * scal;
+      if (i == j) {
+        b.d[vo + j] = c[q];
+        b.e[vo + j] = beta;
+        b.tau[vo + j] = tj;
+      }
     }
   }
-  __syncthreads();
   // trailing phase on this work-group's columns (two per wavefront)
   if (!have) return;
-  const double wa = wp[k0], pa = vp[k0];
-  const double wb = two ? wp[k0 + 1] : 0.0, pb = two ? vp[k0 + 1] : 0.0;
+  const double wa = wof(k0), pa = vp[k0];
+  const double wb = two ? wof(k0 + 1) : 0.0, pb = two ? vp[k0 + 1] : 0.0;
   double sa = 0.0, sb = 0.0;
 #pragma unroll
   for (int r = 0; r < RL; ++r) {
     const int i = j + 1 + lane + 64 * r;
     if (i < n) {
-      const double vpi = vp[i], wi = wp[i], vji = vj[i];
+      const double vpi = vp[i], wi = wof(i);
+      const double vji = (i == j + 1) ? 1.0 : cl[i] * scal;
       const double ua = xa[r] - (vpi * wa + wi * pa);
       const double ub = xb[r] - (vpi * wb + wi * pb);
       ca[i] = ua;
