@@ -21,6 +21,20 @@ def c0_setup():
     return job, pes, inp, ref
 
 
+def test_c0_3d_assembly_paths_agree(c0_setup, monkeypatch):
+    """The 3D assembly on the lower-tile GEMM + scale-and-mirror pass and on
+    the grouped SYRK kernel (DVR_SYRK3_GEMM=0): the same H3 up to summation
+    order, so the same levels, both within 1e-4 cm-1 of the reference."""
+    job, pes, inp, ref = c0_setup
+    gref = np.array(H.golden("c0")["eval_cm"])
+    out = {}
+    for v in ("1", "0"):
+        monkeypatch.setenv("DVR_SYRK3_GEMM", v)
+        out[v] = H.run_device(inp).eigenvalues() * O.AUTOCM
+        assert np.abs(out[v] - gref).max() < 1e-4
+    assert np.abs(out["1"] - out["0"]).max() < 1e-6
+
+
 @pytest.mark.parametrize("eig1d", ["householder", "jacobi"])
 def test_c0_pipeline_matches_reference(c0_setup, eig1d, monkeypatch):
     """Both 1D solvers (LDS Householder and LDS Jacobi sweeps)."""
