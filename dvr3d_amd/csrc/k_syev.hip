@@ -408,10 +408,18 @@ __global__ void __launch_bounds__(TB) k_tri_fused(EigBatch b, int j) {
 // and tau_{j0-1}, the rank-2 update of step j0-1 still pending -- and the
 // column and trailing phases of every remaining step run on it exactly as in
 // k_tri_col / k_tri_upd.  The lower triangle (reflectors) is written back.
-constexpr int TT = 1024;
+// work-group size of the tail kernel: DVR_TAIL_THREADS (256, 512, 1024).
+// On the 448-row Rayleigh-Ritz problems of c3 one tail took 0.55 / 0.42 /
+// 0.48 ms; 512 is the default.
+int tail_threads() {
+  const char* v = getenv("DVR_TAIL_THREADS");
+  const int t = v ? atoi(v) : 512;
+  return t == 256 || t == 1024 ? t : 512;
+}
 constexpr int LDB = TAIL + 1;
 constexpr size_t TAIL_LDS = sizeof(double) * ((size_t)TAIL * LDB + 4 * TAIL);
 
+template <int TT>
 __global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b, int fused) {
   extern __shared__ double sh[];
   double* B = sh;
@@ -574,15 +582,24 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
       DVR_LAUNCH("tri_upd", ubytes, uflops, k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
     }
   }
-  DVR_HIP(hipFuncSetAttribute((const void*)k_tri_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)TAIL_LDS));
+  for (const void* f : {(const void*)k_tri_tail<256>, (const void*)k_tri_tail<512>,
+                        (const void*)k_tri_tail<1024>})
+    DVR_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)TAIL_LDS));
   double tbytes = 0.0;
   for (int p = 0; p < b.nprob; ++p) {
     const double t = (double)(h_n[p] - tail_start(h_n[p]));
     tbytes += 8.0 * 2.0 * t * t;
   }
-  DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail, dim3(b.nprob), dim3(TT), TAIL_LDS, s, b,
-             fused ? 1 : 0);
+  const int tt = tail_threads();
+  if (tt == 256)
+    DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail<256>, dim3(b.nprob), dim3(256), TAIL_LDS, s, b,
+               fused ? 1 : 0);
+  else if (tt == 512)
+    DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail<512>, dim3(b.nprob), dim3(512), TAIL_LDS, s, b,
+               fused ? 1 : 0);
+  else
+    DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail<1024>, dim3(b.nprob), dim3(1024), TAIL_LDS, s,
+               b, fused ? 1 : 0);
   DVR_LAUNCH_CHECK();
 }
 
