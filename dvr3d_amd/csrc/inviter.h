@@ -17,15 +17,6 @@
 
 namespace dvr {
 
-// 1/x from the hardware reciprocal estimate and two Newton steps: within a
-// few ulps for normal x, and about half the dependent instructions of the
-// correctly rounded division on the recurrences below
-__device__ inline double rcp_refined(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(r, fma(-x, r, 1.0), r);
-  return fma(r, fma(-x, r, 1.0), r);
-}
-
 __device__ inline double ii_hash(unsigned long long x) {
   x += 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;

@@ -52,14 +52,6 @@ __device__ inline double readlane_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// 1/x from the hardware estimate and two Newton steps
-__device__ inline double recip_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
 
 // FULL: nb == NB, every panel 16 wide (all guards resolved at compile time)
 template <bool FULL>
@@ -97,7 +89,7 @@ k_ldl_diag(double* __restrict__ A, long lda, int k0, int nb_, double* __restrict
         if (j < bw) {
           const double dj = readlane_d(col[j], c0 + j);
           const double w = col[j];
-          const double l = w * recip_nr(dj);
+          const double l = w * rcp_refined(dj);
 #pragma unroll
           for (int c = j + 1; c < BS; ++c)
             if (c < bw) col[c] -= w * readlane_d(l, c0 + c);
