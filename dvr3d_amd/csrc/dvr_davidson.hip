@@ -30,15 +30,15 @@ namespace dvr {
 
 void DavidsonSolver::gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K,
                           double alpha, const double* A, long lda, const double* B, long ldb,
-                          double beta, double* C, long ldc) {
-  tuner.run(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, false);
+                          double beta, double* C, long ldc, bool lower) {
+  tuner.run(s, name, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lower);
 }
 
-// eigen-decomposition of the symmetric m x m matrix at S (destroyed):
-// ascending values to host w, vectors to U (m x m, ld m)
+// eigen-decomposition of the symmetric m x m matrix whose lower triangle is
+// at S (destroyed): ascending values to host w, vectors to U (m x m, ld m)
 void DavidsonSolver::small_eig(int m, double* S, double* U, std::vector<double>& w,
                                hipStream_t s) {
-  launch_symmetrize(s, S, m);
+  launch_mirror_lower(s, S, m);
   // only (near-)degenerate Ritz values share an inverse-iteration thread; the
   // small loss of orthogonality between the other vectors (eps ||T|| / gap)
   // is removed by two Newton-Schulz steps U <- U (3 I - U^T U) / 2, which mix
@@ -231,7 +231,8 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
   std::vector<double> th, nrm(mmax);
   auto rayleigh_ritz = [&]() {
     double* T = Tm.as<double>();
-    gemm(s, "proj", true, false, m, m, N, 1.0, V.as<double>(), N, AV.as<double>(), N, 0.0, T, m);
+    gemm(s, "proj", true, false, m, m, N, 1.0, V.as<double>(), N, AV.as<double>(), N, 0.0, T, m,
+         true);
     small_eig(m, T, Y.as<double>(), th, s);
     gemm(s, "ritz", false, false, N, m, m, 1.0, V.as<double>(), N, Y.as<double>(), m, 0.0,
          X.as<double>(), N);
@@ -258,7 +259,8 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
   if (si) {
     std::vector<double> w0;
     double* T = Tm.as<double>();
-    gemm(s, "proj", true, false, m, m, N, 1.0, V.as<double>(), N, AV.as<double>(), N, 0.0, T, m);
+    gemm(s, "proj", true, false, m, m, N, 1.0, V.as<double>(), N, AV.as<double>(), N, 0.0, T, m,
+         true);
     small_eig(m, T, nullptr, w0, s);
     sigma = w0[0];
     clock.mark(PH_FACTOR, s);

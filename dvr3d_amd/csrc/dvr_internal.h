@@ -583,6 +583,8 @@ void launch_resid_gather(hipStream_t s, const double* X, const double* AX, long 
                          const double* theta, const int* idx, int ncol, double* R, long ldr,
                          int nrows);
 void launch_symmetrize(hipStream_t s, double* T, int m);
+// T(c, r) <- T(r, c) for r > c: the upper triangle from the lower one
+void launch_mirror_lower(hipStream_t s, double* T, int m);
 void launch_gram_prep(hipStream_t s, double* G, int q, double eps, double* sc);
 void launch_form_m(hipStream_t s, const double* Linv, int q, const double* r, const double* sc,
                    double* M);
@@ -644,7 +646,8 @@ class DavidsonSolver {
 
  private:
   void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,
-            const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc);
+            const double* A, long lda, const double* B, long ldb, double beta, double* C, long ldc,
+            bool lower = false);
   void small_eig(int m, double* S, double* U, std::vector<double>& w, hipStream_t s);
   int expand(const double* H, int q0, hipStream_t s);
   int expand_chunk(const double* H, double* W, int q0, hipStream_t s);

@@ -126,6 +126,21 @@ void launch_resid_gather(hipStream_t s, const double* X, const double* AX, long 
              ldr, nrows);
 }
 
+// T(r, c) -> T(c, r) for r > c (m x m, ld m)
+__global__ void __launch_bounds__(256) k_mirror_lower(double* __restrict__ T, int m) {
+  const long n = (long)m * m;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int r = (int)(t % m), c = (int)(t / m);
+    if (r > c) T[c + (long)r * m] = T[r + (long)c * m];
+  }
+}
+
+void launch_mirror_lower(hipStream_t s, double* T, int m) {
+  if (m <= 1) return;
+  DVR_LAUNCH("mirror", 8.0 * m * m, 0.0, k_mirror_lower, dim3(grid_for((long)m * m)), dim3(256),
+             0, s, T, m);
+}
+
 void launch_symmetrize(hipStream_t s, double* T, int m) {
   if (m <= 1) return;
   DVR_LAUNCH("symm", 16.0 * m * m, 0.0, k_symmetrize, dim3(grid_for((long)m * m)), dim3(256), 0,

@@ -234,13 +234,16 @@ k_ldl_scale_rows(double* __restrict__ Y, long ldy, int m, int nrhs, const double
   }
 }
 
-// A (A + s I) with the diagonal shifted, full square copy from H
+// A <- H - sigma I, lower triangle only: the factorisation, its inverse
+// blocks and the solves read nothing above the diagonal (the GEMMs that
+// update the diagonal blocks also touch entries above it, which are never
+// read back)
 __global__ void __launch_bounds__(256)
 k_shift_copy(const double* __restrict__ H, double* __restrict__ A, long n, double sigma) {
   const long nn = n * n;
   for (long t = blockIdx.x * 256L + threadIdx.x; t < nn; t += (long)gridDim.x * 256) {
     const long r = t % n, c = t / n;
-    A[t] = (r == c) ? H[t] - sigma : H[t];
+    if (r >= c) A[t] = (r == c) ? H[t] - sigma : H[t];
   }
 }
 
@@ -307,7 +310,7 @@ void LdlFactor::ensure(int n_) {
 int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
   double* a = A.as<double>();
   const long lda = n;
-  DVR_LAUNCH("ldl_shift", 16.0 * n * (double)n, 0.0, k_shift_copy, dim3(grid_for((long)n * n)),
+  DVR_LAUNCH("ldl_shift", 8.0 * n * (double)(n + 1), 0.0, k_shift_copy, dim3(grid_for((long)n * n)),
              dim3(256), 0, s, H, a, (long)n, sigma);
   double* dd = d.as<double>();
   double* l64 = linv64.as<double>();
