@@ -413,6 +413,8 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.ldl_sub = env_int("DVR_LDL_SUB", 4);
     C->dav.ldl_xcd = env_int("DVR_LDL_XCD", 1) != 0;
     C->dav.ldl_sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
+    C->dav.ldl_cp = env_int("DVR_LDL_CP", 1) != 0;
+    if (const char* t = getenv("DVR_DAV_EARLY")) C->dav.early_tol = atof(t);
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.keep = env_int("DVR_DAV_KEEP", P.keep);
@@ -893,6 +895,7 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
       f.upd_xcd = env_int("DVR_LDL_XCD", 1) != 0;
       f.lookahead = env_int("DVR_LDL_LOOKAHEAD", 1) != 0;
       f.sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
+      f.upd_cp = env_int("DVR_LDL_CP", 1) != 0;
       f.ensure(n);
       const int nu = f.factor(H.as<double>(), sigma, s);
       if (inertia) *inertia = nu;

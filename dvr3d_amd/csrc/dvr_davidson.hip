@@ -164,9 +164,19 @@ void PhaseClock::collect(double* out) {
 
 enum { PH_RR = 0, PH_SI = 1, PH_EXPAND = 2, PH_FACTOR = 3, PH_START = 4 };
 
+DavidsonSolver::~DavidsonSolver() {
+  if (s2) {
+    (void)hipStreamSynchronize(s2);
+    (void)hipStreamDestroy(s2);
+  }
+  if (ev_s) (void)hipEventDestroy(ev_s);
+}
+
 int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_t s) {
   clock.used = 0;
   const int rc = solve_impl(H, n, P, s);
+  // an early factorisation still running when the solve ended (stall)
+  if (s2) DVR_HIP(hipStreamSynchronize(s2));
   clock.mark(-1, s);
   DVR_HIP(hipStreamSynchronize(s));
   clock.collect(stats.phase_ms);
@@ -199,12 +209,28 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
   C1.ensure(sizeof(double) * (size_t)std::max(nlmax, mmax) * b);
   thd.ensure(sizeof(double) * mmax);
   nrmd.ensure(sizeof(double) * mmax);
-  ldl.sb = ldl_sb;
-  ldl.ensure(N);
-  ldl.use_graph = use_graph;
-  ldl.panel_sub = ldl_sub;
-  ldl.upd_xcd = ldl_xcd;
-  ldl.tuner = &tuner;
+  auto setup = [&](LdlFactor& f) {
+    f.sb = ldl_sb;
+    f.ensure(N);
+    f.use_graph = use_graph;
+    f.panel_sub = ldl_sub;
+    f.upd_xcd = ldl_xcd;
+    f.upd_cp = ldl_cp;
+    f.tuner = &tuner;
+  };
+  setup(ldl);
+  // with early_tol > 0 the factorisation of the next shift runs on s2 into
+  // the second factor while the current slice converges on s
+  const bool early_on = early_tol > 0.0;
+  if (early_on) {
+    setup(ldl2);
+    if (!s2) {
+      DVR_HIP(hipStreamCreate(&s2));
+      DVR_HIP(hipEventCreateWithFlags(&ev_s, hipEventDisableTiming));
+    }
+  }
+  LdlFactor* cur = &ldl;
+  LdlFactor* nxt = &ldl2;
   thL.clear();
   nlocked = 0;
 
@@ -264,7 +290,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
     small_eig(m, T, nullptr, w0, s);
     sigma = w0[0];
     clock.mark(PH_FACTOR, s);
-    ldl.factor(H, sigma, s);
+    cur->factor(H, sigma, s);
     stats.factorizations++;
   }
 
@@ -272,6 +298,17 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
     int locked_slice = 0;
     double th_next = 0.0;
     bool have_next = false;
+    // early mode: the next shift sig_e is fixed and factored on s2; the slice
+    // then takes every level below it and ends when their number equals the
+    // inertia nu_e
+    bool early = false, launch_e = false;
+    double sig_e = 0.0;
+    int nu_e = -1;
+    auto found_below = [&](double sg) {
+      int f = 0;
+      for (double t : thL) f += t < sg;
+      return f;
+    };
     stats.slices++;
     for (;;) {
       if (stats.iterations >= P.max_iter) return 2;   // stalled
@@ -279,7 +316,10 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       clock.mark(PH_RR, s);
       rayleigh_ritz();
       int nl = 0;
-      while (nl < m && nrm[nl] <= tol && locked_slice + nl < K && nlocked + nl < nlmax) ++nl;
+      if (early)
+        while (nl < m && nrm[nl] <= tol && th[nl] < sig_e && nlocked + nl < nlmax) ++nl;
+      else
+        while (nl < m && nrm[nl] <= tol && locked_slice + nl < K && nlocked + nl < nlmax) ++nl;
       if (nl > 0) {
         DVR_HIP(hipMemcpyAsync(XL.as<double>() + (long)nlocked * N, X.p,
                                sizeof(double) * (size_t)N * nl, hipMemcpyDeviceToDevice, s));
@@ -287,7 +327,33 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
         nlocked += nl;
         locked_slice += nl;
       }
-      if (locked_slice >= K || nlocked >= nlmax) {
+      bool end_slice = false;
+      if (early) {
+        // no Ritz value left below the next shift: compare with its inertia
+        if (!(nl < m && th[nl] < sig_e) || nlocked >= nlmax) {
+          if (nu_e < 0) {
+            clock.mark(PH_FACTOR, s);
+            nu_e = nxt->factor_end(s2);
+          }
+          // fewer levels than the inertia: one has not appeared yet, iterate on
+          end_slice = found_below(sig_e) >= nu_e || nlocked >= nlmax;
+        }
+      } else if (locked_slice >= K || nlocked >= nlmax) {
+        end_slice = true;
+      } else if (early_on) {
+        // the Ritz pairs up to one past the top of the slice near convergence:
+        // the next shift goes half-way between the top level and the next
+        const int r = std::min(K - locked_slice, nlmax - nlocked);
+        if (nl + r < m) {
+          bool ok = true;
+          for (int j = nl; j <= nl + r; ++j) ok = ok && nrm[j] <= early_tol;
+          if (ok) {
+            sig_e = 0.5 * (th[nl + r - 1] + th[nl + r]);
+            early = launch_e = true;
+          }
+        }
+      }
+      if (end_slice) {
         const int rest = m - nl;
         if (rest > 0) {
           DVR_HIP(hipMemcpyAsync(V.p, X.as<double>() + (long)nl * N, sizeof(double) * (size_t)N * rest,
@@ -312,7 +378,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       launch_resid_gather(s, X.as<double>(), AX.as<double>(), N, thd.as<double>(),
                           upload(idxd, act, s), na, Rb.as<double>(), N, N);
       if (si) {
-        ldl.solve_graph(Rb.as<double>(), Tb.as<double>(), na, s);
+        cur->solve_graph(Rb.as<double>(), Tb.as<double>(), na, s);
       } else {
         std::vector<double> ta(na);
         for (int c = 0; c < na; ++c) ta[c] = th[act[c]];
@@ -338,17 +404,41 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       const int q = expand(H, na, s);
       if (q == 0) return 2;
       m += q;
+      if (launch_e) {
+        // the factorisation of the next shift, behind this iteration's work
+        // on s (which also ends every earlier use of the second factor)
+        DVR_HIP(hipEventRecord(ev_s, s));
+        DVR_HIP(hipStreamWaitEvent(s2, ev_s, 0));
+        nxt->factor_begin(H, sig_e, s2);
+        stats.factorizations++;
+        launch_e = false;
+      }
     }
     // certify the levels found so far
-    const double top = *std::max_element(thL.begin(), thL.end());
-    const double sig_hi = have_next ? 0.5 * (top + th_next)
-                                    : top + 1e-6 * (std::fabs(top) + 1.0);
-    clock.mark(PH_FACTOR, s);
-    const int nu = ldl.factor(H, sig_hi, s);
-    stats.factorizations++;
-    sigma = sig_hi;
-    int found = 0;
-    for (double t : thL) found += t < sig_hi;
+    int nu = 0, found = 0;
+    if (early) {
+      found = found_below(sig_e);
+      if (found == nu_e) {
+        nu = nu_e;
+        sigma = sig_e;
+        std::swap(cur, nxt);
+      } else {
+        // the space of locked levels filled up before every level below
+        // sig_e was found (a shift placed from rough Ritz values): certify
+        // at the top of what was found instead, as without the early shift
+        early = false;
+      }
+    }
+    if (!early) {
+      const double top = *std::max_element(thL.begin(), thL.end());
+      const double sig_hi = have_next ? 0.5 * (top + th_next)
+                                      : top + 1e-6 * (std::fabs(top) + 1.0);
+      clock.mark(PH_FACTOR, s);
+      nu = cur->factor(H, sig_hi, s);
+      stats.factorizations++;
+      sigma = sig_hi;
+      found = found_below(sig_hi);
+    }
     stats.certified = nu;
     if (found != nu) {
       stats.missing = nu - found;

@@ -547,6 +547,10 @@ struct LdlFactor {
   // factors H - sigma I (H: n x n full square, ld n); returns the inertia
   // count (number of negative pivots = eigenvalues of H below sigma)
   int factor(const double* H, double sigma, hipStream_t s);
+  // the same in two halves: the launches (the host does not wait), then the
+  // wait for s and the inertia count
+  void factor_begin(const double* H, double sigma, hipStream_t s);
+  int factor_end(hipStream_t s);
   // T <- (H - sigma I)^-1 R for nrhs columns; R is destroyed
   void solve(double* R, double* T, int nrhs, hipStream_t s);
   // the same through a captured HIP graph per nrhs (the sequence of ~160
@@ -643,6 +647,12 @@ class DavidsonSolver {
   int ldl_sub = 4;             // LDL^T panel width in 64-column sub-panels
   bool ldl_xcd = true;         // XCD-aware tile order of the trailing updates
   int ldl_sb = 2048;           // super-block of the shift-and-invert triangular solves
+  bool ldl_cp = true;          // trailing updates read C before the k loop
+  // residual norm below which the Ritz pairs up to one past the top of a
+  // slice fix the next shift: its factorisation then starts on a second
+  // stream while the slice converges (0: after the slice, on the same stream)
+  double early_tol = 1e-4;
+  ~DavidsonSolver();
 
  private:
   void gemm(hipStream_t s, const char* name, bool ta, bool tb, int M, int N, int K, double alpha,
@@ -656,7 +666,9 @@ class DavidsonSolver {
   DevBuf V, AV, X, AX, XL, Rb, Tb, Tm, Y, Y2, G, G2, U, C1, thd, nrmd, idxd, scal, work;
   DevBuf gd, gsc, gr, gm, Dd, Fd, Wd, tha;
   EigWork rr;
-  LdlFactor ldl;
+  LdlFactor ldl, ldl2;         // current shift; the next one (early factorisation)
+  hipStream_t s2 = nullptr;    // stream of the early factorisation
+  hipEvent_t ev_s = nullptr;
   PhaseClock clock;
   std::vector<double> thL, gw;
   std::vector<int> perm;

@@ -308,6 +308,19 @@ void LdlFactor::ensure(int n_) {
 }
 
 int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
+  factor_begin(H, sigma, s);
+  return factor_end(s);
+}
+
+int LdlFactor::factor_end(hipStream_t s) {
+  DVR_HIP(hipMemcpyAsync(hd.data(), d.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  DVR_HIP(hipStreamSynchronize(s));
+  int neg = 0;
+  for (int i = 0; i < n; ++i) neg += hd[i] < 0.0;
+  return neg;
+}
+
+void LdlFactor::factor_begin(const double* H, double sigma, hipStream_t s) {
   double* a = A.as<double>();
   const long lda = n;
   DVR_LAUNCH("ldl_shift", 8.0 * n * (double)(n + 1), 0.0, k_shift_copy, dim3(grid_for((long)n * n)),
@@ -439,11 +452,6 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
       }
     }
   }
-  DVR_HIP(hipMemcpyAsync(hd.data(), dd, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  DVR_HIP(hipStreamSynchronize(s));
-  int neg = 0;
-  for (int i = 0; i < n; ++i) neg += hd[i] < 0.0;
-  return neg;
 }
 
 // T <- (L D L^T)^-1 R for nrhs columns (R, T: n x nrhs, ld n).  R is used as

@@ -133,6 +133,27 @@ def test_davidson_block_wider_than_one_gram_factorisation(monkeypatch):
     assert len(ev) == len(gold) and np.abs(ev - gold).max() <= 1e-4
 
 
+@pytest.mark.parametrize("early,slice_", [("0", "256"), ("1e-3", "96"), ("1e-1", "64"),
+                                           ("1e-1", "256")])
+def test_davidson_early_shift_paths(early, slice_, monkeypatch):
+    """The next shift factored on a second stream while a slice converges
+    (DVR_DAV_EARLY = residual that triggers it; 0 = after the slice, on the
+    solver's stream).  With rough Ritz values (1e-1) the shift lands anywhere:
+    the slice then takes every level below it, or, when the locked space fills
+    first, is certified at its top as without the early shift.  c1 levels
+    unchanged against the reference in every case."""
+    monkeypatch.setenv("DVR_DAV_EARLY", early)
+    monkeypatch.setenv("DVR_DAV_SLICE", slice_)
+    job, pes = H.job_and_pes("c1")
+    ctx = H.run_device(H.device_inputs(job, pes, solver=DAVIDSON))
+    st = ctx.solver_stats()
+    assert st["status"] == 0 and st["dense_fallback"] == 0, st
+    ev = ctx.eigenvalues() * O.AUTOCM
+    gold = np.array(H.golden("c1")["eval_cm"])
+    assert st["certified"] >= len(gold)
+    assert len(ev) == len(gold) and np.abs(ev - gold).max() <= 1e-4
+
+
 def test_davidson_wavefunctions_match_dense(monkeypatch):
     monkeypatch.setenv("DVR_DAV_TOL", "1e-10")
     job, pes = H.job_and_pes("c0")
