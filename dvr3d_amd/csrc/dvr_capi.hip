@@ -415,6 +415,7 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.ldl_sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
     C->dav.ldl_cp = env_int("DVR_LDL_CP", 1) != 0;
     if (const char* t = getenv("DVR_DAV_EARLY")) C->dav.early_tol = atof(t);
+    C->dav.first_overlap = std::max(0, env_int("DVR_DAV_FIRST", 0));
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.keep = env_int("DVR_DAV_KEEP", P.keep);

@@ -233,6 +233,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
   LdlFactor* nxt = &ldl2;
   thL.clear();
   nlocked = 0;
+  first_pending = false;
 
   // start: unit vectors on the b smallest diagonal elements
   std::vector<double> dg(N);
@@ -289,10 +290,24 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
          true);
     small_eig(m, T, nullptr, w0, s);
     sigma = w0[0];
-    clock.mark(PH_FACTOR, s);
-    cur->factor(H, sigma, s);
     stats.factorizations++;
+    if (first_overlap > 0 && early_on) {
+      // factored on s2; until it is ready the corrections use the diagonal
+      DVR_HIP(hipEventRecord(ev_s, s));
+      DVR_HIP(hipStreamWaitEvent(s2, ev_s, 0));
+      cur->factor_begin(H, sigma, s2);
+      first_pending = true;
+    } else {
+      clock.mark(PH_FACTOR, s);
+      cur->factor(H, sigma, s);
+    }
   }
+  // every later use of the first factor on s comes after this
+  auto first_done = [&]() {
+    if (!first_pending) return;
+    cur->factor_end(s2);
+    first_pending = false;
+  };
 
   for (;;) {
     int locked_slice = 0;
@@ -334,6 +349,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
           if (nu_e < 0) {
             clock.mark(PH_FACTOR, s);
             nu_e = nxt->factor_end(s2);
+            first_pending = false;   // s2 has drained
           }
           // fewer levels than the inertia: one has not appeared yet, iterate on
           end_slice = found_below(sig_e) >= nu_e || nlocked >= nlmax;
@@ -377,7 +393,14 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       clock.mark(PH_SI, s);
       launch_resid_gather(s, X.as<double>(), AX.as<double>(), N, thd.as<double>(),
                           upload(idxd, act, s), na, Rb.as<double>(), N, N);
-      if (si) {
+      // a fixed number of diagonal corrections while the first factorisation
+      // runs (fixed, so that the iterates do not depend on timing)
+      bool use_si = si;
+      if (si && first_pending) {
+        if (stats.iterations > first_overlap) first_done();
+        else use_si = false;
+      }
+      if (use_si) {
         cur->solve_graph(Rb.as<double>(), Tb.as<double>(), na, s);
       } else {
         std::vector<double> ta(na);
@@ -434,6 +457,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       const double sig_hi = have_next ? 0.5 * (top + th_next)
                                       : top + 1e-6 * (std::fabs(top) + 1.0);
       clock.mark(PH_FACTOR, s);
+      first_done();
       nu = cur->factor(H, sig_hi, s);
       stats.factorizations++;
       sigma = sig_hi;

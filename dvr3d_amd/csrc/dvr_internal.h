@@ -652,6 +652,9 @@ class DavidsonSolver {
   // slice fix the next shift: its factorisation then starts on a second
   // stream while the slice converges (0: after the slice, on the same stream)
   double early_tol = 1e-4;
+  // iterations with diagonal corrections while the first factorisation runs
+  // on the second stream (needs early_tol > 0; 0: factored before them)
+  int first_overlap = 0;
   ~DavidsonSolver();
 
  private:
@@ -669,6 +672,7 @@ class DavidsonSolver {
   LdlFactor ldl, ldl2;         // current shift; the next one (early factorisation)
   hipStream_t s2 = nullptr;    // stream of the early factorisation
   hipEvent_t ev_s = nullptr;
+  bool first_pending = false;  // the first factorisation still running on s2
   PhaseClock clock;
   std::vector<double> thL, gw;
   std::vector<int> perm;

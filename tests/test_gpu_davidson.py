@@ -133,16 +133,21 @@ def test_davidson_block_wider_than_one_gram_factorisation(monkeypatch):
     assert len(ev) == len(gold) and np.abs(ev - gold).max() <= 1e-4
 
 
-@pytest.mark.parametrize("early,slice_", [("0", "256"), ("1e-3", "96"), ("1e-1", "64"),
-                                           ("1e-1", "256")])
-def test_davidson_early_shift_paths(early, slice_, monkeypatch):
+@pytest.mark.parametrize("early,slice_,first", [("0", "256", "6"), ("1e-3", "96", "6"),
+                                                 ("1e-1", "64", "6"), ("1e-1", "256", "6"),
+                                                 ("1e-4", "256", "0"), ("1e-4", "256", "30")])
+def test_davidson_early_shift_paths(early, slice_, first, monkeypatch):
     """The next shift factored on a second stream while a slice converges
     (DVR_DAV_EARLY = residual that triggers it; 0 = after the slice, on the
     solver's stream).  With rough Ritz values (1e-1) the shift lands anywhere:
     the slice then takes every level below it, or, when the locked space fills
     first, is certified at its top as without the early shift.  c1 levels
-    unchanged against the reference in every case."""
+    unchanged against the reference in every case.  DVR_DAV_FIRST = F > 0
+    also factors the first shift on the second stream, with diagonal
+    corrections for the first F iterations; 0 factors it before the first
+    iteration."""
     monkeypatch.setenv("DVR_DAV_EARLY", early)
+    monkeypatch.setenv("DVR_DAV_FIRST", first)
     monkeypatch.setenv("DVR_DAV_SLICE", slice_)
     job, pes = H.job_and_pes("c1")
     ctx = H.run_device(H.device_inputs(job, pes, solver=DAVIDSON))
