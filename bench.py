@@ -116,6 +116,114 @@ def cpu_baseline(cfg: str, budget_s: float):
                        f"{total:.0f} s (numpy/OpenBLAS, {threads} threads)")
 
 
+def progress(msg):
+    """One progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def _deck(cfg, neval=None, max3d=None):
+    """Text of a workload's .job deck, optionally with NEVAL / MAX3D changed
+    (fields 3 and 6 of the 11i5 size card)."""
+    lines = open(os.path.join(REPO, "dvr3d_amd", "jobs", CONFIGS[cfg][0])).read().split("\n")
+    f = [lines[2][i * 5:(i + 1) * 5] for i in range(11)]
+    if neval is not None:
+        f[2] = "%5d" % neval
+    if max3d is not None:
+        f[5] = "%5d" % max3d
+    lines[2] = "".join(f)
+    return "\n".join(lines)
+
+
+def cpu_baseline_reference(cfg: str):
+    """The reference program itself (oracle/_ref/dvr3drjz_fast: DVR3DRJZ
+    built with amdflang -O2 and MKL sequential, the rm2t = 0 loops of mkham3
+    skipped and dsyev given its optimal workspace, which leaves the levels
+    unchanged to 1e-9 cm^-1) on one host core each, several runs at once on
+    different cores.  c0/c1: the full deck.  c3 would take over an hour, so
+    the sample is: the c3 deck with MAX3D = 500 and 1500 (1D and 2D stages in
+    full, the 3D assembly and final dsyev at those dimensions) and the
+    reference's dsyev call alone at N = 3000; the run time at N3 = 20000 is
+    extrapolated as T = T12 + a N3^2 + b N3^3 (b from the dsyev timing, a and
+    T12 from the two runs).  Returns None when the executables are absent."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = os.path.join(REPO, "oracle", "_ref", "dvr3drjz_fast")
+    dtime = os.path.join(REPO, "oracle", "_ref", "dsyev_time")
+    if not (os.path.exists(exe) and os.path.exists(dtime)):
+        return None
+    cores = sorted(os.sched_getaffinity(0))
+    model = ""
+    try:
+        for l in open("/proc/cpuinfo"):
+            if l.startswith("model name"):
+                model = l.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    tmp = tempfile.mkdtemp(prefix="dvr3d_cpu_")
+    jobs = []   # (name, argv, stdin text)
+    if cfg == "c3":
+        jobs = [("m500", [exe], _deck("c3", 100, 500)), ("m1500", [exe], _deck("c3", 100, 1500)),
+                ("dsyev3000", [dtime, "3000"], None), ("c0", [exe], _deck("c0"))]
+    else:
+        jobs = [(cfg, [exe], _deck(cfg))]
+    procs = []
+    for i, (name, argv, deck) in enumerate(jobs):
+        d = os.path.join(tmp, name)
+        os.makedirs(d)
+        src = "c0" if name == "c0" else cfg
+        shutil.copy(os.path.join(REPO, "dvr3d_amd", "jobs", CONFIGS[src][1]), os.path.join(d, "pes.par"))
+        core = cores[(i + 1) % len(cores)] if len(cores) > 1 else cores[0]
+        fin = None
+        if deck is not None:
+            open(os.path.join(d, "in.job"), "w").write(deck)
+            fin = open(os.path.join(d, "in.job"))
+        fo = open(os.path.join(d, "out.txt"), "w")
+        t0 = time.perf_counter()
+        p = subprocess.Popen(["taskset", "-c", str(core), "timeout", "600", *argv], stdin=fin,
+                             stdout=fo, stderr=subprocess.STDOUT, cwd=d,
+                             env=dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1"))
+        procs.append([name, p, t0, None, fo, d])
+    while any(x[3] is None for x in procs):
+        for x in procs:
+            if x[3] is None and x[1].poll() is not None:
+                x[3] = time.perf_counter() - x[2]
+        time.sleep(0.05)
+    t = {}
+    for name, p, _, dt, fo, d in procs:
+        fo.close()
+        if p.returncode != 0:
+            shutil.rmtree(tmp, ignore_errors=True)
+            return None
+        t[name] = dt
+        if name.startswith("dsyev"):
+            t[name] = float(open(os.path.join(d, "out.txt")).read().split()[1])
+    shutil.rmtree(tmp, ignore_errors=True)
+    info = dict(cores=1, kind="reference", cpu_model=model, nproc=os.cpu_count(),
+                affinity_cores=len(cores),
+                build="amdflang -O2, MKL 2021.4 sequential; fast variant (rm2t=0 loops skipped, "
+                      "dsyev optimal LWORK) of SFIT_DVR3D/H2S/dvr3drjz.f90")
+    if cfg != "c3":
+        nlev = {"c0": 100, "c1": 500}.get(cfg, 1)
+        info.update(value=nlev / t[cfg], unit="levels/s",
+                    sample=f"one full {cfg} run of the reference on 1 core: {t[cfg]:.2f} s")
+        return info
+    b = t["dsyev3000"] / 3000.0 ** 3
+    a = (t["m1500"] - t["m500"] - b * (1500.0 ** 3 - 500.0 ** 3)) / (1500.0 ** 2 - 500.0 ** 2)
+    t12 = t["m500"] - a * 500.0 ** 2 - b * 500.0 ** 3
+    n3 = 20000.0
+    total = t12 + a * n3 ** 2 + b * n3 ** 3
+    info.update(value=1000.0 / total, unit="levels/s", extrapolated=True,
+                sample=f"reference on 1 core each (4 runs side by side): c3 deck with MAX3D=500 "
+                       f"{t['m500']:.1f} s, MAX3D=1500 {t['m1500']:.1f} s, its dsyev at N=3000 "
+                       f"{t['dsyev3000']:.2f} s -> 1D+2D stages {t12:.1f} s, 3D assembly "
+                       f"{a * n3 ** 2:.0f} s and dsyev {b * n3 ** 3:.0f} s at N3=20000: "
+                       f"extrapolated {total:.0f} s per solve",
+                c0_full_run=dict(seconds=round(t["c0"], 3), levels_per_s=round(100 / t["c0"], 3)))
+    return info
+
+
 def roofline_of(kprof):
     """Dominant kernel of a profiled step and its roofline position."""
     if not kprof:
@@ -149,19 +257,12 @@ def perturbed_par(base, seed):
     return f
 
 
-def sweep(args):
-    """c4: seeded perturbations of the c1 potential (every PES parameter
-    scaled by 1 + N(0, 0.01)), lowest 300 levels each, solved one after
-    another in two contexts.  The first C4_TRAIN instances run the plain
-    dense path with eigenvectors, and each one's Hamiltonian and eigenpairs
-    train the learned preconditioner on the GPU (dvr_mlp_train, continuing
-    from the previous instance's parameters); the remaining instances run
-    block-Davidson with that learned preconditioner (with the dense path as
-    fallback when it stalls).  Levels of two instances of each part are
-    checked against the FP64 oracle's (tests/golden/c4_oracle.json)."""
+def c4_setup():
+    """Basis of the c4 sweep (the c1 grid) and factories of its contexts and
+    perturbed PES parameters: returns (basis, angular points, context(solver,
+    nvec), params(instance) -> device AB2 parameters)."""
     import math
     from dvr3d_amd import _lib, host
-    os.environ.setdefault("DVR_DAV_MAXIT", "100")
     job = host.read_job(os.path.join(REPO, "dvr3d_amd", "jobs", "c1.job"))
     b = host.Basis(job)
     lines = [l for l in open(os.path.join(REPO, "dvr3d_amd", "jobs", "ab2_c0.par")).read().split("\n")
@@ -185,6 +286,22 @@ def sweep(args):
         return np.array([p[0], p[1], p[2], p[3],
                          math.cos(p[4] * (4.0 * math.atan(1.0)) / 180.0), p[5]])
 
+    return b, xa, context, params
+
+
+def sweep(args):
+    """c4: seeded perturbations of the c1 potential (every PES parameter
+    scaled by 1 + N(0, 0.01)), lowest 300 levels each, solved one after
+    another in two contexts.  The first C4_TRAIN instances run the plain
+    dense path with eigenvectors, and each one's Hamiltonian and eigenpairs
+    train the learned preconditioner on the GPU (dvr_mlp_train, continuing
+    from the previous instance's parameters); the remaining instances run
+    block-Davidson with that learned preconditioner (with the dense path as
+    fallback when it stalls).  Levels of two instances of each part are
+    checked against the FP64 oracle's (tests/golden/c4_oracle.json)."""
+    from dvr3d_amd import _lib, host
+    os.environ.setdefault("DVR_DAV_MAXIT", "100")
+    b, xa, context, params = c4_setup()
     n = args.instances
     ntrain = min(C4_TRAIN, n)
     gold = json.load(open(os.path.join(REPO, "tests", "golden", "c4_oracle.json")))["eval_au"]
@@ -418,10 +535,13 @@ def main():
             out["kprof"] = ctx.kernel_profile()
         return out
 
+    progress(f"{cfg}: {args.solver} solve, {args.warmup} warm-up + {args.steps} timed steps")
     main_run = measure(args.solver, args.steps, args.warmup, True)
+    progress(f"{cfg}: {args.solver} {main_run['ms_per_step']:.1f} ms per step")
     plain = None
     if args.solver == "davidson" and args.plain_steps > 0:
         plain = measure("dense", args.plain_steps, 1, False)
+        progress(f"{cfg}: plain dense {plain['ms_per_step']:.1f} ms per step")
     kprof = main_run.pop("kprof")
     line = {
         "metric": "converged levels/s at <=1e-3 cm^-1 vs FP64 oracle: chosen form vs plain FP64 GPU",
@@ -454,7 +574,15 @@ def main():
             "stage_ms": plain["stage_ms"]}
         line["config"]["speedup_vs_plain"] = round(plain["ms_per_step"] / main_run["ms_per_step"], 3)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(cfg, 30.0)
+        progress("CPU baseline: the reference program on host cores")
+        ref = cpu_baseline_reference(cfg)
+        progress("CPU baseline: the oracle port")
+        port = cpu_baseline(cfg, 30.0)
+        if ref is not None:
+            line["cpu_baseline"] = ref
+            line["cpu_baseline_port"] = port
+        else:
+            line["cpu_baseline"] = port
     print(json.dumps(line))
 
 

@@ -46,6 +46,7 @@ struct dvr_ctx {
   EigWork w2, w3;
   DavidsonSolver dav;
   int dav_status = -1;
+  std::string dav_error;   // message of an exception inside the Davidson solver
   bool keep_h = false, h_kept = false, split32 = false;
   double refine_resid = 0.0;
   DevBuf Zr;
@@ -428,7 +429,16 @@ void run_pipeline(dvr_ctx_t* C) {
     }
     if (const char* t = getenv("DVR_DAV_TOL")) P.tol = atof(t);
     C->dav.tuner.enabled = env_int("DVR_TUNE", 1) != 0;
-    C->dav_status = C->dav.solve(C->H3.as<double>(), n3, P, s);
+    C->dav_error.clear();
+    try {
+      C->dav_status = C->dav.solve(C->H3.as<double>(), n3, P, s);
+    } catch (const std::exception& e) {
+      // an error inside the iterative solver hands the problem to the dense
+      // one (H3 is intact: the solver works on copies); a device fault
+      // resurfaces there
+      C->dav_status = 4;
+      C->dav_error = e.what();
+    }
     if (C->dav_status == 0) {
       C->h_eval.assign(C->dav.eval.begin(), C->dav.eval.begin() + meval);
       if (C->split32) {
@@ -884,6 +894,7 @@ int dvr_mlp_backward(const float* params, int64_t nrows, const float* feats, con
 int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int32_t nrhs,
                   double* t, int32_t* inertia) {
   if (!a || n <= 0 || nrhs < 0 || (nrhs > 0 && (!r || !t))) return DVR_ERR_ARG;
+  bool bad = false;
   try {
     hipStream_t s;
     DVR_HIP(hipStreamCreate(&s));
@@ -900,7 +911,10 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
       f.ensure(n);
       const int nu = f.factor(H.as<double>(), sigma, s);
       if (inertia) *inertia = nu;
-      if (nrhs > 0) {
+      if (nu < 0) {
+        DVR_HIP(hipStreamSynchronize(s));
+        bad = true;
+      } else if (nrhs > 0) {
         R.ensure(sizeof(double) * (size_t)n * nrhs);
         T.ensure(sizeof(double) * (size_t)n * nrhs);
         DVR_HIP(hipMemcpyAsync(R.p, r, sizeof(double) * (size_t)n * nrhs, hipMemcpyHostToDevice,
@@ -916,7 +930,7 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
     fprintf(stderr, "dvr_ldl_solve: %s\n", e.what());
     return DVR_ERR_HIP;
   }
-  return DVR_OK;
+  return bad ? DVR_ERR_NOCONV : DVR_OK;
 }
 
 int dvr_set_profiling(dvr_ctx_t* c, int32_t on) {

@@ -67,6 +67,7 @@ void DavidsonSolver::small_eig(int m, double* S, double* U, std::vector<double>&
 // factorisation (64) go through in chunks of 64, each orthogonalised
 // against the basis grown by the chunks before it.
 int DavidsonSolver::expand(const double* H, int q0, hipStream_t s) {
+  if (q0 <= 0) return 0;
   if (q0 <= 64) return expand_chunk(H, Tb.as<double>(), q0, s);
   const int m0 = m;
   int qt = 0;
@@ -174,7 +175,16 @@ DavidsonSolver::~DavidsonSolver() {
 
 int DavidsonSolver::solve(const double* H, int n, const DavParams& P, hipStream_t s) {
   clock.used = 0;
-  const int rc = solve_impl(H, n, P, s);
+  int rc = 0;
+  try {
+    rc = solve_impl(H, n, P, s);
+  } catch (...) {
+    // nothing may still read H or the factors when the caller takes over
+    if (s2) (void)hipStreamSynchronize(s2);
+    (void)hipStreamSynchronize(s);
+    clock.used = 0;
+    throw;
+  }
   // an early factorisation still running when the solve ended (stall)
   if (s2) DVR_HIP(hipStreamSynchronize(s2));
   clock.mark(-1, s);
@@ -299,13 +309,16 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       first_pending = true;
     } else {
       clock.mark(PH_FACTOR, s);
-      cur->factor(H, sigma, s);
+      // a zero or non-finite pivot (shift on an eigenvalue): the solves
+      // would not be usable
+      if (cur->factor(H, sigma, s) < 0) return 3;
     }
   }
   // every later use of the first factor on s comes after this
+  bool first_bad = false;
   auto first_done = [&]() {
     if (!first_pending) return;
-    cur->factor_end(s2);
+    first_bad = cur->factor_end(s2) < 0;
     first_pending = false;
   };
 
@@ -350,6 +363,8 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
             clock.mark(PH_FACTOR, s);
             nu_e = nxt->factor_end(s2);
             first_pending = false;   // s2 has drained
+            // no trustworthy inertia at this shift: certification failed
+            if (nu_e < 0) return 3;
           }
           // fewer levels than the inertia: one has not appeared yet, iterate on
           end_slice = found_below(sig_e) >= nu_e || nlocked >= nlmax;
@@ -385,6 +400,10 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       std::vector<int> act;
       for (int j = nl; j < m && (int)act.size() < b; ++j)
         if (!(nrm[j] <= tol)) act.push_back(j);
+      // every Ritz pair left is converged but cannot be locked (the space is
+      // exhausted, or a level below the next shift has not appeared): no
+      // correction to expand with, the iteration has stalled
+      if (act.empty()) return 2;
       int mk = m - nl;
       // full space: thick restart on the lowest Ritz vectors
       if (mk + (int)act.size() > mmax) mk = keep > 0 ? std::min(keep, mmax - (int)act.size())
@@ -401,6 +420,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
         else use_si = false;
       }
       if (use_si) {
+        if (first_bad) return 3;
         cur->solve_graph(Rb.as<double>(), Tb.as<double>(), na, s);
       } else {
         std::vector<double> ta(na);
@@ -460,6 +480,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       first_done();
       nu = cur->factor(H, sig_hi, s);
       stats.factorizations++;
+      if (nu < 0) return 3;
       sigma = sig_hi;
       found = found_below(sig_hi);
     }

@@ -546,6 +546,8 @@ struct LdlFactor {
   void ensure(int n);
   // factors H - sigma I (H: n x n full square, ld n); returns the inertia
   // count (number of negative pivots = eigenvalues of H below sigma)
+  // (-1 when a pivot is not finite or below n eps max|d|: the count is then
+  // not trustworthy)
   int factor(const double* H, double sigma, hipStream_t s);
   // the same in two halves: the launches (the host does not wait), then the
   // wait for s and the inertia count
@@ -573,6 +575,7 @@ struct LdlFactor {
     const void* key[6] = {};
   };
   std::vector<std::pair<int, Captured>> graphs;
+  int graph_sb = 0;   // super-block size the graphs were captured with
   void clear_graphs();
 };
 

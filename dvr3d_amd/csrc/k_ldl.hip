@@ -20,6 +20,8 @@
 // the identity), so that one application costs a few GEMMs per super-block.
 #include "dvr_internal.h"
 
+#include <cmath>
+
 namespace dvr {
 
 namespace {
@@ -297,6 +299,10 @@ void launch_ldl_block(hipStream_t s, double* A, long lda, int nb, double* d, dou
 }
 
 void LdlFactor::ensure(int n_) {
+  // the captured solve graphs hold n, sb and the super-block count as kernel
+  // arguments: a new order or super-block size needs new graphs
+  if (n_ != n || (n_ + sb - 1) / sb != nsb || sb != graph_sb) clear_graphs();
+  graph_sb = sb;
   n = n_;
   nsb = (n + sb - 1) / sb;
   A.ensure(sizeof(double) * (size_t)n * n);
@@ -315,8 +321,19 @@ int LdlFactor::factor(const double* H, double sigma, hipStream_t s) {
 int LdlFactor::factor_end(hipStream_t s) {
   DVR_HIP(hipMemcpyAsync(hd.data(), d.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   DVR_HIP(hipStreamSynchronize(s));
+  // Sylvester's count is only meaningful when every pivot is finite and
+  // clearly non-zero: a NaN/Inf (the reciprocal of a zero pivot) or a pivot
+  // below n eps max|d| makes the count ambiguous, reported as -1
   int neg = 0;
-  for (int i = 0; i < n; ++i) neg += hd[i] < 0.0;
+  double dmax = 0.0, dmin = HUGE_VAL;
+  for (int i = 0; i < n; ++i) {
+    const double x = hd[i];
+    if (!std::isfinite(x)) return -1;
+    neg += x < 0.0;
+    dmax = std::max(dmax, std::fabs(x));
+    dmin = std::min(dmin, std::fabs(x));
+  }
+  if (!(dmin > (double)n * 2.220446049250313e-16 * dmax)) return -1;
   return neg;
 }
 
@@ -505,7 +522,14 @@ void LdlFactor::solve(double* R, double* T, int nrhs, hipStream_t s) {
 
 namespace dvr {
 
+// An executable graph is destroyed only after the streams it may have been
+// launched on have drained: a replay can still be queued behind earlier work
+// (the solver does not wait for its solves), and the runtime and a tracing
+// profiler keep referring to the graph's kernel nodes until the dispatches
+// complete.
 void LdlFactor::clear_graphs() {
+  if (graphs.empty()) return;
+  (void)hipDeviceSynchronize();
   for (auto& g : graphs)
     if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
   graphs.clear();
@@ -539,6 +563,7 @@ void LdlFactor::solve_graph(double* R, double* T, int nrhs, hipStream_t s) {
   if (!std::equal(key, key + 6, key2)) clear_graphs();
   for (auto it = graphs.begin(); it != graphs.end(); ++it)
     if (it->first == nrhs) {
+      DVR_HIP(hipStreamSynchronize(s));
       (void)hipGraphExecDestroy(it->second.exec);
       graphs.erase(it);
       break;

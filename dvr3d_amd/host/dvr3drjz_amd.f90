@@ -200,7 +200,7 @@ program dvr3drjz_amd
   mst = 0.0d0
   do ik = 1, nk
     kz = sz%kd + ik - 1
-    if (nk > 1) write(6, '(//5x,a,i3)') 'k block, k =', kz
+    if (sz%zrot /= 0) write(6, '(/5x,a,i3,a,i3)') 'Solutions with J =', sz%jrot, ' k =', kz
     rc = dvr_get_eigenvalues(ctx(ik), eval, meval)
     if (job%zpmin == 0) then
       write(6, 1010) meval

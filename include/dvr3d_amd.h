@@ -140,7 +140,8 @@ int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
 
 /* Statistics of the last final solve (no counterpart in the reference, which
  * only has the dense dsyev call).  Fills up to n of: [0] Davidson status
- * (-1 not run, 0 certified, 2 stalled, 3 missed level), [1] iterations,
+ * (-1 not run, 0 certified, 2 stalled, 3 missed level or no trustworthy
+ * inertia at a shift, 4 error inside the iterative solver), [1] iterations,
  * [2] H-vector products, [3] shift-and-invert solves, [4] LDL^T
  * factorisations, [5] slices, [6] inertia count at the last shift,
  * [7] levels missing at the failed certification, [8] 1 when the dense
@@ -157,7 +158,10 @@ int dvr_get_solver_stats(const dvr_ctx_t* ctx, int32_t* stats, int32_t n);
  * Factors A - sigma I = L D L^T (A: n x n column-major, full square, host
  * memory), returns in *inertia the number of negative pivots (= eigenvalues
  * of A below sigma by Sylvester's law) and, for nrhs > 0, solves
- * (A - sigma I) t = r for the nrhs columns of r (n x nrhs) into t. */
+ * (A - sigma I) t = r for the nrhs columns of r (n x nrhs) into t.  The
+ * factorisation is unpivoted: when a pivot is not finite or below
+ * n eps max|d| the count is ambiguous, *inertia is set to -1 and
+ * DVR_ERR_NOCONV is returned (nothing is solved). */
 int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int32_t nrhs,
                   double* t, int32_t* inertia);
 

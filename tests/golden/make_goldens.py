@@ -28,6 +28,7 @@ import re
 import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -164,7 +165,7 @@ def write_iwave_fixture(cfg: str, path: str):
 def main(argv):
     if argv and argv[0] == "--jrot":
         cfg = argv[1]
-        rundir = os.path.join(REPO, "_work", "runs", cfg)
+        rundir = os.path.join(tempfile.mkdtemp(prefix="dvr3d_ref_"), cfg)
         os.makedirs(rundir, exist_ok=True)
         shutil.copy(os.path.join(JOBS, cfg + ".job"), os.path.join(rundir, cfg + ".job"))
         shutil.copy(os.path.join(JOBS, PES_OF[cfg]), os.path.join(rundir, "pes.par"))
@@ -191,7 +192,7 @@ def main(argv):
         res["source"] = "oracle/_ref/dvr3drjz_ref on dvr3d_amd/jobs/%s.job" % cfg
         json.dump(res, open(os.path.join(REPO, "tests", "golden", cfg + "_ref.json"), "w"), indent=1)
         return
-    rundir = os.path.join(REPO, "_work", "runs")
+    rundir = tempfile.mkdtemp(prefix="dvr3d_ref_")
     cfgs = [a for a in argv if not a.startswith("--")]
     for cfg in cfgs:
         res = run_config(cfg, os.path.join(rundir, cfg))

@@ -26,6 +26,7 @@ import re
 import shutil
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -87,7 +88,7 @@ def main():
     for name in JOBS:
         jobfile = os.path.join(REPO, "dvr3d_amd", "jobs", name + ".job")
         job = O.load_job(jobfile)
-        rundir = os.path.join(REPO, "_work", "runs", name)
+        rundir = os.path.join(tempfile.mkdtemp(prefix="dvr3d_ref_"), name)
         os.makedirs(rundir, exist_ok=True)
         shutil.copy(jobfile, os.path.join(rundir, name + ".job"))
         with open(os.path.join(rundir, name + ".job")) as fi, \

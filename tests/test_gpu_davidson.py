@@ -202,7 +202,52 @@ def test_graph_replay_and_tuner_match_eager(monkeypatch):
         runs[(graph, tune)] = ctx.eigenvalues() * O.AUTOCM
         # a second solve in the same context replays the captured graphs
         ctx.run()
-        assert np.array_equal(ctx.eigenvalues() * O.AUTOCM, runs[(graph, tune)]) or graph == 1
-        assert np.abs(ctx.eigenvalues() * O.AUTOCM - runs[(graph, tune)]).max() <= 1e-6
+        d = np.abs(ctx.eigenvalues() * O.AUTOCM - runs[(graph, tune)]).max()
+        print(f"graph {graph} tune {tune}: rerun differs by {d:.3e} cm-1")
+        assert np.array_equal(ctx.eigenvalues() * O.AUTOCM, runs[(graph, tune)])
     assert np.array_equal(runs[(0, 0)], runs[(1, 0)])
     assert np.abs(runs[(1, 1)] - runs[(0, 0)]).max() <= 1e-6
+
+
+def test_davidson_context_rerun_with_smaller_basis():
+    """One Davidson context run twice on a zcut=T deck (c0 grid, EMAX2 =
+    18000 cm^-1) with two potentials whose final dimensions differ (244, then
+    184): the second solve must not reuse anything sized or captured for the
+    first (device buffers do not shrink, so the pointers stay the same).
+    Levels within 1e-4 cm^-1 of the oracle, certified, without fallback."""
+    import copy
+    job, pes = H.job_and_pes("c0")
+    job = copy.deepcopy(job)
+    job["nml"]["zcut"] = True
+    job["emax2"] = 18000.0
+    pes2 = copy.deepcopy(pes)
+    pes2.kb *= 1.3
+    pes2.de *= 1.1
+    inp = H.device_inputs(job, pes, solver=DAVIDSON)
+    ctx = H.run_device(inp)
+    for k, p in enumerate((pes, pes2)):
+        if k:
+            ctx.eval_potential_ab2(inp["rc"], inp["rb"], inp["xa"],
+                                   np.array([p.de, p.a, p.re, p.kb, p.cte, p.c]))
+            ctx.run()
+        ref = O.run(job, p)
+        st = ctx.solver_stats()
+        n3 = ctx.sizes()[2]
+        print(f"potential {k}: nham3 {n3}, {st['iterations']} iterations, status {st['status']}")
+        assert n3 == ref["nham3"] == (244, 184)[k]
+        assert st["status"] == 0 and st["dense_fallback"] == 0, st
+        assert st["iterations"] < 100
+        ev = ctx.eigenvalues()
+        assert np.abs(ev - ref["eval"][: len(ev)]).max() * O.AUTOCM <= 1e-4
+
+
+def test_ldl_solve_zero_pivot_reported():
+    """A shift on an eigenvalue gives a zero pivot: the inertia is ambiguous
+    and dvr_ldl_solve reports DVR_ERR_NOCONV instead of a count."""
+    from dvr3d_amd import _lib
+    a = np.diag(np.arange(1.0, 9.0))
+    with pytest.raises(_lib.DvrError) as e:
+        _lib.ldl_solve(a, 3.0)
+    assert e.value.code == _lib.DVR_ERR_NOCONV
+    nu, _ = _lib.ldl_solve(a, 3.5)
+    assert nu == 3

@@ -137,6 +137,9 @@ def test_k2s_jrot_levels_and_iwave_match_reference(tmp_path):
     g = json.load(open(os.path.join(H.GOLD, "k2s_ref.json")))["eval_au"]
     blocks = _levels_per_k(out)
     assert len(blocks) == 3
+    # the reference's per-k heading of a zrot run (D:2682-2684, format 1040)
+    heads = [l for l in out.splitlines() if "Solutions with J =" in l]
+    assert heads == [f"     Solutions with J =  2 k = {k:2d}" for k in range(3)]
     for k, b in enumerate(blocks):
         assert len(b) == 30
         assert np.abs(b - np.array(g[str(k)])).max() * O.AUTOCM < 1e-4

@@ -227,3 +227,34 @@ def test_h3p_jobs_match_reference(name):
         assert r["ndim2d"].tolist() == gb["nham2"]
         assert r["iv2"].tolist() == gb["iv2"] and r["nham3"] == gb["nham3"]
         assert np.abs(r["eval"] - np.array(gb["eval_au"])).max() < 1e-12
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(H.REPO, "oracle", "_ref", "dvr3drjz_fast")),
+                    reason="reference executables not built (oracle/Makefile)")
+def test_fast_reference_build_equals_as_shipped(tmp_path):
+    """The fast reference build (rm2t = 0 loops skipped, dsyev with its
+    optimal workspace; the CPU baseline of bench.py) against the as-shipped
+    build on c0: stdout levels and ILEV identical, IWAVE eigenvalues within
+    1e-9 cm^-1 and wavefunctions within 1e-10 (sign-aligned)."""
+    import shutil
+    import subprocess
+    from tests import fortran_io as F
+    outs = {}
+    for name in ("dvr3drjz_ref", "dvr3drjz_fast"):
+        d = tmp_path / name
+        d.mkdir()
+        shutil.copy(os.path.join(H.JOBS, "ab2_c0.par"), d / "pes.par")
+        with open(os.path.join(H.JOBS, "c0.job")) as fi:
+            r = subprocess.run([os.path.join(H.REPO, "oracle", "_ref", name)], stdin=fi,
+                               cwd=d, capture_output=True, text=True, timeout=300, check=True)
+        outs[name] = (r.stdout, (d / "fort.14").read_text(), F.read_records(d / "fort.26"))
+    (sa, la, wa), (sb, lb, wb) = outs["dvr3drjz_ref"], outs["dvr3drjz_fast"]
+    strip = lambda t: [l for l in t.splitlines() if "secs" not in l and "time" not in l]
+    assert strip(sa) == strip(sb)
+    assert la == lb
+    assert len(wa) == len(wb)
+    assert wa[:10] == wb[:10]
+    assert np.abs(F.doubles(wa[10]) - F.doubles(wb[10])).max() * O.AUTOCM < 1e-9
+    for x, y in zip(wa[11:], wb[11:]):
+        x, y = F.doubles(x), F.doubles(y)
+        assert np.abs(x - np.sign(x @ y) * y).max() < 1e-10
