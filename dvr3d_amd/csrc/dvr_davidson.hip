@@ -94,11 +94,17 @@ int DavidsonSolver::expand_chunk(const double* H, double* Wt, int q0, hipStream_
       gemm(s, "orth_upd", false, false, N, q, m, -1.0, Vp, N, Cp, m, 1.0, Z, N);
     }
   };
+  // norms of the raw corrections: a column that the projections reduce
+  // below 1e-10 of its norm lies in the span of the basis and is dropped
+  rawd.ensure(sizeof(double) * 64);
+  launch_coldot(s, Wt, Wt, N, q0, rawd.as<double>());
+  std::vector<double> raw2(q0);
+  DVR_HIP(hipMemcpyAsync(raw2.data(), rawd.p, sizeof(double) * q0, hipMemcpyDeviceToHost, s));
   cgs(Wt, q0);
   cgs(Wt, q0);
   double* Vn = Vp + (long)m * N;
   double* tmp = X.as<double>();   // free between the restart and the next RR
-  int q = cholqr(Wt, q0, tmp, Vn, s);
+  int q = cholqr(Wt, q0, tmp, Vn, s, raw2.data());
   if (q == 0) return 0;
   cgs(Vn, q);
   q = cholqr(Vn, q, tmp, Wt, s);
@@ -113,8 +119,11 @@ int DavidsonSolver::expand_chunk(const double* H, double* Wt, int q0, hipStream_
 // Cholesky QR of the q columns at Z: G = Z^T Z is scaled to unit diagonal,
 // regularised by 1e-14 and factored as L D L^T; out receives the columns of
 // Z S L^-T D^-1/2 whose pivot exceeds 1e-12 (the others are numerically
-// dependent and dropped).  tmp: N x q workspace.  Returns the number kept.
-int DavidsonSolver::cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s) {
+// dependent and dropped) and, with raw2 (host, the squared norms of the
+// columns before their projection), whose squared norm is above 1e-20
+// raw2.  tmp: N x q workspace.  Returns the number kept.
+int DavidsonSolver::cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s,
+                           const double* raw2) {
   double* Gp = G.as<double>();
   gemm(s, "gram", true, false, q, q, N, 1.0, Z, N, Z, N, 0.0, Gp, q);
   gd.ensure(sizeof(double) * 64);
@@ -130,7 +139,8 @@ int DavidsonSolver::cholqr(const double* Z, int q, double* tmp, double* out, hip
   std::vector<double> r(q, 0.0);
   std::vector<int> keep;
   for (int k = 0; k < q; ++k)
-    if (sch[k] > 0.0 && dh[k] > 1e-12) {
+    if (sch[k] > 0.0 && dh[k] > 1e-12 &&
+        (!raw2 || 1.0 / (sch[k] * sch[k]) > 1e-20 * raw2[k])) {
       r[k] = 1.0 / std::sqrt(dh[k]);
       keep.push_back(k);
     }
@@ -199,9 +209,14 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
   const int nev = std::min(P.nev, n);
   const int b = std::max(1, std::min(std::min(P.block, n), 128));   // Gram matrices of <= 64 columns
   const int K = std::max(b, P.slice);
-  const int mmax = std::min(n, std::max(P.mmax, 2 * b));
-  const int keep = P.keep > 0 ? std::max(P.keep, b) : 0;
   const int nlmax = std::min(n, nev + 1);
+  // the locked levels plus the search space plus one correction block must
+  // leave room in R^n: otherwise the corrections have nowhere to go and the
+  // orthonormalisation only produces rounding noise.  Too small a problem
+  // for that is left to the dense solver.
+  if (n < nlmax + 3 * b) return 5;
+  const int mmax = std::min(std::min(n, std::max(P.mmax, 2 * b)), n - nlmax - b);
+  const int keep = P.keep > 0 ? std::min(std::max(P.keep, b), mmax - b) : 0;
   const double tol = P.tol;
   V.ensure(sizeof(double) * (size_t)N * mmax);
   AV.ensure(sizeof(double) * (size_t)N * mmax);

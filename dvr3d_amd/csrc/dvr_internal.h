@@ -636,7 +636,8 @@ struct PhaseClock {
 };
 class DavidsonSolver {
  public:
-  // 0: certified; 2: stalled; 3: a level below the last shift was missed
+  // 0: certified; 2: stalled; 3: a level below the last shift was missed;
+  // 5: too small a problem for the iteration (n < nev + 1 + 3 block)
   int solve(const double* H, int n, const DavParams& P, hipStream_t s);
   int solve_impl(const double* H, int n, const DavParams& P, hipStream_t s);
   // the lowest nvec eigenvectors (ascending) into Z (n x nvec)
@@ -667,10 +668,11 @@ class DavidsonSolver {
   void small_eig(int m, double* S, double* U, std::vector<double>& w, hipStream_t s);
   int expand(const double* H, int q0, hipStream_t s);
   int expand_chunk(const double* H, double* W, int q0, hipStream_t s);
-  int cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s);
+  int cholqr(const double* Z, int q, double* tmp, double* out, hipStream_t s,
+             const double* raw2 = nullptr);
   int N = 0, m = 0, nlocked = 0;
   DevBuf V, AV, X, AX, XL, Rb, Tb, Tm, Y, Y2, G, G2, U, C1, thd, nrmd, idxd, scal, work;
-  DevBuf gd, gsc, gr, gm, Dd, Fd, Wd, tha;
+  DevBuf gd, gsc, gr, gm, Dd, Fd, Wd, tha, rawd;
   EigWork rr;
   LdlFactor ldl, ldl2;         // current shift; the next one (early factorisation)
   hipStream_t s2 = nullptr;    // stream of the early factorisation

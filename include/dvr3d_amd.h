@@ -141,7 +141,8 @@ int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
 /* Statistics of the last final solve (no counterpart in the reference, which
  * only has the dense dsyev call).  Fills up to n of: [0] Davidson status
  * (-1 not run, 0 certified, 2 stalled, 3 missed level or no trustworthy
- * inertia at a shift, 4 error inside the iterative solver), [1] iterations,
+ * inertia at a shift, 4 error inside the iterative solver, 5 too small a
+ * problem for the iteration: N3 < NEVAL + 1 + 3 x 64), [1] iterations,
  * [2] H-vector products, [3] shift-and-invert solves, [4] LDL^T
  * factorisations, [5] slices, [6] inertia count at the last shift,
  * [7] levels missing at the failed certification, [8] 1 when the dense
