@@ -29,7 +29,8 @@ EXPORTS = (
     "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_run_blocks", "dvr_get_sizes",
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
-    "dvr_diag", "dvr_syev_batch", "dvr_set_profiling", "dvr_get_profile",
+    "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_set_profiling",
+    "dvr_get_profile",
     "dvr_get_solver_stats", "dvr_ldl_solve", "dvr_keep_hamiltonian", "dvr_set_preconditioner",
     "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
 )
@@ -86,6 +87,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_get_stage_times.argtypes = [P, D]
     lib.dvr_diag.argtypes = [D, C.c_int32, C.c_int32, D]
     lib.dvr_syev_batch.argtypes = [C.c_int32, I, D, I, D, D]
+    lib.dvr_syev_batch_noalloc.argtypes = [C.c_int32, I, D, I, D, D]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
@@ -270,17 +272,20 @@ def run_blocks(ctxs):
         raise DvrError(rc, "dvr_run_blocks failed")
 
 
-def syev_batch(mats, nvec=None, vectors=True):
-    """Lowest eigenpairs of several symmetric matrices (lower triangle read)."""
+def syev_batch(mats, nvec=None, vectors=True, noalloc=False):
+    """Lowest eigenpairs of several symmetric matrices (lower triangle read);
+    noalloc: through dvr_syev_batch_noalloc (orders <= 128, static device
+    workspace)."""
     lib = load()
     n = np.array([m.shape[0] for m in mats], dtype=np.int32)
     nv = n.copy() if nvec is None else np.array(nvec, dtype=np.int32)
     a = np.concatenate([np.asfortranarray(m, dtype=np.float64).ravel(order="F") for m in mats])
     w = np.zeros(int(nv.sum()))
     z = np.zeros(int((n.astype(np.int64) * nv).sum())) if vectors else None
-    rc = lib.dvr_syev_batch(len(mats), _ip(n), _dp(a), _ip(nv), _dp(w), _dp(z))
+    fn = lib.dvr_syev_batch_noalloc if noalloc else lib.dvr_syev_batch
+    rc = fn(len(mats), _ip(n), _dp(a), _ip(nv), _dp(w), _dp(z))
     if rc != DVR_OK:
-        raise DvrError(rc, "dvr_syev_batch failed")
+        raise DvrError(rc, "dvr_syev_batch%s failed" % ("_noalloc" if noalloc else ""))
     ws, zs, wo, zo = [], [], 0, 0
     for p in range(len(mats)):
         ws.append(w[wo:wo + nv[p]])

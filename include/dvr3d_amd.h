@@ -228,6 +228,21 @@ int dvr_diag(double* ham, int32_t maxham, int32_t nham, double* eig);
 int dvr_syev_batch(int32_t nprob, const int32_t* n, const double* a,
                    const int32_t* nvec, double* w, double* z);
 
+/*
+ * dvr_syev_batch without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight solve loops, harnesses that forbid hipMalloc.
+ * Same arguments and results as dvr_syev_batch; every order n[p] must be
+ * <= 128 (the problems are solved in LDS), and the device workspace of the
+ * call, sum over runs of consecutive problems of one order n with up to m
+ * wanted vectors of nb (7 n^2 + n + n m) doubles, must fit in a static array
+ * of 4 Mi doubles (32 MiB) that each GPU holds until the process ends;
+ * larger requests return DVR_ERR_ARG.  Runs on the null stream of the
+ * current device.  Calls that overlap on one GPU share that array and race.
+ */
+int dvr_syev_batch_noalloc(int32_t nprob, const int32_t* n, const double* a,
+                           const int32_t* nvec, double* w, double* z);
+
 #ifdef __cplusplus
 }
 #endif
