@@ -38,7 +38,7 @@ MLP_NPARAM = 17537
 PREC_SHIFT_INVERT, PREC_DIAGONAL, PREC_LEARNED = 0, 1, 2
 SOLVER_STATS = ("status", "iterations", "matvecs", "solves", "factorizations", "slices",
                 "certified", "missing", "dense_fallback", "rr_us", "si_us", "expand_us",
-                "factor_us", "start_us")
+                "factor_us", "start_us", "twostage")
 
 
 class Dims(C.Structure):

@@ -44,6 +44,9 @@ struct dvr_ctx {
   DevBuf H2, sdesc2, Z2;
   DevBuf cint, blk3, e2sel, cblk, sdesc3, H3, Z3, psi;
   EigWork w2, w3;
+  TwoStage ts;
+  DevBuf H3b;                 // copy of H3 reduced by the two-stage path
+  bool twostage_used = false;
   DavidsonSolver dav;
   int dav_status = -1;
   std::string dav_error;   // message of an exception inside the Davidson solver
@@ -470,10 +473,18 @@ void run_pipeline(dvr_ctx_t* C) {
                            hipMemcpyDeviceToDevice, s));
     C->h_kept = true;
   }
+  C->twostage_used = false;
   if (dense) {
     const int want = std::min(n3, meval + 16);
-    C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
-    C->w3.values(std::vector<int>{want}, s);
+    // eigenvalues only: two-stage reduction (band reduction on MFMA, bulge
+    // chasing) of a copy of H3; with eigenvectors, or for small N3, the
+    // one-stage reduction whose reflectors the back-transformation uses
+    if (nvec == 0 && n3 >= 4 * twostage_bandwidth() && env_int("DVR_TWOSTAGE", 1)) {
+      C->twostage_used = twostage_values(C->H3.as<double>(), n3, want, s, C->ts, C->w3, C->H3b);
+    } else {
+      C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
+      C->w3.values(std::vector<int>{want}, s);
+    }
     C->h_eval.assign(C->w3.hval.begin(), C->w3.hval.begin() + meval);
   }
   DVR_HIP(hipEventRecord(C->ev[6], s));
@@ -755,15 +766,15 @@ int dvr_get_vectors(const dvr_ctx_t* c, double* vec) {
 int dvr_get_solver_stats(const dvr_ctx_t* c, int32_t* st, int32_t n) {
   if (!c || !c->ran || !st || n < 1) return DVR_ERR_ARG;
   const DavStats& d = c->dav.stats;
-  const int32_t v[14] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
+  const int32_t v[15] = {c->dav_status, d.iterations, d.matvecs, d.solves, d.factorizations,
                          d.slices, d.certified, d.missing,
                          c->dims.solver == DVR_SOLVER_DAVIDSON && c->dav_status != 0,
                          (int32_t)std::lround(d.phase_ms[0] * 1e3),
                          (int32_t)std::lround(d.phase_ms[1] * 1e3),
                          (int32_t)std::lround(d.phase_ms[2] * 1e3),
                          (int32_t)std::lround(d.phase_ms[3] * 1e3),
-                         (int32_t)std::lround(d.phase_ms[4] * 1e3)};
-  for (int i = 0; i < n && i < 14; ++i) st[i] = v[i];
+                         (int32_t)std::lround(d.phase_ms[4] * 1e3), c->twostage_used ? 1 : 0};
+  for (int i = 0; i < n && i < 15; ++i) st[i] = v[i];
   return DVR_OK;
 }
 
@@ -987,10 +998,30 @@ int dvr_syev_batch(int32_t nprob, const int32_t* n, const double* a,
     }
     DVR_HIP(hipMemcpyAsync(A.p, stage.data(), sizeof(double) * tot, hipMemcpyHostToDevice, s));
     EigWork W;
+    int nmx = 0, nmn = 1 << 30;
+    for (int x : hn) {
+      nmx = std::max(nmx, x);
+      nmn = std::min(nmn, x);
+    }
+    // eigenvalues only, every problem of order >= 256: two-stage reduction
+    // of each problem in turn
+    if (!z && nmn >= 4 * twostage_bandwidth() && env_int("DVR_TWOSTAGE", 1)) {
+      TwoStage ts;
+      DevBuf cp;
+      long wo = 0;
+      for (int p = 0; p < nprob; ++p) {
+        const int want = std::min(hn[p], cntv[p] + (cntv[p] < hn[p] ? 16 : 0));
+        EigWork Wp;
+        twostage_values(A.as<double>() + aoff[p], hn[p], want, s, ts, Wp, cp);
+        std::memcpy(w + wo, Wp.hval.data(), sizeof(double) * cntv[p]);
+        wo += cntv[p];
+      }
+      DVR_HIP(hipStreamSynchronize(s));
+      (void)hipStreamDestroy(s);
+      return DVR_OK;
+    }
     // DVR_SYEV=wg selects the single-work-group solver, =multi the
-    // multi-launch one; by default problems up to kWgMaxN use the former
-    int nmx = 0;
-    for (int x : hn) nmx = std::max(nmx, x);
+    // multi-launch one
     const char* senv = getenv("DVR_SYEV");
     const bool use_wg = senv ? std::string(senv) == "wg" : false;
     W.tridiag(hn, A.as<double>(), aoff, s, use_wg);

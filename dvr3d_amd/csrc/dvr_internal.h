@@ -328,10 +328,12 @@ struct EigWork {
   DevBuf soffb;
 
   // Reduce A_p (at A + aoff[p], n_p x n_p) to tridiagonal form.  With the
-  // single-work-group solver this only sets the batch up; values() does the
-  // work.
+  // single-work-group solver, or with reduce = false (d and e are then
+  // filled by the caller, who calls bounds() afterwards), this only sets the
+  // batch up.
   void tridiag(const std::vector<int>& n_, double* A,
-               const std::vector<long>& aoff_, hipStream_t s, bool use_wg = false) {
+               const std::vector<long>& aoff_, hipStream_t s, bool use_wg = false,
+               bool reduce = true) {
     wg = use_wg;
     hn = n_;
     const int np = (int)hn.size();
@@ -364,11 +366,13 @@ struct EigWork {
     b.gu = gu.as<double>();
     b.tnrm = tn.as<double>();
     b.pivmin = piv.as<double>();
-    if (!wg) {
+    if (!wg && reduce) {
       launch_tridiag(s, b, hn.data());
       launch_tridiag_bounds(s, b);
     }
   }
+  // Gershgorin bounds and e^2 of a tridiagonal filled by the caller
+  void bounds(hipStream_t s) { launch_tridiag_bounds(s, b); }
 
   // Lowest cnt[p] eigenvalues of every problem -> hval (synchronises).
   void values(const std::vector<int>& cnt, hipStream_t s) {
@@ -468,6 +472,25 @@ struct EigWork {
     DVR_HIP(hipStreamSynchronize(s));
   }
 };
+
+// ---------------------------------------------------------------------------
+// Two-stage tridiagonalisation (k_twostage.hip): band reduction on MFMA
+// (bandwidth 64), then bulge chasing
+// ---------------------------------------------------------------------------
+struct TwoStage {
+  DevBuf work, vyv, wb, qa, qb, G, dl, Linv, sc, M, Uinv, Rb, Zs, Mm, Sv, Tst, zero, flags, taul;
+  DevBuf AB, vs, taus;
+  // A (n x n, ld lda, both triangles) is destroyed; d[n], e[n-1] on the
+  // device.  Returns 0, or 1 when a panel was rank deficient.
+  int reduce(double* A, long lda, int n, double* d, double* e, hipStream_t s);
+};
+int twostage_bandwidth();
+// Lowest `want` eigenvalues of the symmetric A (device, n x n, ld n, both
+// triangles; left intact) through the two-stage reduction of a copy in
+// `copy`, with the one-stage reduction of A as fallback; the values end in
+// W.hval (W.heoff[0] = 0).  Returns true when the two-stage path was used.
+bool twostage_values(const double* A, int n, int want, hipStream_t s, TwoStage& ts, EigWork& W,
+                     DevBuf& copy);
 
 // ---------------------------------------------------------------------------
 // General FP64 GEMM on MFMA (k_gemm.hip)

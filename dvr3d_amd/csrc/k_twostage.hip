@@ -1,0 +1,617 @@
+// Two-stage reduction of a dense symmetric matrix to tridiagonal form, FP64,
+// for the plain final solve (diag3d -> diag -> dsyev, source/
+// dvr3drjz_segmented.f90:2661-2764, 2638-2659; dsyev reduces with DSYTRD,
+// source/dsyev.f:6209, one stage, BLAS-2 at the reference's LWORK = 3N).
+//
+// Stage 1 (band reduction, bandwidth BW = 64), per panel of BW columns:
+//   * the panel below the band, P (m x BW), is factored P = Q R by shifted
+//     Cholesky QR three times (Gram matrix on MFMA, LDL^T of the 64 x 64
+//     Gram block, Q = P M^T), which needs no column-by-column sweep;
+//   * Householder reconstruction (modified LU of Q1 - S, Q1 the top block,
+//     S = diag(+-1)) turns Q into the compact WY form H = I - V T V^T with
+//     H(:, 1:BW) = Q S, so that H^T P = [S R; 0];
+//   * the trailing matrix A22 <- H^T A22 H as X = A22 V T, M = T^T V^T X,
+//     Y = X - V M / 2, A22 -= V Y^T + Y V^T: one GEMM of width BW for X and
+//     one lower-tile rank-2BW GEMM, both on FP64 MFMA; the upper tiles are
+//     then mirrored for the next panel's product.
+//   All of it is GEMM-shaped: 4/3 n^3 flops on MFMA instead of the one-stage
+//   reduction's 4/3 n^3 flops of matrix-vector products.
+// Stage 2 (band to tridiagonal) chases the bulges with Householder
+// reflections of length BW: sweep s annihilates column s below the
+// subdiagonal and pushes the fill-in down the band in tasks of one BW block
+// each.  Task k of sweep s runs in launch 2 s + k: tasks in one launch touch
+// disjoint blocks (sweep s - 1 is two blocks ahead), so every launch runs all
+// of its tasks at once, one work-group each, with the two blocks in LDS.
+// A panel whose Cholesky QR breaks down (a rank-deficient panel) is reported,
+// and the caller then uses the one-stage reduction.
+#include "dvr_internal.h"
+
+#include <cmath>
+
+namespace dvr {
+
+namespace {
+
+constexpr int BW = 64;          // bandwidth of the first stage
+constexpr int LDB = 2 * BW;     // rows of the band storage (diagonals 0 .. 2 BW - 1)
+constexpr int LAG = 2;          // launches between consecutive sweeps of stage 2
+constexpr int LD = BW + 1;      // LDS leading dimension of 64 x 64 blocks
+
+__device__ inline double wsum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
+// block sum over 256 threads (red: 4 doubles of LDS)
+__device__ inline double bsum(double v, double* red) {
+  v = wsum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1 kernels
+// ---------------------------------------------------------------------------
+
+// After gram_prep + LDL^T of the Gram block (d, Linv, column scales sc):
+// M = diag(d)^-1/2 Linv diag(sc), so that Q = Z M^T has orthonormal columns.
+// flags[0] |= 1 when a pivot is below 1e-12 on a non-zero column or when
+// only some columns are zero (a rank-deficient panel); on the first pass an
+// all-zero panel sets zero[0] = 1 (nothing to reduce) instead.
+__global__ void __launch_bounds__(256)
+k_cholqr_m(const double* __restrict__ d, const double* __restrict__ Linv,
+           const double* __restrict__ sc, double* __restrict__ M, int* __restrict__ flags,
+           int* __restrict__ zero, int first) {
+  __shared__ double r[BW];
+  __shared__ int nz, bad;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    nz = 0;
+    bad = 0;
+  }
+  __syncthreads();
+  if (t < BW) {
+    const bool z = !(sc[t] > 0.0);
+    if (z) atomicAdd(&nz, 1);
+    if (!z && !(d[t] > 1e-12)) atomicAdd(&bad, 1);
+    r[t] = d[t] > 0.0 ? 1.0 / sqrt(d[t]) : 0.0;
+  }
+  __syncthreads();
+  const bool allzero = nz == BW;
+  if (t == 0) {
+    if (first) zero[0] = allzero ? 1 : 0;
+    if (!allzero && (nz > 0 || bad > 0)) flags[0] |= 1;
+  }
+  for (int idx = t; idx < BW * BW; idx += 256) {
+    const int i = idx % BW, j = idx / BW;
+    M[idx] = allzero ? 0.0 : r[i] * Linv[idx] * sc[j];
+  }
+}
+
+// Householder reconstruction from the top block Q1 (ld ldq) of an
+// orthonormal Q: modified LU Q1 - S = L1 U without pivoting, s_i = -sign of
+// the pivot (so every pivot has modulus >= 1), then
+//   V(1:BW, :) = L1 (unit lower), Uinv = U^-1, T = -U S L1^-T, Sv = s.
+// The rows below (V2 = Q2 U^-1) are one GEMM by the caller.  A zero panel
+// gets T = 0 (H = I), Sv = 1.
+__global__ void __launch_bounds__(256)
+k_house_rec(const double* __restrict__ Q, long ldq, double* __restrict__ V, long ldv,
+            double* __restrict__ Uinv, double* __restrict__ T, double* __restrict__ Sv,
+            const int* __restrict__ zero) {
+  extern __shared__ double sm[];
+  double* W = sm;                 // LU in place
+  double* X = W + BW * LD;        // U^-1
+  double* Y = X + BW * LD;        // L1^-1
+  double* s = Y + BW * LD;        // signs
+  const int t = threadIdx.x;
+  const bool zp = zero[0] != 0;
+  for (int idx = t; idx < BW * BW; idx += 256) {
+    const int i = idx % BW, j = idx / BW;
+    W[i + j * LD] = Q[i + (long)j * ldq];
+  }
+  __syncthreads();
+  if (zp) {
+    for (int idx = t; idx < BW * BW; idx += 256) {
+      const int i = idx % BW, j = idx / BW;
+      T[idx] = 0.0;
+      Uinv[idx] = i == j ? 1.0 : 0.0;
+      V[i + (long)j * ldv] = i == j ? 1.0 : 0.0;
+    }
+    if (t < BW) Sv[t] = 1.0;
+    return;
+  }
+  for (int i = 0; i < BW; ++i) {
+    if (t == 0) {
+      const double p = W[i + i * LD];
+      const double si = p >= 0.0 ? -1.0 : 1.0;
+      s[i] = si;
+      W[i + i * LD] = p - si;
+    }
+    __syncthreads();
+    const double piv = W[i + i * LD];
+    if (t > i && t < BW) W[t + i * LD] /= piv;
+    __syncthreads();
+    const int nr = BW - 1 - i;
+    for (int idx = t; idx < nr * nr; idx += 256) {
+      const int k = i + 1 + idx % nr, j = i + 1 + idx / nr;
+      W[k + j * LD] -= W[k + i * LD] * W[i + j * LD];
+    }
+    __syncthreads();
+  }
+  // U^-1 (column j by thread j) and L1^-1 (column j by thread 64 + j)
+  if (t < BW) {
+    const int j = t;
+    for (int i = BW - 1; i > j; --i) X[i + j * LD] = 0.0;
+    X[j + j * LD] = 1.0 / W[j + j * LD];
+    for (int i = j - 1; i >= 0; --i) {
+      double a = 0.0;
+      for (int k = i + 1; k <= j; ++k) a += W[i + k * LD] * X[k + j * LD];
+      X[i + j * LD] = -a / W[i + i * LD];
+    }
+  } else if (t < 2 * BW) {
+    const int j = t - BW;
+    for (int i = 0; i < j; ++i) Y[i + j * LD] = 0.0;
+    Y[j + j * LD] = 1.0;
+    for (int i = j + 1; i < BW; ++i) {
+      double a = 0.0;
+      for (int k = j; k < i; ++k) a += W[i + k * LD] * Y[k + j * LD];
+      Y[i + j * LD] = -a;
+    }
+  }
+  __syncthreads();
+  // T = -U S L1^-T: T(i, j) = -sum_{k >= i} U(i, k) s_k L1inv(j, k), upper
+  for (int idx = t; idx < BW * BW; idx += 256) {
+    const int i = idx % BW, j = idx / BW;
+    double a = 0.0;
+    if (i <= j)
+      for (int k = i; k <= j; ++k) a += W[i + k * LD] * s[k] * Y[j + k * LD];
+    T[idx] = -a;
+    Uinv[idx] = X[i + j * LD];
+    V[i + (long)j * ldv] = i > j ? W[i + j * LD] : (i == j ? 1.0 : 0.0);
+  }
+  if (t < BW) Sv[t] = s[t];
+}
+
+// The reduced panel into A: rows j0+BW .. j0+BW+m of columns j0 .. j0+BW:
+// the band block S R (upper triangle of the top BW rows) and, for the
+// back-transformation, the Householder vectors V below it (strict lower
+// part of the top block and all rows below).
+__global__ void __launch_bounds__(256)
+k_panel_store(double* __restrict__ A, long lda, int j0, int m, const double* __restrict__ R,
+              const double* __restrict__ Sv, const double* __restrict__ V) {
+  const long n = (long)m * BW;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) {
+    const int i = (int)(t % m), c = (int)(t / m);
+    double v;
+    if (i < BW && i <= c) v = Sv[i] * R[i + c * BW];
+    else v = V[i + (long)m * c];
+    A[(j0 + BW + i) + (long)(j0 + c) * lda] = v;
+  }
+}
+
+// Upper tiles (64 x 64, strictly above the diagonal) of the m x m block at A
+// from the transposed lower tiles, through LDS.  Tile pair q enumerates
+// (I, J), I < J, column-wise.
+__global__ void __launch_bounds__(256)
+k_mirror_tiles(double* __restrict__ A, long lda, int m) {
+  __shared__ double tl[BW][BW + 1];
+  // decode q -> (I, J), J >= 1, I < J: q = J (J - 1) / 2 + I
+  const long q = blockIdx.x;
+  int J = (int)((1.0 + sqrt(1.0 + 8.0 * (double)q)) * 0.5);
+  while ((long)J * (J - 1) / 2 > q) --J;
+  while ((long)(J + 1) * J / 2 <= q) ++J;
+  const int I = (int)(q - (long)J * (J - 1) / 2);
+  const int r0 = J * BW, c0 = I * BW;       // source tile (J, I): rows r0.., cols c0..
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int c = ty; c < BW; c += 4) {
+    const int gi = r0 + tx, gj = c0 + c;
+    tl[tx][c] = (gi < m && gj < m) ? A[gi + (long)gj * lda] : 0.0;
+  }
+  __syncthreads();
+  // destination (I, J): element (c0 + a, r0 + b) = source (r0 + b, c0 + a)
+  for (int bcol = ty; bcol < BW; bcol += 4) {
+    const int gi = c0 + tx, gj = r0 + bcol;
+    if (gi < m && gj < m) A[gi + (long)gj * lda] = tl[bcol][tx];
+  }
+}
+
+// Last panel when fewer than BW rows remain below it (0 < m < BW): one
+// work-group, Householder reflections in LDS column by column (the panel P,
+// m x BW, becomes upper trapezoidal) and the m x m trailing block updated
+// from both sides with each reflection.  The reflectors stay below the
+// diagonal of P, their factors in tau.
+__global__ void __launch_bounds__(256)
+k_last_panel(double* __restrict__ A, long lda, int j0, int m, double* __restrict__ tau) {
+  extern __shared__ double lp[];
+  double (*P)[BW + 1] = reinterpret_cast<double (*)[BW + 1]>(lp);            // P[row][col]
+  double (*C)[BW + 1] = reinterpret_cast<double (*)[BW + 1]>(lp + BW * LD);  // trailing, full
+  double* v = lp + 2 * BW * LD;
+  double* w = v + BW;
+  double* red = w + BW;
+  double* sc = red + 4;
+  const int t = threadIdx.x;
+  const int r0 = j0 + BW;
+  for (int idx = t; idx < m * BW; idx += 256) {
+    const int i = idx % m, c = idx / m;
+    P[i][c] = A[(r0 + i) + (long)(j0 + c) * lda];
+  }
+  for (int idx = t; idx < m * m; idx += 256) {
+    const int i = idx % m, j = idx / m;
+    const int a = i >= j ? i : j, b = i >= j ? j : i;
+    C[i][j] = A[(r0 + a) + (long)(r0 + b) * lda];
+  }
+  __syncthreads();
+  for (int c = 0; c + 1 < m; ++c) {
+    double part = 0.0;
+    for (int i = c + 1 + t; i < m; i += 256) part += P[i][c] * P[i][c];
+    const double sigma = bsum(part, red);
+    if (t == 0) {
+      const double alpha = P[c][c];
+      double tv = 0.0, beta = alpha, scale = 0.0;
+      if (sigma > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        tv = (beta - alpha) / beta;
+        scale = 1.0 / (alpha - beta);
+      }
+      sc[0] = tv;
+      sc[1] = scale;
+      P[c][c] = beta;
+      tau[c] = tv;
+    }
+    __syncthreads();
+    const double tv = sc[0];
+    // v over all m rows of the trailing block, zero above row c
+    if (t < m) v[t] = t < c ? 0.0 : (t == c ? 1.0 : P[t][c] * sc[1]);
+    __syncthreads();
+    if (t > c && t < m) P[t][c] = v[t];
+    // left application to the remaining panel columns
+    for (int cc = c + 1 + t; cc < BW; cc += 256) {
+      double a = 0.0;
+      for (int i = c; i < m; ++i) a += v[i] * P[i][cc];
+      a *= tv;
+      for (int i = c; i < m; ++i) P[i][cc] -= a * v[i];
+    }
+    // both sides of the whole trailing block (its rows above c are coupled
+    // to the rows the reflection mixes)
+    if (t < m) {
+      double a = 0.0;
+      for (int j = c; j < m; ++j) a += C[t][j] * v[j];
+      w[t] = tv * a;
+    }
+    __syncthreads();
+    double kp = t < m ? w[t] * v[t] : 0.0;
+    const double K = bsum(kp, red);
+    const double al = -0.5 * tv * K;
+    if (t < m) w[t] += al * v[t];
+    __syncthreads();
+    for (int idx = t; idx < m * m; idx += 256) {
+      const int i = idx % m, j = idx / m;
+      C[i][j] -= v[i] * w[j] + w[i] * v[j];
+    }
+    __syncthreads();
+  }
+  for (int idx = t; idx < m * BW; idx += 256) {
+    const int i = idx % m, c = idx / m;
+    A[(r0 + i) + (long)(j0 + c) * lda] = P[i][c];
+  }
+  for (int idx = t; idx < m * m; idx += 256) {
+    const int i = idx % m, j = idx / m;
+    if (i >= j) A[(r0 + i) + (long)(r0 + j) * lda] = C[i][j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2 kernels
+// ---------------------------------------------------------------------------
+
+// Band storage AB[d + j LDB] = A(j + d, j), d < LDB (zero beyond the band).
+__global__ void __launch_bounds__(256)
+k_band_extract(const double* __restrict__ A, long lda, int n, double* __restrict__ AB) {
+  const long tot = (long)n * LDB;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < tot; t += (long)gridDim.x * 256) {
+    const int dd = (int)(t % LDB), j = (int)(t / LDB);
+    AB[t] = (dd <= BW && j + dd < n) ? A[(j + dd) + (long)j * lda] : 0.0;
+  }
+}
+
+// Householder vector of x (length len, in LDS) in place: x <- v (v0 = 1),
+// returns tau (0 when x(1:) = 0) and beta (the new leading entry) in out.
+__device__ inline void house_lds(double* x, int len, double* red, double* out) {
+  const int t = threadIdx.x;
+  double part = (t >= 1 && t < len) ? x[t] * x[t] : 0.0;
+  const double sigma = bsum(part, red);
+  const double alpha = x[0];
+  double tv = 0.0, beta = alpha, scale = 0.0;
+  if (sigma > 0.0) {
+    beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+    tv = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  __syncthreads();
+  if (t == 0) x[0] = 1.0;
+  else if (t < len) x[t] *= scale;
+  if (t == 0) {
+    out[0] = tv;
+    out[1] = beta;
+  }
+  __syncthreads();
+}
+
+// D <- H D H for the bb x bb symmetric block D (LDS, full, ld LD) with
+// H = I - tau v v^T.
+__device__ inline void two_sided(double* D, const double* v, double tau, int bb, double* w,
+                                 double* red) {
+  const int t = threadIdx.x;
+  // w = tau D v: four threads per row
+  const int row = t >> 2, part = t & 3;
+  double a = 0.0;
+  if (row < bb)
+    for (int j = part; j < bb; j += 4) a += D[row + j * LD] * v[j];
+  a += __shfl_down(a, 2, 4);
+  a += __shfl_down(a, 1, 4);
+  if (part == 0 && row < bb) w[row] = tau * a;
+  __syncthreads();
+  const double K = bsum(t < bb ? w[t] * v[t] : 0.0, red);
+  const double al = -0.5 * tau * K;
+  if (t < bb) w[t] += al * v[t];
+  __syncthreads();
+  for (int idx = t; idx < bb * bb; idx += 256) {
+    const int i = idx % bb, j = idx / bb;
+    D[i + j * LD] -= v[i] * w[j] + w[i] * v[j];
+  }
+  __syncthreads();
+}
+
+// One launch of the bulge chase: work-group i runs task k = t - LAG s of
+// sweep s = s0 + i.  Vs[s BW ...], taus[s]: the reflector of the sweep's
+// last task.
+__global__ void __launch_bounds__(256)
+k_sb2st_step(double* __restrict__ AB, int n, int t, int s0, double* __restrict__ Vs,
+             double* __restrict__ taus) {
+  extern __shared__ double sm[];
+  double* D = sm;                // BW x BW (ld LD), full symmetric
+  double* O = D + BW * LD;       // off-diagonal block, bb x BW (ld LD)
+  double* v = O + BW * LD;
+  double* vp = v + BW;
+  double* w = vp + BW;
+  double* red = w + BW;          // 4
+  double* hv = red + 4;          // tau, beta
+  const int tid = threadIdx.x;
+  const int s = s0 + blockIdx.x;
+  const int k = t - LAG * s;
+  if (s > n - 3 || k < 0) return;
+  const int r0 = s + 1 + k * BW;
+  if (r0 > n - 1) return;
+  const int bb = min(BW, n - r0);
+  double tau;
+  if (k == 0) {
+    // annihilate column s below the subdiagonal
+    if (tid < bb) v[tid] = AB[(1 + tid) + (long)s * LDB];
+    __syncthreads();
+    house_lds(v, bb, red, hv);
+    tau = hv[0];
+    if (tid < bb) AB[(1 + tid) + (long)s * LDB] = tid == 0 ? hv[1] : 0.0;
+  } else {
+    const int c0 = r0 - BW;
+    // O(r, c) = A(r0 + r, c0 + c), diagonal BW + r - c of column c0 + c
+    for (int idx = tid; idx < bb * BW; idx += 256) {
+      const int r = idx % bb, c = idx / bb;
+      O[r + c * LD] = AB[(BW + r - c) + (long)(c0 + c) * LDB];
+    }
+    if (tid < BW) vp[tid] = Vs[(long)s * BW + tid];
+    __syncthreads();
+    const double taup = taus[s];
+    // O <- O (I - taup vp vp^T)
+    {
+      const int row = tid >> 2, part = tid & 3;
+      double a = 0.0;
+      if (row < bb)
+        for (int c = part; c < BW; c += 4) a += O[row + c * LD] * vp[c];
+      a += __shfl_down(a, 2, 4);
+      a += __shfl_down(a, 1, 4);
+      if (part == 0 && row < bb) w[row] = taup * a;
+      __syncthreads();
+      for (int idx = tid; idx < bb * BW; idx += 256) {
+        const int r = idx % bb, c = idx / bb;
+        O[r + c * LD] -= w[r] * vp[c];
+      }
+      __syncthreads();
+    }
+    // reflector of column 0 of O
+    if (tid < bb) v[tid] = O[tid];
+    __syncthreads();
+    house_lds(v, bb, red, hv);
+    tau = hv[0];
+    // O(:, 1:) <- (I - tau v v^T) O(:, 1:)
+    if (tid >= 1 && tid < BW) {
+      double a = 0.0;
+      for (int r = 0; r < bb; ++r) a += v[r] * O[r + tid * LD];
+      w[tid] = tau * a;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < bb * BW; idx += 256) {
+      const int r = idx % bb, c = idx / bb;
+      if (c == 0) O[r] = r == 0 ? hv[1] : 0.0;
+      else O[r + c * LD] -= v[r] * w[c];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < bb * BW; idx += 256) {
+      const int r = idx % bb, c = idx / bb;
+      AB[(BW + r - c) + (long)(c0 + c) * LDB] = O[r + c * LD];
+    }
+  }
+  // the diagonal block rows r0 .. r0 + bb, both sides
+  for (int idx = tid; idx < bb * bb; idx += 256) {
+    const int i = idx % bb, j = idx / bb;
+    D[i + j * LD] = i >= j ? AB[(i - j) + (long)(r0 + j) * LDB] : AB[(j - i) + (long)(r0 + i) * LDB];
+  }
+  __syncthreads();
+  two_sided(D, v, tau, bb, w, red);
+  for (int idx = tid; idx < bb * bb; idx += 256) {
+    const int i = idx % bb, j = idx / bb;
+    if (i >= j) AB[(i - j) + (long)(r0 + j) * LDB] = D[i + j * LD];
+  }
+  if (tid < BW) Vs[(long)s * BW + tid] = tid < bb ? v[tid] : 0.0;
+  if (tid == 0) taus[s] = tau;
+}
+
+__global__ void __launch_bounds__(256)
+k_band_to_tri(const double* __restrict__ AB, int n, double* __restrict__ d, double* __restrict__ e) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    d[i] = AB[(long)i * LDB];
+    if (i + 1 < n) e[i] = AB[1 + (long)i * LDB];
+  }
+}
+
+int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
+
+size_t house_rec_lds() { return sizeof(double) * (3 * BW * LD + BW); }
+size_t last_panel_lds() { return sizeof(double) * (2 * BW * LD + 2 * BW + 6); }
+size_t sb2st_lds() { return sizeof(double) * (2 * BW * LD + 3 * BW + 4 + 2); }
+
+}  // namespace
+
+int twostage_bandwidth() { return BW; }
+
+// Reduce A (n x n, ld lda, both triangles valid; destroyed) to tridiagonal
+// form (d[n], e[n-1] on the device).  Returns 0, or 1 when a panel was rank
+// deficient (the caller then reduces a copy of A in one stage).
+int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream_t s) {
+  if (n < 4 * BW) throw HipError("two-stage reduction: n < 256");
+  const int m0 = n - BW;
+  // workspace for the split-K GEMMs: the Gram-type products (BW x BW, K = m)
+  // and the product A22 W (m x BW, K = m) at the first, largest panel
+  const int ks_g = gemm_auto_ksplit(BW, BW, m0);
+  const int ks_x = gemm_auto_ksplit(m0, BW, m0);
+  work.ensure(sizeof(double) * std::max<size_t>(
+      {gemm_work_doubles(BW, BW, ks_g), gemm_work_doubles(m0, BW, ks_x), (size_t)1}));
+  vyv.ensure(sizeof(double) * (size_t)m0 * 3 * BW);
+  wb.ensure(sizeof(double) * (size_t)m0 * BW);
+  qa.ensure(sizeof(double) * (size_t)m0 * BW);
+  qb.ensure(sizeof(double) * (size_t)m0 * BW);
+  const int npan = (n + BW - 1) / BW;
+  for (DevBuf* x : {&G, &dl, &Linv, &sc, &M, &Uinv, &Rb, &Zs, &Mm})
+    x->ensure(sizeof(double) * BW * BW);
+  Sv.ensure(sizeof(double) * BW);
+  Tst.ensure(sizeof(double) * (size_t)npan * BW * BW);
+  zero.ensure(sizeof(int) * npan);
+  flags.ensure(sizeof(int));
+  taul.ensure(sizeof(double) * BW);
+  DVR_HIP(hipMemsetAsync(flags.p, 0, sizeof(int), s));
+  DVR_HIP(hipFuncSetAttribute((const void*)k_house_rec,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)house_rec_lds()));
+  DVR_HIP(hipFuncSetAttribute((const void*)k_last_panel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)last_panel_lds()));
+  DVR_HIP(hipFuncSetAttribute((const void*)k_sb2st_step,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sb2st_lds()));
+  double* W = work.as<double>();
+  auto gemm = [&](const char* name, bool ta, bool tb, int M_, int N_, int K_, double alpha,
+                  const double* A_, long lda_, const double* B_, long ldb_, double beta, double* C_,
+                  long ldc_) {
+    const int ks = gemm_auto_ksplit(M_, N_, K_);
+    launch_gemm(s, name, ta, tb, M_, N_, K_, alpha, A_, lda_, B_, ldb_, beta, C_, ldc_, ks,
+                ks > 1 ? W : nullptr, false);
+  };
+  const double eps = 2.220446049250313e-16;
+  int p = 0;
+  int j0 = 0;
+  for (; n - j0 - BW >= BW; j0 += BW, ++p) {
+    const int m = n - j0 - BW;
+    double* P = A + (j0 + BW) + (long)j0 * lda;
+    double* A22 = A + (j0 + BW) * (1 + lda);
+    // P = Q R: shifted Cholesky QR, then two plain passes
+    const double* Z = P;
+    long ldz = lda;
+    double* qout[3] = {qa.as<double>(), qb.as<double>(), qa.as<double>()};
+    for (int pass = 0; pass < 3; ++pass) {
+      gemm("s1_gram", true, false, BW, BW, m, 1.0, Z, ldz, Z, ldz, 0.0, G.as<double>(), BW);
+      const double shift = pass == 0 ? 11.0 * ((double)m * BW + BW * (BW + 1.0)) * eps * BW : 0.0;
+      launch_gram_prep(s, G.as<double>(), BW, shift, sc.as<double>());
+      launch_ldl_block(s, G.as<double>(), BW, BW, dl.as<double>(), Linv.as<double>());
+      DVR_LAUNCH("s1_cholqr", 8.0 * 3 * BW * BW, 0.0, k_cholqr_m, dim3(1), dim3(256), 0, s,
+                 dl.as<double>(), Linv.as<double>(), sc.as<double>(), M.as<double>(),
+                 flags.as<int>(), zero.as<int>() + p, pass == 0 ? 1 : 0);
+      gemm("s1_q", false, true, m, BW, BW, 1.0, Z, ldz, M.as<double>(), BW, 0.0, qout[pass], m);
+      Z = qout[pass];
+      ldz = m;
+    }
+    const double* Q = qa.as<double>();
+    double* V = vyv.as<double>();
+    double* X = V + (size_t)m * BW;
+    double* T = Tst.as<double>() + (size_t)p * BW * BW;
+    DVR_LAUNCH("s1_house_rec", 8.0 * 4 * BW * BW, 0.0, k_house_rec, dim3(1), dim3(256),
+               house_rec_lds(), s, Q, (long)m, V, (long)m, Uinv.as<double>(), T, Sv.as<double>(),
+               zero.as<int>() + p);
+    gemm("s1_r", true, false, BW, BW, m, 1.0, Q, m, P, lda, 0.0, Rb.as<double>(), BW);
+    gemm("s1_v2", false, false, m - BW, BW, BW, 1.0, Q + BW, m, Uinv.as<double>(), BW, 0.0, V + BW,
+         m);
+    DVR_HIP(hipMemcpyAsync(V + (size_t)2 * m * BW, V, sizeof(double) * (size_t)m * BW,
+                           hipMemcpyDeviceToDevice, s));
+    // X = A22 V T, M = T^T V^T X, Y = X - V M / 2 (in place of X)
+    gemm("s1_vt", false, false, m, BW, BW, 1.0, V, m, T, BW, 0.0, wb.as<double>(), m);
+    gemm("s1_symm", false, false, m, BW, m, 1.0, A22, lda, wb.as<double>(), m, 0.0, X, m);
+    gemm("s1_vtx", true, false, BW, BW, m, 1.0, V, m, X, m, 0.0, Zs.as<double>(), BW);
+    gemm("s1_m", true, false, BW, BW, BW, 1.0, T, BW, Zs.as<double>(), BW, 0.0, Mm.as<double>(),
+         BW);
+    gemm("s1_y", false, false, m, BW, BW, -0.5, V, m, Mm.as<double>(), BW, 1.0, X, m);
+    // A22 -= [V Y] [Y V]^T on the lower tiles, then the upper tiles mirrored
+    launch_gemm_var(s, "s1_syr2k", false, true, m, m, 2 * BW, -1.0, V, m, X, m, 1.0, A22, lda,
+                    true, 16, true, true);
+    const long nt = (m + BW - 1) / BW;
+    if (nt > 1)
+      DVR_LAUNCH("s1_mirror", 8.0 * (double)m * m, 0.0, k_mirror_tiles,
+                 dim3((unsigned)(nt * (nt - 1) / 2)), dim3(256), 0, s, A22, lda, m);
+    DVR_LAUNCH("s1_store", 16.0 * m * BW, 0.0, k_panel_store, dim3(grid_of((long)m * BW)),
+               dim3(256), 0, s, A, lda, j0, m, Rb.as<double>(), Sv.as<double>(), V);
+  }
+  const int mlast = n - j0 - BW;
+  if (mlast > 0)
+    DVR_LAUNCH("s1_last", 8.0 * 2 * (double)(mlast + BW) * (mlast + BW), 0.0, k_last_panel, dim3(1),
+               dim3(256), last_panel_lds(), s, A, lda, j0, mlast, taul.as<double>());
+  // ---- stage 2
+  AB.ensure(sizeof(double) * (size_t)n * LDB);
+  vs.ensure(sizeof(double) * (size_t)n * BW);
+  taus.ensure(sizeof(double) * n);
+  DVR_LAUNCH("s2_band", 8.0 * 2 * (double)n * LDB, 0.0, k_band_extract, dim3(grid_of((long)n * LDB)),
+             dim3(256), 0, s, A, lda, n, AB.as<double>());
+  const int nsweep = n - 2;   // sweeps 0 .. n - 3
+  auto kmax = [&](int sw) { return (n - 2 - sw) / BW; };   // last task with r0 <= n - 1
+  const int tmax = LAG * (nsweep - 1) + kmax(0);
+  int slo = 0;
+  for (int t = 0; t <= tmax; ++t) {
+    const int shi = std::min(nsweep - 1, t / LAG);
+    while (slo <= shi && t - LAG * slo > kmax(slo)) ++slo;
+    if (slo > shi) continue;
+    DVR_LAUNCH("s2_chase", 0.0, 0.0, k_sb2st_step, dim3(shi - slo + 1), dim3(256), sb2st_lds(), s,
+               AB.as<double>(), n, t, slo, vs.as<double>(), taus.as<double>());
+  }
+  DVR_LAUNCH("s2_tri", 16.0 * n, 0.0, k_band_to_tri, dim3(grid_of(n)), dim3(256), 0, s,
+             AB.as<double>(), n, d, e);
+  DVR_LAUNCH_CHECK();
+  int hf = 0;
+  DVR_HIP(hipMemcpyAsync(&hf, flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  DVR_HIP(hipStreamSynchronize(s));
+  return hf ? 1 : 0;
+}
+
+bool twostage_values(const double* A, int n, int want, hipStream_t s, TwoStage& ts, EigWork& W,
+                     DevBuf& copy) {
+  copy.ensure(sizeof(double) * (size_t)n * n);
+  DVR_HIP(hipMemcpyAsync(copy.p, A, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, s));
+  W.tridiag(std::vector<int>{n}, copy.as<double>(), std::vector<long>{0}, s, false, false);
+  if (ts.reduce(copy.as<double>(), n, n, W.b.d, W.b.e, s) == 0) {
+    W.bounds(s);
+    W.values(std::vector<int>{want}, s);
+    return true;
+  }
+  // a rank-deficient panel: the one-stage reduction of the intact A
+  DVR_HIP(hipMemcpyAsync(copy.p, A, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToDevice, s));
+  W.tridiag(std::vector<int>{n}, copy.as<double>(), std::vector<long>{0}, s);
+  W.values(std::vector<int>{want}, s);
+  return false;
+}
+
+}  // namespace dvr

@@ -151,7 +151,8 @@ int dvr_get_stage_times(const dvr_ctx_t* ctx, double* ms);
  * solves, [11] the basis expansions (orthogonalisation and H-vector
  * products), [12] the LDL^T factorisations (a factorisation run on the
  * second stream beside a slice counts only for the time the solver's stream
- * waits for it), [13] the start (diagonal, start block, first projection). */
+ * waits for it), [13] the start (diagonal, start block, first projection),
+ * [14] 1 when the dense eigenvalues came from the two-stage reduction. */
 int dvr_get_solver_stats(const dvr_ctx_t* ctx, int32_t* stats, int32_t n);
 
 /* Shift-and-invert kernel of the Davidson solver, exposed for testing and
