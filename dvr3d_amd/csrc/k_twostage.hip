@@ -317,144 +317,136 @@ k_band_extract(const double* __restrict__ A, long lda, int n, double* __restrict
   }
 }
 
-// Householder vector of x (length len, in LDS) in place: x <- v (v0 = 1),
-// returns tau (0 when x(1:) = 0) and beta (the new leading entry) in out.
-__device__ inline void house_lds(double* x, int len, double* red, double* out) {
-  const int t = threadIdx.x;
-  double part = (t >= 1 && t < len) ? x[t] * x[t] : 0.0;
-  const double sigma = bsum(part, red);
-  const double alpha = x[0];
-  double tv = 0.0, beta = alpha, scale = 0.0;
-  if (sigma > 0.0) {
-    beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-    tv = (beta - alpha) / beta;
-    scale = 1.0 / (alpha - beta);
-  }
-  __syncthreads();
-  if (t == 0) x[0] = 1.0;
-  else if (t < len) x[t] *= scale;
-  if (t == 0) {
-    out[0] = tv;
-    out[1] = beta;
-  }
-  __syncthreads();
-}
-
-// D <- H D H for the bb x bb symmetric block D (LDS, full, ld LD) with
-// H = I - tau v v^T.
-__device__ inline void two_sided(double* D, const double* v, double tau, int bb, double* w,
-                                 double* red) {
-  const int t = threadIdx.x;
-  // w = tau D v: four threads per row
-  const int row = t >> 2, part = t & 3;
-  double a = 0.0;
-  if (row < bb)
-    for (int j = part; j < bb; j += 4) a += D[row + j * LD] * v[j];
-  a += __shfl_down(a, 2, 4);
-  a += __shfl_down(a, 1, 4);
-  if (part == 0 && row < bb) w[row] = tau * a;
-  __syncthreads();
-  const double K = bsum(t < bb ? w[t] * v[t] : 0.0, red);
-  const double al = -0.5 * tau * K;
-  if (t < bb) w[t] += al * v[t];
-  __syncthreads();
-  for (int idx = t; idx < bb * bb; idx += 256) {
-    const int i = idx % bb, j = idx / bb;
-    D[i + j * LD] -= v[i] * w[j] + w[i] * v[j];
-  }
-  __syncthreads();
-}
-
 // One launch of the bulge chase: work-group i runs task k = t - LAG s of
 // sweep s = s0 + i.  Vs[s BW ...], taus[s]: the reflector of the sweep's
-// last task.
+// previous task.
+//   k = 0: reflector of column s below the subdiagonal;
+//   k > 0: the off-diagonal block O (rows r0 .., columns r0 - BW ..) gets
+//          the previous reflector from the right, then a reflector of its
+//          column 0 from the left;
+//   both: the reflector from both sides of the diagonal block at r0.
+// Each block is updated in one pass: with y = taup O vp, x = O(:, 0) - y vp0
+// (the new column 0), v, tau, beta from x, a = v.y and u = v^T O,
+//   O <- O - y vp^T - v w^T,  w_c = tau (u_c - a vp_c)   (c >= 1),
+//   D <- D - v z^T - z v^T,   z = tau D v - tau^2 (v.D v) v / 2.
+// 256 threads: thread (row r = tid / 4, quarter q = tid % 4) handles the
+// 16 columns q, q + 4, ... of row r (or the 16 rows of column r); the
+// quarter sums meet through lane shuffles.  All global loads are issued at
+// the start.
 __global__ void __launch_bounds__(256)
 k_sb2st_step(double* __restrict__ AB, int n, int t, int s0, double* __restrict__ Vs,
              double* __restrict__ taus) {
   extern __shared__ double sm[];
-  double* D = sm;                // BW x BW (ld LD), full symmetric
-  double* O = D + BW * LD;       // off-diagonal block, bb x BW (ld LD)
-  double* v = O + BW * LD;
-  double* vp = v + BW;
-  double* w = vp + BW;
-  double* red = w + BW;          // 4
-  double* hv = red + 4;          // tau, beta
-  const int tid = threadIdx.x;
+  double* O = sm;                // off-diagonal block (ld LD)
+  double* vb = O + BW * LD;      // v
+  double* pb = vb + BW;          // vp
+  double* wb = pb + BW;          // w, then z
+  double* red = wb + BW;         // 10
+  const int tid = threadIdx.x, r = tid >> 2, q = tid & 3;
   const int s = s0 + blockIdx.x;
   const int k = t - LAG * s;
   if (s > n - 3 || k < 0) return;
   const int r0 = s + 1 + k * BW;
   if (r0 > n - 1) return;
   const int bb = min(BW, n - r0);
-  double tau;
-  if (k == 0) {
-    // annihilate column s below the subdiagonal
-    if (tid < bb) v[tid] = AB[(1 + tid) + (long)s * LDB];
-    __syncthreads();
-    house_lds(v, bb, red, hv);
-    tau = hv[0];
-    if (tid < bb) AB[(1 + tid) + (long)s * LDB] = tid == 0 ? hv[1] : 0.0;
-  } else {
-    const int c0 = r0 - BW;
-    // O(r, c) = A(r0 + r, c0 + c), diagonal BW + r - c of column c0 + c
-    for (int idx = tid; idx < bb * BW; idx += 256) {
-      const int r = idx % bb, c = idx / bb;
-      O[r + c * LD] = AB[(BW + r - c) + (long)(c0 + c) * LDB];
-    }
-    if (tid < BW) vp[tid] = Vs[(long)s * BW + tid];
-    __syncthreads();
-    const double taup = taus[s];
-    // O <- O (I - taup vp vp^T)
-    {
-      const int row = tid >> 2, part = tid & 3;
-      double a = 0.0;
-      if (row < bb)
-        for (int c = part; c < BW; c += 4) a += O[row + c * LD] * vp[c];
-      a += __shfl_down(a, 2, 4);
-      a += __shfl_down(a, 1, 4);
-      if (part == 0 && row < bb) w[row] = taup * a;
-      __syncthreads();
-      for (int idx = tid; idx < bb * BW; idx += 256) {
-        const int r = idx % bb, c = idx / bb;
-        O[r + c * LD] -= w[r] * vp[c];
-      }
-      __syncthreads();
-    }
-    // reflector of column 0 of O
-    if (tid < bb) v[tid] = O[tid];
-    __syncthreads();
-    house_lds(v, bb, red, hv);
-    tau = hv[0];
-    // O(:, 1:) <- (I - tau v v^T) O(:, 1:)
-    if (tid >= 1 && tid < BW) {
-      double a = 0.0;
-      for (int r = 0; r < bb; ++r) a += v[r] * O[r + tid * LD];
-      w[tid] = tau * a;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < bb * BW; idx += 256) {
-      const int r = idx % bb, c = idx / bb;
-      if (c == 0) O[r] = r == 0 ? hv[1] : 0.0;
-      else O[r + c * LD] -= v[r] * w[c];
-    }
-    __syncthreads();
-    for (int idx = tid; idx < bb * BW; idx += 256) {
-      const int r = idx % bb, c = idx / bb;
-      AB[(BW + r - c) + (long)(c0 + c) * LDB] = O[r + c * LD];
-    }
+  const int c0 = r0 - BW;
+  const bool act = r < bb;
+  // ---- loads into registers: diagonal block (row r, both triangles from
+  // the lower band), O, vp
+  double dreg[16], oreg[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int j = q + 4 * i;
+    const int a = r >= j ? r : j, b = r >= j ? j : r;   // (a, b) lower
+    dreg[i] = (a < bb) ? AB[(a - b) + (long)(r0 + b) * LDB] : 0.0;
+    oreg[i] = (k > 0 && act) ? AB[(BW + r - j) + (long)(c0 + j) * LDB] : 0.0;
   }
-  // the diagonal block rows r0 .. r0 + bb, both sides
-  for (int idx = tid; idx < bb * bb; idx += 256) {
-    const int i = idx % bb, j = idx / bb;
-    D[i + j * LD] = i >= j ? AB[(i - j) + (long)(r0 + j) * LDB] : AB[(j - i) + (long)(r0 + i) * LDB];
+  double xcol = 0.0;
+  if (k == 0 && q == 0 && act) xcol = AB[(1 + r) + (long)s * LDB];
+  if (tid < BW) pb[tid] = k > 0 ? Vs[(long)s * BW + tid] : 0.0;
+  const double taup = k > 0 ? taus[s] : 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) O[r + (q + 4 * i) * LD] = oreg[i];
+  __syncthreads();
+  // ---- y = taup O vp (row r), x = new column 0
+  double y = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) y += oreg[i] * pb[q + 4 * i];
+  y += __shfl_xor(y, 1, 64);
+  y += __shfl_xor(y, 2, 64);
+  y *= taup;
+  const int lane = tid & 63, wv = tid >> 6;
+  const double x = k == 0 ? __shfl(xcol, lane & ~3, 64) : O[r] - y * pb[0];
+  // sigma = sum_{r >= 1} x_r^2, xi = sum_{r >= 1} x_r y_r (one thread per row)
+  double s1 = (q == 0 && act && r >= 1) ? x * x : 0.0;
+  double s2 = (q == 0 && act && r >= 1) ? x * y : 0.0;
+  s1 = wsum(s1);
+  s2 = wsum(s2);
+  if (lane == 0) {
+    red[wv] = s1;
+    red[4 + wv] = s2;
+  }
+  if (tid == 0) {
+    red[8] = y;      // y_0
+    red[9] = x;      // x_0, the leading entry of the reflected column
   }
   __syncthreads();
-  two_sided(D, v, tau, bb, w, red);
-  for (int idx = tid; idx < bb * bb; idx += 256) {
-    const int i = idx % bb, j = idx / bb;
-    if (i >= j) AB[(i - j) + (long)(r0 + j) * LDB] = D[i + j * LD];
+  const double sigma = red[0] + red[1] + red[2] + red[3];
+  const double xi = red[4] + red[5] + red[6] + red[7];
+  const double y0 = red[8], al0 = red[9];
+  double tau = 0.0, beta = al0, scale = 0.0;
+  if (sigma > 0.0) {
+    beta = -copysign(sqrt(al0 * al0 + sigma), al0);
+    tau = (beta - al0) / beta;
+    scale = 1.0 / (al0 - beta);
   }
-  if (tid < BW) Vs[(long)s * BW + tid] = tid < bb ? v[tid] : 0.0;
+  const double v = r == 0 ? 1.0 : (act ? x * scale : 0.0);
+  if (q == 0) vb[r] = v;
+  if (k == 0 && q == 0 && act) AB[(1 + r) + (long)s * LDB] = r == 0 ? beta : 0.0;
+  __syncthreads();
+  if (k > 0) {
+    const double av = y0 + scale * xi;           // v . y
+    // u_c = v^T O(:, c) for column c = r (16 rows per quarter)
+    double u = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) u += vb[q + 4 * i] * O[(q + 4 * i) + r * LD];
+    u += __shfl_xor(u, 1, 64);
+    u += __shfl_xor(u, 2, 64);
+    if (q == 0) wb[r] = tau * (u - av * pb[r]);
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int c = q + 4 * i;
+        const double val = c == 0 ? (r == 0 ? beta : 0.0) : oreg[i] - y * pb[c] - v * wb[c];
+        AB[(BW + r - c) + (long)(c0 + c) * LDB] = val;
+      }
+    }
+    __syncthreads();
+  }
+  // ---- diagonal block from both sides
+  double z = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) z += dreg[i] * vb[q + 4 * i];
+  z += __shfl_xor(z, 1, 64);
+  z += __shfl_xor(z, 2, 64);
+  z *= tau;
+  double kv = (q == 0) ? v * z : 0.0;
+  kv = wsum(kv);
+  if (lane == 0) red[wv] = kv;
+  __syncthreads();
+  const double K = red[0] + red[1] + red[2] + red[3];
+  z -= 0.5 * tau * K * v;
+  if (q == 0) wb[r] = z;
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int j = q + 4 * i;
+      if (j <= r) AB[(r - j) + (long)(r0 + j) * LDB] = dreg[i] - v * wb[j] - z * vb[j];
+    }
+  }
+  if (q == 0) Vs[(long)s * BW + r] = v;
   if (tid == 0) taus[s] = tau;
 }
 
@@ -470,7 +462,7 @@ int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
 
 size_t house_rec_lds() { return sizeof(double) * (3 * BW * LD + BW); }
 size_t last_panel_lds() { return sizeof(double) * (2 * BW * LD + 2 * BW + 6); }
-size_t sb2st_lds() { return sizeof(double) * (2 * BW * LD + 3 * BW + 4 + 2); }
+size_t sb2st_lds() { return sizeof(double) * (BW * LD + 3 * BW + 12); }
 
 }  // namespace
 
