@@ -474,12 +474,18 @@ int twostage_bandwidth() { return BW; }
 int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream_t s) {
   if (n < 4 * BW) throw HipError("two-stage reduction: n < 256");
   const int m0 = n - BW;
-  // workspace for the split-K GEMMs: the Gram-type products (BW x BW, K = m)
-  // and the product A22 W (m x BW, K = m) at the first, largest panel
-  const int ks_g = gemm_auto_ksplit(BW, BW, m0);
-  const int ks_x = gemm_auto_ksplit(m0, BW, m0);
-  work.ensure(sizeof(double) * std::max<size_t>(
-      {gemm_work_doubles(BW, BW, ks_g), gemm_work_doubles(m0, BW, ks_x), (size_t)1}));
+  // workspace of the split-K GEMMs: the largest over every panel's shapes
+  // (the K split of the product A22 W grows as m shrinks, so the first
+  // panel is not always the largest)
+  size_t wneed = 1;
+  for (int jj = 0; n - jj - BW >= BW; jj += BW) {
+    const int m = n - jj - BW;
+    const int shp[4][3] = {{BW, BW, m}, {m, BW, m}, {m, BW, BW}, {m - BW, BW, BW}};
+    for (const auto& x : shp)
+      if (x[0] > 0)
+        wneed = std::max(wneed, gemm_work_doubles(x[0], x[1], gemm_auto_ksplit(x[0], x[1], x[2])));
+  }
+  work.ensure(sizeof(double) * wneed);
   vyv.ensure(sizeof(double) * (size_t)m0 * 3 * BW);
   wb.ensure(sizeof(double) * (size_t)m0 * BW);
   qa.ensure(sizeof(double) * (size_t)m0 * BW);
@@ -504,6 +510,8 @@ int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream
                   const double* A_, long lda_, const double* B_, long ldb_, double beta, double* C_,
                   long ldc_) {
     const int ks = gemm_auto_ksplit(M_, N_, K_);
+    if (gemm_work_doubles(M_, N_, ks) > wneed)
+      throw HipError("two-stage reduction: split-K workspace too small");
     launch_gemm(s, name, ta, tb, M_, N_, K_, alpha, A_, lda_, B_, ldb_, beta, C_, ldc_, ks,
                 ks > 1 ? W : nullptr, false);
   };
