@@ -140,7 +140,7 @@ def cpu_baseline_reference(cfg: str):
     skipped and dsyev given its optimal workspace, which leaves the levels
     unchanged to 1e-9 cm^-1) on one host core each, several runs at once on
     different cores.  c0/c1: the full deck.  c3 would take over an hour, so
-    the sample is: the c3 deck with MAX3D = 500 and 1500 (1D and 2D stages in
+    the sample is: the c3 deck with MAX3D = 500 and 2500 (1D and 2D stages in
     full, the 3D assembly and final dsyev at those dimensions) and the
     reference's dsyev call alone at N = 3000; the run time at N3 = 20000 is
     extrapolated as T = T12 + a N3^2 + b N3^3 (b from the dsyev timing, a and
@@ -164,7 +164,7 @@ def cpu_baseline_reference(cfg: str):
     tmp = tempfile.mkdtemp(prefix="dvr3d_cpu_")
     jobs = []   # (name, argv, stdin text)
     if cfg == "c3":
-        jobs = [("m500", [exe], _deck("c3", 100, 500)), ("m1500", [exe], _deck("c3", 100, 1500)),
+        jobs = [("m500", [exe], _deck("c3", 100, 500)), ("m2500", [exe], _deck("c3", 100, 2500)),
                 ("dsyev3000", [dtime, "3000"], None), ("c0", [exe], _deck("c0"))]
     else:
         jobs = [(cfg, [exe], _deck(cfg))]
@@ -210,13 +210,13 @@ def cpu_baseline_reference(cfg: str):
                     sample=f"one full {cfg} run of the reference on 1 core: {t[cfg]:.2f} s")
         return info
     b = t["dsyev3000"] / 3000.0 ** 3
-    a = (t["m1500"] - t["m500"] - b * (1500.0 ** 3 - 500.0 ** 3)) / (1500.0 ** 2 - 500.0 ** 2)
+    a = max(0.0, (t["m2500"] - t["m500"] - b * (2500.0 ** 3 - 500.0 ** 3)) / (2500.0 ** 2 - 500.0 ** 2))
     t12 = t["m500"] - a * 500.0 ** 2 - b * 500.0 ** 3
     n3 = 20000.0
     total = t12 + a * n3 ** 2 + b * n3 ** 3
     info.update(value=1000.0 / total, unit="levels/s", extrapolated=True,
                 sample=f"reference on 1 core each (4 runs side by side): c3 deck with MAX3D=500 "
-                       f"{t['m500']:.1f} s, MAX3D=1500 {t['m1500']:.1f} s, its dsyev at N=3000 "
+                       f"{t['m500']:.1f} s, MAX3D=2500 {t['m2500']:.1f} s, its dsyev at N=3000 "
                        f"{t['dsyev3000']:.2f} s -> 1D+2D stages {t12:.1f} s, 3D assembly "
                        f"{a * n3 ** 2:.0f} s and dsyev {b * n3 ** 3:.0f} s at N3=20000: "
                        f"extrapolated {total:.0f} s per solve",
