@@ -234,9 +234,13 @@ def setcon(job: dict) -> Consts:
     beta1 = job["we1"] * math.sqrt(0.5 * ur1 / job["diss1"])   # D:1123
     a1 = 4.0 * job["diss1"] / job["we1"]
     iu1 = int(a1 + 0.5)
-    beta2 = job["we2"] * math.sqrt(0.5 * ur2 / job["diss2"])   # D:1147
-    a2 = 4.0 * job["diss2"] / job["we2"]
-    iu2 = int(a2 + 0.5)
+    if job["diss2"] > 0.0 and job["we2"] > 0.0:
+        beta2 = job["we2"] * math.sqrt(0.5 * ur2 / job["diss2"])   # D:1147
+        a2 = 4.0 * job["diss2"] / job["we2"]
+        iu2 = int(a2 + 0.5)
+    else:
+        # no r2 basis card (IDIA = -2 decks use the r1 basis for both bonds)
+        beta2, a2, iu2 = 0.0, 0.0, 0
     return Consts(g1, g2, ur1, ur2, urr1, urr2, beta1, beta2, a1, a2, iu1, iu2)
 
 
