@@ -29,7 +29,7 @@ EXPORTS = (
     "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_run_blocks", "dvr_get_sizes",
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
-    "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_set_profiling",
+    "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_nfmain", "dvr_set_profiling",
     "dvr_get_profile",
     "dvr_get_solver_stats", "dvr_ldl_solve", "dvr_keep_hamiltonian", "dvr_set_preconditioner",
     "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
@@ -39,6 +39,12 @@ PREC_SHIFT_INVERT, PREC_DIAGONAL, PREC_LEARNED = 0, 1, 2
 SOLVER_STATS = ("status", "iterations", "matvecs", "solves", "factorizations", "slices",
                 "certified", "missing", "dense_fallback", "rr_us", "si_us", "expand_us",
                 "factor_us", "start_us", "twostage")
+
+
+class NfDims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "npnt", "nalf", "ipar", "max3d", "neval", "zcut", "nvec", "jrot", "kz", "kmin",
+        "zrot", "zlin")]
 
 
 class Dims(C.Structure):
@@ -88,6 +94,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_diag.argtypes = [D, C.c_int32, C.c_int32, D]
     lib.dvr_syev_batch.argtypes = [C.c_int32, I, D, I, D, D]
     lib.dvr_syev_batch_noalloc.argtypes = [C.c_int32, I, D, I, D, D]
+    lib.dvr_nfmain.argtypes = [C.POINTER(NfDims), D, D, D, D, C.c_double, C.c_double, D,
+                               C.c_double, I, I, I, D, D]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
@@ -370,3 +378,30 @@ def diag(ham: np.ndarray):
     if rc != DVR_OK:
         raise DvrError(rc, "dvr_diag failed")
     return e, h
+
+
+def nfmain(dims: dict, hr, htheta, r, xalf, ur1, urr1, vgrid, emax2_au=0.0):
+    """dvr_nfmain: the symmetrised-Radau driver for one k block.  vgrid is
+    V[g, b, a] = V(r_a, r_b, x_g).  Returns a dict with iv2, nham3, iang,
+    eval and (nvec > 0) wvfunc (nvec x npair*iang)."""
+    lib = load()
+    d = NfDims(**{k: int(dims.get(k, 0)) for k, _ in NfDims._fields_})
+    npnt, nalf, ipar = d.npnt, d.nalf, d.ipar
+    npair = npnt * (npnt + 1 - 2 * ipar) // 2
+    iv2 = np.zeros(nalf, dtype=np.int32)
+    n3 = np.zeros(1, dtype=np.int32)
+    iang = np.zeros(1, dtype=np.int32)
+    ev = np.zeros(d.neval)
+    wv = np.zeros(max(1, d.nvec * npair * nalf)) if d.nvec > 0 else None
+    args = [_f64(np.asfortranarray(hr).ravel(order="F")),
+            _f64(np.asfortranarray(htheta).ravel(order="F")), _f64(r), _f64(xalf)]
+    vg = _f64(np.ascontiguousarray(vgrid).ravel())
+    rc = lib.dvr_nfmain(C.byref(d), *[_dp(x) for x in args], float(ur1), float(urr1), _dp(vg),
+                        float(emax2_au), _ip(iv2), _ip(n3), _ip(iang), _dp(ev), _dp(wv))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_nfmain failed")
+    n = int(n3[0])
+    out = dict(iv2=iv2, nham3=n, iang=int(iang[0]), eval=ev[: min(d.neval, n)])
+    if d.nvec > 0:
+        out["wvfunc"] = wv[: d.nvec * npair * int(iang[0])].reshape(d.nvec, -1)
+    return out

@@ -8,6 +8,7 @@
 #include <tuple>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <stdexcept>
 #include <vector>
@@ -485,6 +486,11 @@ struct TwoStage {
   int reduce(double* A, long lda, int n, double* d, double* e, hipStream_t s);
 };
 int twostage_bandwidth();
+// DVR_TWOSTAGE=0 switches the two-stage reduction off (one-stage instead)
+inline bool env_twostage() {
+  const char* v = getenv("DVR_TWOSTAGE");
+  return !v || atoi(v) != 0;
+}
 // Lowest `want` eigenvalues of the symmetric A (device, n x n, ld n, both
 // triangles; left intact) through the two-stage reduction of a copy in
 // `copy`, with the one-stage reduction of A as fallback; the values end in

@@ -211,6 +211,41 @@ int dvr_get_profile(const dvr_ctx_t* ctx, int32_t idx, char* name, int32_t name_
                     int32_t* launches, double* seconds, double* bytes, double* flops);
 
 /*
+ * Symmetrised-Radau driver (IDIA = -2, AB2 molecules): the body of
+ * SUBROUTINE nfmain(hr,htheta,r,theta,kz) (D:3006-3122) for one k block in
+ * one call -- every angle's symmetrised radial Hamiltonian (blc2d1,
+ * D:3125), their eigenpairs (diag), the selection (choosr D:3223 or cut2dr
+ * D:3323, then testiv D:3380), the 3D Hamiltonian (bloc3d/vecmul D:3455),
+ * the final eigensolve (diag3d) and transr's wavefunction records (D:3599).
+ *   hr     [npnt*npnt]  radial DVR kinetic matrix (k1k2 of r1, D:1603)
+ *   htheta [nalf*nalf]  angular L matrix (lmatrx, D:1846), full range
+ *   r [npnt], xalf [nalf]  radial points, angular points (cos theta)
+ *   ur1, urr1           vibrational and rotational reduced masses (setcon)
+ *   vgrid  [nalf*npnt*npnt]  V[(g*npnt + b)*npnt + a] = POTV(r_a, r_b, x_g)
+ *   emax2               cut energy in hartree (zcut = 1)
+ * Outputs: iv2[nalf] 2D functions kept per angle, *nham3, *iang (angles
+ * with functions), eval[min(neval, nham3)] ascending (hartree) and, for
+ * nvec > 0, wvfunc: nvec records of npair*iang amplitudes (radial function
+ * outer, angle inner), npair = npnt(npnt+1-2 ipar)/2, as transr writes them.
+ * Returns DVR_ERR_CORE_3D when cut2dr keeps more than max3d functions.
+ * J = 0 only (jrot != 0 returns DVR_ERR_ARG): the J > 0 terms (blc2d1's
+ * rotational diagonal, testiv) are not yet checked against a reference run.
+ */
+typedef struct dvr_nf_dims {
+  int32_t npnt;   /* radial points of each bond (npnt1 = npnt2, D:248) */
+  int32_t nalf;   /* angular points (full range) */
+  int32_t ipar;   /* 0: even, 1: odd symmetrised radial functions */
+  int32_t max3d, neval;
+  int32_t zcut;   /* 1: 2D levels below emax2 (cut2dr); 0: the lowest max3d (choosr) */
+  int32_t nvec;   /* levels whose wavefunction records are wanted (0..neval) */
+  int32_t jrot, kz, kmin, zrot, zlin;
+} dvr_nf_dims_t;
+int dvr_nfmain(const dvr_nf_dims_t* dims, const double* hr, const double* htheta,
+               const double* r, const double* xalf, double ur1, double urr1,
+               const double* vgrid, double emax2, int32_t* iv2, int32_t* nham3,
+               int32_t* iang, double* eval, double* wvfunc);
+
+/*
  * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)
  * (D:2638-2659: dsyev('V','L',nham,ham,maxham,eig,...)).  On entry the lower
  * triangle of the column-major ham(maxham,nham) holds the symmetric matrix; on
