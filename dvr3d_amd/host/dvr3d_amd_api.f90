@@ -16,7 +16,23 @@ module dvr3d_amd_api
     integer(c_int) :: nvec, solver
   end type
 
+  type, bind(C) :: dvr_nf_dims_t
+    integer(c_int) :: npnt, nalf, ipar
+    integer(c_int) :: max3d, neval
+    integer(c_int) :: zcut, nvec
+    integer(c_int) :: jrot, kz, kmin, zrot, zlin
+  end type
+
   interface
+    integer(c_int) function dvr_nfmain(dims, hr, htheta, r, xalf, ur1, urr1, vgrid, emax2, &
+                                       iv2, nham3, iang, eval, wvfunc) bind(C, name="dvr_nfmain")
+      import :: c_int, c_double, dvr_nf_dims_t
+      type(dvr_nf_dims_t), intent(in) :: dims
+      real(c_double), intent(in) :: hr(*), htheta(*), r(*), xalf(*), vgrid(*)
+      real(c_double), value :: ur1, urr1, emax2
+      integer(c_int), intent(out) :: iv2(*), nham3, iang
+      real(c_double), intent(out) :: eval(*), wvfunc(*)
+    end function
     integer(c_int) function dvr_ctx_create(dims, device, ctx) bind(C, name="dvr_ctx_create")
       import :: c_int, c_ptr, dvr_dims_t
       type(dvr_dims_t), intent(in) :: dims

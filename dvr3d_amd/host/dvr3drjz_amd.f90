@@ -45,6 +45,13 @@ program dvr3drjz_amd
     write(6, '(5x,a,i4)') 'error reading the .job input, code', ierr
     stop 1
   end if
+  if (job%idia == -2) then
+    ! symmetrised Radau coordinates: the nfmain driver (D:988-1003)
+    call run_nf()
+    call system_clock(t1)
+    write(6, '(/i10,a/)') int((t1 - t0) / trate), ' secs CPU time used'
+    stop
+  end if
   ierr = dvrh_sizes(job, sz)
   if (ierr /= 0) then
     write(6, '(5x,a,i4)') 'option outside the theta-first jhmain path, code', ierr
@@ -264,4 +271,143 @@ program dvr3drjz_amd
   end do
   call system_clock(t1)
   write(6, '(/i10,a/)') int((t1 - t0) / trate), ' secs CPU time used'
+
+contains
+
+  ! IDIA = -2, J = 0: set-up on the host, the potential through POTV on the
+  ! (r_a, r_b, x_g) grid, everything else in one dvr_nfmain call; stdout,
+  ! ILEV and IWAVE as nfmain, diag3d, testiv and transr write them
+  ! (D:3066-3122, 2703-2761, 3447-3448, 3623-3641; set-up records D:820-845).
+  subroutine run_nf()
+    type(dvr_nf_dims_t) :: nd
+    real(c_double), allocatable :: rr(:), hr(:, :), xa(:), wa(:), pl(:, :), ht(:, :), vg(:, :, :)
+    real(c_double), allocatable :: wv(:)
+    integer(c_int), allocatable :: iv2(:)
+    integer(c_int) :: n3, iang
+    integer :: npnt, nalf, npair, g, nbass
+    npnt = job%npnt2; nalf = job%nalf
+    allocate(rr(npnt), hr(npnt, npnt), xa(nalf), wa(nalf), pl(0:nalf - 1, nalf), ht(nalf, nalf))
+    ierr = dvrh_setup_nf(job, sz, rr, hr, xa, wa, pl, ht, cst)
+    if (ierr == 10) then
+      write(6, '(5x,a)') 'IDIA = -2 is available for J = 0 with Morse radial bases in this build'
+      stop 1
+    else if (ierr /= 0) then
+      write(6, '(//5x,a,i4//)') 'points & weights in error, adjust algorithm; code', ierr
+      stop 1
+    end if
+    do i = 1, 72
+      title(i:i) = job%title(i)
+    end do
+    write(6, 1030) npnt, npnt, nalf, sz%neval, sz%max3d
+    write(6, '(5x,a,a)') 'Title: ', title
+    xmass = job%xmass; xmassr = job%xmassr
+    g1 = cst%g1; g2 = cst%g2
+    ztran = job%ztran /= 0
+    zpfun = job%zpfun /= 0
+    npair = sz%max2d
+    allocate(vg(npnt, npnt, nalf), iv2(nalf))
+    do g = 1, nalf
+      do b = 1, npnt
+        do a = 1, npnt
+          call potv(v, rr(a), rr(b), xa(g))
+          vg(a, b, g) = v
+        end do
+      end do
+    end do
+    nd%npnt = npnt; nd%nalf = nalf; nd%ipar = sz%ipar
+    nd%max3d = sz%max3d; nd%neval = sz%neval; nd%zcut = job%zcut
+    nd%nvec = 0
+    if (ztran) nd%nvec = sz%neval
+    nd%jrot = 0; nd%kz = 0; nd%kmin = 0; nd%zrot = 0; nd%zlin = job%zlin
+    allocate(eval(sz%neval), evalcm(sz%neval))
+    allocate(wv(max(1, int(nd%nvec, 8) * npair * nalf)))
+    write(6, '(5x,a)') 'starting the 2d calculations.'
+    rc = dvr_nfmain(nd, hr, ht, rr, xa, cst%ur1, cst%urr1, vg, job%emax2 / autocm, iv2, n3, &
+                    iang, eval, wv)
+    if (rc == DVR_ERR_CORE_3D) then
+      write(6, '(a,i4,a,e12.5,a)') 'number of 2d eigenvalues greater than ', sz%max3d, &
+          ' increase max3d or reduce emax2 which is ', job%emax2, ' cm-1.'
+      stop
+    else if (rc /= DVR_OK) then
+      write(6, '(5x,a,i4)') 'dvr_nfmain failed, code', rc
+      stop 1
+    end if
+    do g = 1, nalf
+      write(6, '(5x,a,i2,a,i3,a)') 'for gamma = ', g, ' selected ', iv2(g), ' energies.'
+    end do
+    write(6, '(25x,a,i5)') 'total = ', n3
+    write(6, '(5x,a)') 'building the 3d hamiltonian.'
+    write(6, '(5x,a)') 'diagonalising the 3d hamiltonian.'
+    meval = min(sz%neval, n3)
+    if (job%zpmin == 0) then
+      write(6, 1010) meval
+      write(6, 1020) (eval(i), i = 1, meval)
+    end if
+    if (zpfun) then
+      ! ILEV (D:467-471, 2708-2719): title for even parity, then
+      ! J, p, jdia = 0, jpar = 0, isym = +-2 and the levels
+      open(unit=job%ilev, form='formatted')
+      if (mod(sz%ipar, 2) == 0) then
+        write(job%ilev, '(9a8)') title
+      else
+        do
+          read(job%ilev, *, iostat=ierr)
+          if (ierr /= 0) exit
+        end do
+      end if
+      isym = 2
+      if (sz%ipar == 1) isym = -2
+      write(job%ilev, '(6i4)') 0, 0, 0, 0, isym, meval
+      write(job%ilev, '(4d20.12)') (eval(i), i = 1, meval)
+      close(job%ilev)
+    end if
+    do i = 1, meval
+      evalcm(i) = eval(i) * autocm
+    end do
+    write(6, 1031) meval
+    write(6, 1020) (evalcm(i), i = 1, meval)
+    if (sz%ipar == 1) then
+      ii = 1
+      ezero = job%ezero
+    else
+      ezero = evalcm(1)
+      ii = 2
+    end if
+    write(6, 1070)
+    write(6, 1021) (evalcm(i) - ezero, i = ii, meval)
+    if (ztran) then
+      zembed = .true.; zmors1 = job%zmors1 /= 0; zmors2 = job%zmors2 /= 0
+      zncor = .true.; zquad2 = job%zquad2 /= 0
+      open(unit=job%iwave, form='unformatted')
+      write(job%iwave) -2, sz%ipar, nalf, npnt, npnt, 0, 0, sz%neval, 0
+      ! one radial basis for both bonds: its flags and parameters twice
+      ! (D:1181-1184)
+      write(job%iwave) zembed, zmors1, zmors1, xmass, g1, g2, zncor, zquad2
+      write(job%iwave) job%re1, job%diss1, job%we1, job%re1, job%diss1, job%we1
+      mbass = nalf * npair
+      write(job%iwave) mbass, 0, nalf, mbass
+      write(job%iwave) rr
+      write(job%iwave) xa
+      write(job%iwave) 0, nalf - 1, nalf, 0
+      write(job%iwave) ((pl(i, a) * wa(a), i = 0, nalf - 1), a = 1, nalf)
+      nbass = iang * npair
+      write(job%iwave) iang, nbass
+      write(job%iwave) (iv2(g), g = 1, nalf)
+      write(job%iwave) sz%neval
+      write(job%iwave) (eval(i), i = 1, sz%neval)
+      do i = 1, sz%neval
+        write(job%iwave) wv(int(i - 1, 8) * nbass + 1:int(i, 8) * nbass)
+      end do
+      close(job%iwave)
+    end if
+1010 format(/5x,'lowest',i4,' eigenvalues in hartrees:',/)
+1031 format(//5x,'Lowest',i4,' eigenvalues in wavenumbers:'/)
+1020 format(5d24.12/)
+1021 format(5f15.6/)
+1030 format(/5x,i5,3x,'radial r1 dvr points used,'/5x,i5,3x,'radial r2 dvr points used,', &
+            /5x,i5,3x,'angular dvr points used, with',/5x,i5,3x,'lowest eigenvectors chosen from', &
+            /5x,i5,3x,'maximum dimension secular problem'/)
+1070 format(//'  Band origins in wavenumbers:'/)
+  end subroutine run_nf
+
 end program dvr3drjz_amd

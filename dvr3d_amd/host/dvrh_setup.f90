@@ -21,7 +21,7 @@ module dvrh_setup
   implicit none
   private
   public :: dvrh_job_t, dvrh_sizes_t, dvrh_consts_t
-  public :: dvrh_read_job_unit, dvrh_read_job, dvrh_sizes, dvrh_setup_basis, dvrh_wsum
+  public :: dvrh_read_job_unit, dvrh_read_job, dvrh_sizes, dvrh_setup_basis, dvrh_wsum, dvrh_setup_nf
   public :: amtoau, autocm
 
   real(c_double), parameter :: amtoau = 1.8228883d03     ! D:1037
@@ -252,9 +252,14 @@ contains
     c%beta1 = job%we1 * sqrt(0.5d0 * c%ur1 / job%diss1)
     c%a1 = 4.0d0 * job%diss1 / job%we1
     c%iu1 = int(c%a1 + 0.5d0)
-    c%beta2 = job%we2 * sqrt(0.5d0 * c%ur2 / job%diss2)
-    c%a2 = 4.0d0 * job%diss2 / job%we2
-    c%iu2 = int(c%a2 + 0.5d0)
+    if (job%diss2 > 0.0d0 .and. job%we2 > 0.0d0) then
+      c%beta2 = job%we2 * sqrt(0.5d0 * c%ur2 / job%diss2)
+      c%a2 = 4.0d0 * job%diss2 / job%we2
+      c%iu2 = int(c%a2 + 0.5d0)
+    else
+      ! no r2 basis card (IDIA = -2 decks use the r1 basis for both bonds)
+      c%beta2 = 0.0d0; c%a2 = 0.0d0; c%iu2 = 0
+    end if
   end subroutine
 
   ! ------------------------------------------------------------------ radial
@@ -616,6 +621,53 @@ contains
     end if
     call angular_dvr(job, sz, kz, xalf, walf, pleg, xl, cst%cswalf, cst%tswalf, ierr)
     dvrh_setup_basis = ierr
+  end function
+
+  ! Set-up of the symmetrised-Radau driver (IDIA = -2, J = 0): the sizes of
+  ! insize (npnt1 = npnt2, idvr = nalf, D:246-252, 333-345), the r1 radial
+  ! DVR used for both bonds and the full-range angular DVR with its L
+  ! matrix.  Returns 0, 10 for J > 0 or other options this build does not
+  ! take for IDIA = -2, or the quadrature's error code.
+  integer(c_int) function dvrh_setup_nf(job, sz, r, hr, xalf, walf, pleg, xl, cst) &
+      bind(C, name="dvrh_setup_nf")
+    type(dvrh_job_t), intent(in) :: job
+    type(dvrh_sizes_t), intent(inout) :: sz
+    real(c_double), intent(out) :: r(job%npnt2), hr(job%npnt2, job%npnt2)
+    real(c_double), intent(out) :: xalf(job%nalf), walf(job%nalf), pleg(0:job%nalf - 1, job%nalf)
+    real(c_double), intent(out) :: xl(job%nalf, job%nalf)
+    type(dvrh_consts_t), intent(out) :: cst
+    integer :: ierr, npair
+    dvrh_setup_nf = 0
+    if (job%idia /= -2 .or. job%jrot /= 0 .or. job%zperp /= 0 .or. job%ztwod /= 0 .or. &
+        job%zmors1 == 0 .or. job%zmors2 == 0 .or. (job%ipar /= 0 .and. job%ipar /= 1)) then
+      dvrh_setup_nf = 10; return
+    end if
+    sz%nalf2 = (job%nalf + 1) / 2
+    sz%maxleg = job%nalf - 1
+    sz%idvr = job%nalf
+    sz%npnta = job%nalf
+    sz%ipar = job%ipar
+    sz%idia = -2; sz%jrot = 0; sz%kmin = 0; sz%zrot = 0; sz%zembed = 1
+    sz%kd = 0; sz%ku = 0
+    npair = job%npnt2 * (job%npnt2 + 1 - 2 * job%ipar) / 2
+    sz%max2d = npair
+    if (job%zall /= 0) then
+      sz%max3d = npair * job%nalf
+    else
+      sz%max3d = min(job%max3d, npair * job%nalf)
+    end if
+    sz%neval = job%neval
+    if (sz%neval <= 0) sz%neval = 10
+    sz%neval = min(sz%max3d, sz%neval)
+    sz%zvec = job%zvec
+    call set_consts(job, cst)
+    call radial_dvr(job%npnt2, job%re1, cst%beta1, cst%a1, cst%iu1, cst%ur1, r, hr, &
+                    cst%csx1, cst%csa1, cst%tsx1, cst%tsa1, ierr)
+    if (ierr /= 0) then
+      dvrh_setup_nf = ierr; return
+    end if
+    call angular_dvr(job, sz, 0, xalf, walf, pleg, xl, cst%cswalf, cst%tswalf, ierr)
+    dvrh_setup_nf = ierr
   end function
 
   ! (w1 + w2) and the rotational diagonal term of mkham1 at every (c, b)

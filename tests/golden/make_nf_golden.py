@@ -20,7 +20,7 @@ Written:
   tests/golden/nf_iwave_ref.npz the records transr writes to fort.26 for
                                 both decks (angle count, basis size, the
                                 per-angle counts, the levels and the first
-                                wavefunction)
+                                two wavefunctions; records 2-8 as raw bytes)
 The grid points come from oracle/nfmain_oracle.py, whose set-up reproduces
 the reference's quadratures bit for bit (tests/test_oracle.py).
 
@@ -98,6 +98,8 @@ def main():
         iw[name + "_nrec"] = np.array([len(recs)])
         iw[name + "_rec1"] = F.ints(recs[0])
         k = 8   # records 1-3, 4 (mbass ...), 5 (r1), 6-8 angular (no r2 record)
+        for i in range(1, k):
+            iw[f"{name}_hdr{i + 1}"] = np.frombuffer(recs[i], np.uint8).copy()
         iw[name + "_iang_nbass"] = F.ints(recs[k])
         iw[name + "_iv2"] = F.ints(recs[k + 1])
         iw[name + "_neval"] = F.ints(recs[k + 2])

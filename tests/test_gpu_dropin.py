@@ -252,3 +252,58 @@ def test_h3p_reference_jobs_through_executable(tmp_path, name):
         a, b2 = F.doubles(recs[pos + 5]), ref[f"k{i}_psi1"]
         assert np.abs(a - np.sign(a @ b2) * b2).max() < 1e-7
         pos += 5 + int(ref[f"k{i}_meval"][0])
+
+
+@pytest.mark.parametrize("name", ["water_J0", "c0r"])
+def test_symmetrised_radau_decks_through_executable(tmp_path, name):
+    """IDIA = -2 decks through the executable: the reference's own water deck
+    (water/jobs/dvr0.job) with the reference's water POTV supplied as a table
+    of its values on the deck's grid (kind 3 of the example POTV), and the c0
+    surface in symmetrised Radau coordinates.  The per-angle selections, the
+    printed levels and every IWAVE record transr writes (D:3555-3646) against
+    the reference run of the same decks."""
+    from oracle import nfmain_oracle as NF
+    g = json.load(open(os.path.join(H.GOLD, "nf_ref.json")))[name]
+    ref = np.load(os.path.join(H.GOLD, "nf_iwave_ref.npz"))
+    deck = os.path.join(H.JOBS, name + ".job")
+    if name == "water_J0":
+        b = NF.nf_basis(O.load_job(deck))
+        s = b.sizes
+        V = np.load(os.path.join(H.GOLD, "water_vgrid.npz"))["v"]
+        G, B, A = np.meshgrid(np.arange(s.nalf), np.arange(s.npnt), np.arange(s.npnt),
+                              indexing="ij")
+        tab = np.stack([b.r[A.ravel()], b.r[B.ravel()], b.xalf[G.ravel()], V.ravel()], axis=1)
+        with open(tmp_path / "table.bin", "wb") as f:
+            np.array([len(tab)], dtype="<f8").tofile(f)
+            tab.astype("<f8").tofile(f)
+        (tmp_path / "pes.par").write_text("3\n" + str(tmp_path / "table.bin") + "\n")
+    else:
+        shutil.copy(os.path.join(H.JOBS, "ab2_c0.par"), tmp_path / "pes.par")
+    shutil.copy(deck, tmp_path / "in.job")
+    with open(tmp_path / "in.job") as fi, open(tmp_path / "out.txt", "w") as fo:
+        r = subprocess.run([EXE], stdin=fi, stdout=fo, stderr=subprocess.STDOUT, cwd=tmp_path,
+                           timeout=300)
+    out = open(tmp_path / "out.txt").read()
+    assert r.returncode == 0, out[-2000:]
+    iv2 = [int(m) for m in re.findall(r"for gamma =\s*\d+ selected\s+(\d+) energies", out)]
+    assert iv2 == g["iv2"]
+    assert int(re.search(r"total =\s*(\d+)", out).group(1)) == g["nham3"]
+    lev = _levels_per_k(out)
+    assert len(lev) == 1 and len(lev[0]) == len(g["eval_au"])
+    assert np.abs(lev[0] - np.array(g["eval_au"])).max() * O.AUTOCM < 1e-4
+    recs = F.read_records(tmp_path / "fort.26")
+    assert len(recs) == int(ref[name + "_nrec"][0])
+    assert np.array_equal(F.ints(recs[0]), ref[name + "_rec1"])
+    for i in (2, 4, 7):     # logicals and masses; mbass ...; the angular k block
+        assert np.array_equal(np.frombuffer(recs[i - 1], np.uint8), ref[f"{name}_hdr{i}"]), i
+    for i in (3, 5, 6, 8):  # Morse parameters; r; the angular points; pleg * w
+        a = np.frombuffer(recs[i - 1], "<f8")
+        b2 = np.frombuffer(ref[f"{name}_hdr{i}"].tobytes(), "<f8")
+        assert len(a) == len(b2) and np.abs(a - b2).max() <= 1e-13 * max(1.0, np.abs(b2).max()), i
+    assert np.array_equal(F.ints(recs[8]), ref[name + "_iang_nbass"])
+    assert np.array_equal(F.ints(recs[9]), ref[name + "_iv2"])
+    assert np.array_equal(F.ints(recs[10]), ref[name + "_neval"])
+    assert np.abs(F.doubles(recs[11]) - ref[name + "_eval"]).max() * O.AUTOCM < 1e-4
+    for k, key in enumerate(("_psi1", "_psi2")):
+        a, b2 = F.doubles(recs[12 + k]), ref[name + key]
+        assert np.abs(a - np.sign(a @ b2) * b2).max() < 1e-7
