@@ -21,7 +21,8 @@
 // subdiagonal and pushes the fill-in down the band in tasks of one BW block
 // each.  Task k of sweep s runs in launch 2 s + k: tasks in one launch touch
 // disjoint blocks (sweep s - 1 is two blocks ahead), so every launch runs all
-// of its tasks at once, one work-group each, with the two blocks in LDS.
+// of its tasks at once, one work-group of 256 threads each, with the diagonal
+// block in registers and the block below it in LDS.
 // A panel whose Cholesky QR breaks down (a rank-deficient panel) is reported,
 // and the caller then uses the one-stage reduction.
 #include "dvr_internal.h"
