@@ -29,7 +29,8 @@ EXPORTS = (
     "dvr_set_potential", "dvr_eval_potential_ab2", "dvr_run", "dvr_run_blocks", "dvr_get_sizes",
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
-    "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_nfmain", "dvr_set_profiling",
+    "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_nfmain", "dvr_rotlev3",
+    "dvr_set_profiling",
     "dvr_get_profile",
     "dvr_get_solver_stats", "dvr_ldl_solve", "dvr_keep_hamiltonian", "dvr_set_preconditioner",
     "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
@@ -45,6 +46,16 @@ class NfDims(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "npnt", "nalf", "ipar", "max3d", "neval", "zcut", "nvec", "jrot", "kz", "kmin",
         "zrot", "zlin")]
+
+
+class RotDims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in (
+        "nblk", "kfirst", "jrot", "idia", "ipar", "npnt1", "npnt2", "zembed", "zquad2", "nlim",
+        "nsets")] + [("neval", C.c_int32 * 2), ("nvec", C.c_int32)]
+
+
+class RotBlock(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("maxleg", "idvr", "lincr", "mvib")]
 
 
 class Dims(C.Structure):
@@ -96,6 +107,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_syev_batch_noalloc.argtypes = [C.c_int32, I, D, I, D, D]
     lib.dvr_nfmain.argtypes = [C.POINTER(NfDims), D, D, D, D, C.c_double, C.c_double, D,
                                C.c_double, I, I, I, D, D]
+    lib.dvr_rotlev3.argtypes = [C.POINTER(RotDims), C.POINTER(RotBlock), D, D, D, D, I, D, D,
+                                D]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
@@ -405,3 +418,60 @@ def nfmain(dims: dict, hr, htheta, r, xalf, ur1, urr1, vgrid, emax2_au=0.0):
     if d.nvec > 0:
         out["wvfunc"] = wv[: d.nvec * npair * int(iang[0])].reshape(d.nvec, -1)
     return out
+
+
+def rotlev3(dims: dict, blocks, pleg, dvrvec, evib, rm2):
+    """dvr_rotlev3: ROTLEV3's Hamiltonian and solve on the device.  blocks:
+    per k block dict(maxleg, idvr, lincr, mvib); pleg: per block the
+    (maxleg+1, idvr) pleg*w array; dvrvec: per block mvib x (idvr*npnt1*npnt2)
+    vectors (angle fastest, as IWAVE stores them); evib: per block its mvib
+    levels.  Returns per set dict(eval, vec, d) (d: per block nkbas x neval)
+    and the per-block (jstart, nang)."""
+    lib = load()
+    nb = len(blocks)
+    ne = list(dims.get("neval", (0, 0))) + [0, 0]
+    d = RotDims(**{k: int(dims.get(k, 0)) for k, _ in RotDims._fields_ if k != "neval"})
+    d.neval[0], d.neval[1] = int(ne[0]), int(ne[1])
+    B = (RotBlock * nb)(*[RotBlock(int(b["maxleg"]), int(b["idvr"]), int(b["lincr"]),
+                                   int(b["mvib"])) for b in blocks])
+    pl = _f64(np.concatenate([np.asarray(p, dtype=np.float64).ravel(order="F") for p in pleg]))
+    xs = [np.asarray(x, dtype=np.float64).reshape(-1) for x in dvrvec]
+    xv = _f64(np.concatenate(xs)) if sum(len(x) for x in xs) else np.zeros(1)
+    ev = _f64(np.concatenate([np.asarray(e, dtype=np.float64)[: b["mvib"]]
+                              for e, b in zip(evib, blocks)]))
+    rm = _f64(rm2)
+    nrad = d.npnt1 * d.npnt2
+    jdia = max(d.idia, 1)
+    jn = np.zeros(2 * nb, dtype=np.int32)
+    # sizes of the outputs (the angular count of block b as JTRAN forms it)
+    nang = []
+    for i, b in enumerate(blocks):
+        k = d.kfirst + i
+        js = k + (1 if k % jdia != d.ipar else 0)
+        nang.append((b["maxleg"] + b["lincr"] - js) // jdia + 1)
+    mv = [int(b["mvib"]) for b in blocks]
+    nbass = sum(mv)
+    sets = [(0, nbass, d.neval[0])] + ([(1, nbass - mv[0], d.neval[1])] if d.nsets == 2 else [])
+    eval_ = np.zeros(sum(x[2] for x in sets))
+    nv = sum(x[1] * x[2] for x in sets)
+    ndv = sum(sum(nang[b] * nrad * x[2] for b in range(x[0], nb)) for x in sets)
+    vec = np.zeros(max(1, nv)) if d.nvec else None
+    dvec = np.zeros(max(1, ndv)) if d.nvec else None
+    rc = lib.dvr_rotlev3(C.byref(d), B, _dp(pl), _dp(xv), _dp(ev), _dp(rm), _ip(jn), _dp(eval_),
+                         _dp(vec), _dp(dvec))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_rotlev3 failed")
+    out, eo, vo, do = [], 0, 0, 0
+    for first, n, nev in sets:
+        r = dict(eval=eval_[eo:eo + nev].copy())
+        eo += nev
+        if d.nvec:
+            r["vec"] = vec[vo:vo + n * nev].reshape(nev, n).T.copy()
+            vo += n * nev
+            r["d"] = []
+            for b in range(first, nb):
+                nk = nang[b] * nrad
+                r["d"].append(dvec[do:do + nk * nev].reshape(nev, nk).T.copy())
+                do += nk * nev
+        out.append(r)
+    return out, jn.reshape(nb, 2)

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 HIP_SOURCES = ["k_grid.hip", "k_jacobi.hip", "k_syrk.hip", "k_syev.hip", "k_syev_wg.hip", "k_gemm.hip", "k_ldl.hip", "k_dav.hip", "k_mlp.hip", "dvr_davidson.hip", "dvr_mlp.hip", "dvr_refine.hip",
-               "k_syev_lds.hip", "k_twostage.hip", "dvr_nfmain.hip", "dvr_noalloc.hip", "dvr_capi.hip"]
+               "k_syev_lds.hip", "k_twostage.hip", "dvr_nfmain.hip", "dvr_rotlev3.hip", "dvr_noalloc.hip", "dvr_capi.hip"]
 ARCH = os.environ.get("DVR_OFFLOAD_ARCH", "gfx950")
 # MFMA accumulators in ordinary VGPRs: without this the compiler keeps them
 # in AGPRs and copies them to VGPRs and back around every k slice of the GEMM

@@ -246,6 +246,46 @@ int dvr_nfmain(const dvr_nf_dims_t* dims, const double* hr, const double* htheta
                int32_t* iang, double* eval, double* wvfunc);
 
 /*
+ * The rotational step ROTLEV3 (source/rotlev3.f90, cited R:<line>): the body
+ * of VRMAIN (R:482-629) for one run -- JTRAN of every k block's vibrational
+ * vectors (R:839-904), the Coriolis blocks of SOLRT (R:671-836), the dense
+ * solve of DGROT (R:1046-1197, DSYEVX 'I': lowest NEVAL levels) for the first
+ * parity set over all blocks and, with nsets = 2 (KMIN > 1), for the second
+ * over blocks 1..nblk-1, and the vectors of DSTORE (R:1297-1323).
+ * The caller reads the IWAVE file and selects (SELECT, R:301-478); block b
+ * (b = 0 .. nblk-1) has k = kfirst + b and keeps its mvib lowest functions.
+ *   pleg    concatenated per block: (maxleg+1) x idvr, the pleg*w record
+ *   dvrvec  concatenated per block: mvib vectors of idvr*npnt1*npnt2
+ *           amplitudes (angle fastest), as in IWAVE
+ *   evib    concatenated per block: the mvib levels (hartree)
+ *   rm2     the r**-2 record of IWAVE (nlim values)
+ * Outputs: jst_nang[2 b], [2 b + 1] = first l and count of the associated
+ * Legendre functions of block b (JTRAN R:879-885; may be NULL); eval:
+ * neval[0] then neval[1] levels; with nvec != 0, vec: the eigenvectors of
+ * each set (ibass x neval, ibass = the set's function count) one set after
+ * the other, and dvec: per set, per block of the set, nkbas x neval with
+ * nkbas = nang * npnt1 * npnt2 (the DSTORE record of that block).
+ */
+typedef struct dvr_rot_dims {
+  int32_t nblk;       /* k blocks */
+  int32_t kfirst;     /* k of block 0: 0 (KMIN >= 1) or 1 (KMIN = 0) */
+  int32_t jrot, idia, ipar;
+  int32_t npnt1, npnt2;
+  int32_t zembed, zquad2;
+  int32_t nlim;       /* length of rm2 */
+  int32_t nsets;      /* 1, or 2 for KMIN > 1 */
+  int32_t neval[2];   /* levels per set */
+  int32_t nvec;       /* 1: eigenvectors and DSTORE vectors wanted */
+} dvr_rot_dims_t;
+typedef struct dvr_rot_block {
+  int32_t maxleg, idvr, lincr;   /* the block's (kz, maxleg, idvr, lincr) record */
+  int32_t mvib;                  /* vibrational functions kept */
+} dvr_rot_block_t;
+int dvr_rotlev3(const dvr_rot_dims_t* dims, const dvr_rot_block_t* blocks, const double* pleg,
+                const double* dvrvec, const double* evib, const double* rm2, int32_t* jst_nang,
+                double* eval, double* vec, double* dvec);
+
+/*
  * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)
  * (D:2638-2659: dsyev('V','L',nham,ham,maxham,eig,...)).  On entry the lower
  * triangle of the column-major ham(maxham,nham) holds the symmetric matrix; on

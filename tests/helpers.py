@@ -77,3 +77,27 @@ def run_device(inp, analytic=True, vgrid=None):
         ctx.set_potential(vgrid)
     ctx.run()
     return ctx
+
+
+def write_h3p_table(tmp, name):
+    """pes.par (kind 3 of the example POTV) and its table: the reference's
+    H3+ POTV on every k block's grid of deck `name` (tests/golden/h3p_vgrid.npz)."""
+    from dvr3d_amd import host
+    V = np.load(os.path.join(GOLD, "h3p_vgrid.npz"))
+    job = host.read_job(os.path.join(JOBS, name + ".job"))
+    sz = host.sizes(job)
+    rows = []
+    for k in range(sz.kd, sz.ku + 1):
+        b = host.Basis(job, k)
+        x = b.xalf[: b.sz.npnta]
+        if job.zr2r1:
+            R1, R2, X = np.meshgrid(b.r1, b.r2, x, indexing="ij")
+        else:
+            R2, R1, X = np.meshgrid(b.r2, b.r1, x, indexing="ij")
+        rows.append(np.stack([R1.ravel(), R2.ravel(), X.ravel(), V[f"{name}_k{k}"].ravel()], axis=1))
+    tab = np.concatenate(rows)
+    with open(os.path.join(tmp, "table.bin"), "wb") as f:
+        np.array([len(tab)], dtype="<f8").tofile(f)
+        tab.astype("<f8").tofile(f)
+    with open(os.path.join(tmp, "pes.par"), "w") as f:
+        f.write("3\n" + os.path.join(str(tmp), "table.bin") + "\n")
