@@ -3,6 +3,7 @@
   dvr3d_amd/libdvr3d_amd.so      HIP kernels + C ABI (hipcc, gfx950)
   dvr3d_amd/libdvr3d_host.so     Fortran host set-up with bind(C) entry points
   dvr3d_amd/host/dvr3drjz_amd    drop-in executable (Fortran host + POTV)
+  dvr3d_amd/host/rotlev3_amd     drop-in ROTLEV3 (Fortran host)
 
 Every call recompiles.  The oracle (reference executable) is built separately
 by oracle/Makefile.
@@ -81,6 +82,11 @@ def build_host(verbose=False):
     rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
     _run(["amdflang", main_o, setup_o, api_o, potv_o, noise_o, "-lm", "-L" + HERE, "-ldvr3d_amd",
           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + rocm_lib, "-o", exe], verbose=verbose)
+    rot_o = os.path.join(BUILD, "rotlev3_amd.o")
+    _run(fc + ["-c", os.path.join(HOST, "rotlev3_amd.f90"), "-o", rot_o], verbose=verbose)
+    rot = os.path.join(HOST, "rotlev3_amd")
+    _run(["amdflang", rot_o, api_o, "-L" + HERE, "-ldvr3d_amd", "-Wl,-rpath,$ORIGIN/..",
+          "-Wl,-rpath," + rocm_lib, "-o", rot], verbose=verbose)
     return lib, exe
 
 

@@ -23,7 +23,28 @@ module dvr3d_amd_api
     integer(c_int) :: jrot, kz, kmin, zrot, zlin
   end type
 
+  type, bind(C) :: dvr_rot_dims_t
+    integer(c_int) :: nblk, kfirst, jrot, idia, ipar
+    integer(c_int) :: npnt1, npnt2, zembed, zquad2, nlim
+    integer(c_int) :: nsets
+    integer(c_int) :: neval(2)
+    integer(c_int) :: nvec
+  end type
+
+  type, bind(C) :: dvr_rot_block_t
+    integer(c_int) :: maxleg, idvr, lincr, mvib
+  end type
+
   interface
+    integer(c_int) function dvr_rotlev3(dims, blocks, pleg, dvrvec, evib, rm2, jst_nang, &
+                                        eval, vec, dvec) bind(C, name="dvr_rotlev3")
+      import :: c_int, c_double, dvr_rot_dims_t, dvr_rot_block_t
+      type(dvr_rot_dims_t), intent(in) :: dims
+      type(dvr_rot_block_t), intent(in) :: blocks(*)
+      real(c_double), intent(in) :: pleg(*), dvrvec(*), evib(*), rm2(*)
+      integer(c_int), intent(out) :: jst_nang(*)
+      real(c_double), intent(out) :: eval(*), vec(*), dvec(*)
+    end function
     integer(c_int) function dvr_nfmain(dims, hr, htheta, r, xalf, ur1, urr1, vgrid, emax2, &
                                        iv2, nham3, iang, eval, wvfunc) bind(C, name="dvr_nfmain")
       import :: c_int, c_double, dvr_nf_dims_t
