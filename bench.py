@@ -17,7 +17,8 @@ where the reference as shipped cannot finish).
 --config c2 runs the J = 2 ABC workload instead: the three k blocks of the
 rotational loop (every level of each block), solved concurrently on one GPU
 (one context and stream per block) and, for comparison, one after another as
-the reference's ccmain loop does.
+the reference's ccmain loop does.  --config rot times the rotational step
+(ROTLEV3) on the c2 vibrational output.
 
 Prints ONE JSON line.  --gpus N with N > 1 is refused: this build runs on one
 GPU.
@@ -41,6 +42,7 @@ CONFIGS = {
     "c3": ("c3.job", "ab2_c3.par"),
     "c4": ("c1.job", "ab2_c0.par"),   # 256 seeded perturbations of c1, 300 levels each
     "c2": ("c2.job", "abc_c2.par"),   # J = 2 ABC, k = 0, 1, 2, all levels of every block
+    "rot": ("c2.job", "abc_c2.par"),  # ROTLEV3 on the c2 vibrational output
 }
 C4_LEVELS = 300
 C4_TRAIN = 32        # c4 instances solved densely and used to train the learned preconditioner
@@ -422,6 +424,94 @@ def krot(args):
     }))
 
 
+def rotbench(args):
+    """rot: the rotational step (ROTLEV3, dvr_rotlev3) on the vibrational
+    output of the c2 deck (J = 2, k = 0, 1, 2, ABC Jacobi, idvr = 100,
+    npnt1 = 30, npnt2 = 60): the lowest --rot-nvib levels of every k block and
+    their wavefunctions (as IWAVE holds them) are computed once, untimed;
+    one step is one dvr_rotlev3 call, KMIN = 2 (e set over k = 0, 1, 2 and
+    f set over k = 1, 2, --rot-neval levels each, eigenvectors and the
+    DIPOLE3 vectors of every block).  The call takes host arrays, as the
+    reference reads its IWAVE file: the upload of the wavefunctions is part
+    of the step and is reported beside it.  Levels are checked against the
+    oracle on the same input when the CPU baseline runs."""
+    from dvr3d_amd import _lib, host
+    jobs = os.path.join(REPO, "dvr3d_amd", "jobs")
+    os.environ["DVR_PES"] = os.path.join(jobs, CONFIGS["c2"][1])
+    job = host.read_job(os.path.join(jobs, CONFIGS["c2"][0]))
+    sz = host.sizes(job)
+    ks = list(range(sz.kd, sz.ku + 1))
+    M = args.rot_nvib
+    progress(f"rot: c2 vibrational blocks k = {ks}, lowest {M} levels with vectors (setup)")
+    ctxs, bases = [], []
+    for k in ks:
+        b = host.Basis(job, k)
+        na = b.sz.npnta
+        ctx = _lib.Context(b.dims(nvec=M, solver=_lib.DVR_SOLVER_DENSE))
+        ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, b.wterm,
+                      job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
+        ctx.set_potential(b.potential_grid())
+        ctxs.append(ctx)
+        bases.append(b)
+    _lib.run_blocks(ctxs)
+    evib = [c.eigenvalues()[:M].copy() for c in ctxs]
+    vecs = [c.wavefunctions(M).reshape(M, -1).copy() for c in ctxs]
+    del ctxs
+    b0 = bases[0]
+    zembed = bool(b0.sz.zembed)
+    rm2 = (0.5 / (b0.r2 ** 2 * b0.consts.urr2) if zembed else
+           0.5 / (b0.r1 ** 2 * b0.consts.urr1))
+    blocks = [dict(maxleg=b.sz.maxleg, idvr=b.sz.idvr, lincr=k, mvib=M)
+              for b, k in zip(bases, ks)]
+    plegw = [np.asfortranarray(b.pleg * b.walf[None, :]) for b in bases]
+    ne = args.rot_neval
+    dims = dict(nblk=len(ks), kfirst=0, jrot=sz.jrot, idia=sz.idia, ipar=sz.ipar,
+                npnt1=job.npnt1, npnt2=job.npnt2, zembed=int(zembed), zquad2=int(job.zquad2),
+                nlim=len(rm2), nsets=2, neval=(ne, ne), nvec=1)
+    xall = np.concatenate([v.reshape(-1) for v in vecs])   # the IWAVE vectors, read once
+    run = lambda: _lib.rotlev3(dims, blocks, plegw, xall, evib, rm2)
+    for _ in range(args.warmup):
+        run()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, jn = run()
+    dt = (time.perf_counter() - t0) / args.steps
+    nrad = job.npnt1 * job.npnt2
+    nk = [int(x) * nrad for x in jn[:, 1]]
+    line = {
+        "metric": "ROTLEV3 step (c2 J=2 vibrational output, both parity sets) per second",
+        "value": round(1.0 / dt, 4), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"rot: ROTLEV3 on c2 (J={sz.jrot}, k={ks}, idvr={sz.idvr}, "
+                               f"npnt1={job.npnt1}, npnt2={job.npnt2}), {M} vibrational "
+                               f"functions per k, lowest {ne} levels per parity set",
+                   "nbass": [len(ks) * M, (len(ks) - 1) * M],
+                   "vector_bytes_uploaded": int(xall.nbytes),
+                   "nkbas": nk},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import rotlev3_oracle as R
+        t2 = time.perf_counter()
+        w = R.Iwave(idia=sz.idia, ipar=sz.ipar, npntt=sz.idvr, npnt1=job.npnt1,
+                    npnt2=job.npnt2, jrot=sz.jrot, kmin0=1, meval=M, nlim=len(rm2),
+                    zembed=zembed, zmors1=True, zmors2=True, xmass=np.zeros(3), g1=0.0,
+                    g2=0.0, z1da=False, zquad2=bool(job.zquad2), morse=np.zeros(6), rm2=rm2,
+                    r1=b0.r1, r2=b0.r2)
+        w.blocks = [R.KBlock(b.xalf, k, b.sz.maxleg, b.sz.idvr, k, pl, e,
+                             [v.tobytes() for v in x])
+                    for b, k, pl, e, x in zip(bases, ks, plegw, evib, vecs)]
+        sel, rh, sets = R.run(w, R.RotInput(nvib=M, neval=ne, kmin=2, neval2=ne, ztran=True))
+        tc = time.perf_counter() - t2
+        err = max(float(np.abs(o["eval"] - s_["eval"]).max()) for o, s_ in zip(out, sets))
+        line["cpu_baseline"] = {"value": round(1.0 / tc, 5), "unit": "steps/s",
+                                "cores": os.cpu_count(), "kind": "port",
+                                "sample": "one full step of the oracle (numpy/OpenBLAS) on the "
+                                          "same input"}
+        line["config"]["max_err_cm_vs_oracle"] = err * 219474.63
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -435,6 +525,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain FP64 dense path run beside the chosen form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--rot-nvib", type=int, default=1000,
+                    help="rot: vibrational functions per k block")
+    ap.add_argument("--rot-neval", type=int, default=500, help="rot: levels per parity set")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 512 MiB three times (calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters against known bytes)")
@@ -471,6 +564,9 @@ def main():
         return
     if args.config == "c2":
         krot(args)
+        return
+    if args.config == "rot":
+        rotbench(args)
         return
     from dvr3d_amd import _lib, host
     cfg = args.config

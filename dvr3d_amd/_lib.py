@@ -425,8 +425,9 @@ def rotlev3(dims: dict, blocks, pleg, dvrvec, evib, rm2):
     per k block dict(maxleg, idvr, lincr, mvib); pleg: per block the
     (maxleg+1, idvr) pleg*w array; dvrvec: per block mvib x (idvr*npnt1*npnt2)
     vectors (angle fastest, as IWAVE stores them); evib: per block its mvib
-    levels.  Returns per set dict(eval, vec, d) (d: per block nkbas x neval)
-    and the per-block (jstart, nang)."""
+    levels.  dvrvec may also be one 1-D array holding every block's vectors
+    one block after another.  Returns per set dict(eval, vec, d) (d: per
+    block nkbas x neval) and the per-block (jstart, nang)."""
     lib = load()
     nb = len(blocks)
     ne = list(dims.get("neval", (0, 0))) + [0, 0]
@@ -435,8 +436,11 @@ def rotlev3(dims: dict, blocks, pleg, dvrvec, evib, rm2):
     B = (RotBlock * nb)(*[RotBlock(int(b["maxleg"]), int(b["idvr"]), int(b["lincr"]),
                                    int(b["mvib"])) for b in blocks])
     pl = _f64(np.concatenate([np.asarray(p, dtype=np.float64).ravel(order="F") for p in pleg]))
-    xs = [np.asarray(x, dtype=np.float64).reshape(-1) for x in dvrvec]
-    xv = _f64(np.concatenate(xs)) if sum(len(x) for x in xs) else np.zeros(1)
+    if isinstance(dvrvec, np.ndarray) and dvrvec.ndim == 1:
+        xv = _f64(dvrvec)          # already concatenated block after block
+    else:
+        xs = [np.asarray(x, dtype=np.float64).reshape(-1) for x in dvrvec]
+        xv = _f64(np.concatenate(xs)) if sum(len(x) for x in xs) else np.zeros(1)
     ev = _f64(np.concatenate([np.asarray(e, dtype=np.float64)[: b["mvib"]]
                               for e, b in zip(evib, blocks)]))
     rm = _f64(rm2)

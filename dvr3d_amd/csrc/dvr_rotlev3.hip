@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -175,8 +176,23 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
     return DVR_ERR_ARG;
 
   hipStream_t s = nullptr;
+  // DVR_ROT_TIMES=1: device time of each phase (events on the stream),
+  // printed to stderr at the end of the call
+  const char* tv = getenv("DVR_ROT_TIMES");
+  const bool times = tv && atoi(tv) != 0;
+  std::vector<hipEvent_t> ev_t;
+  std::vector<const char*> ev_n;
+  auto mark = [&](const char* name) {
+    if (!times) return;
+    hipEvent_t e;
+    DVR_HIP(hipEventCreate(&e));
+    DVR_HIP(hipEventRecord(e, s));
+    ev_t.push_back(e);
+    ev_n.push_back(name);
+  };
   try {
     DVR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    mark("start");
     DevBuf dpl, dx, de, dP, dH, dH1, dA, dB, dJ, dI, dw, work;
     std::vector<DevBuf> coef(nblk);
     // host copies stay alive until the stream has been synchronised
@@ -190,6 +206,7 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
       DVR_HIP(hipMemcpyAsync(dx.p, dvrvec, sizeof(double) * xoff[nblk], hipMemcpyHostToDevice,
                              s));
     }
+    mark("upload");
     // ---- JTRAN: coef_b (nang nrad x mvib) = P_sel (nang x idvr) X_b (idvr x nrad mvib)
     for (int b = 0; b < nblk; ++b) {
       const dvr_rot_block_t& B = blk[b];
@@ -205,6 +222,7 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
                       dP.as<double>(), nang[b], dx.as<double>() + xoff[b], B.idvr, 0.0,
                       coef[b].as<double>(), nang[b], false, 16, false, false);
     }
+    mark("jtran");
     // ---- H: levels on the diagonal, Coriolis blocks below it
     dH.ensure(sizeof(double) * (size_t)nbass * nbass);
     DVR_HIP(hipMemsetAsync(dH.p, 0, sizeof(double) * (size_t)nbass * nbass, s));
@@ -243,6 +261,7 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
     }
     // the dense solver reads both triangles
     launch_mirror_lower(s, dH.as<double>(), nbass);
+    mark("coriolis");
     // the f set (KMIN > 1) is the trailing block without k = 0; copied
     // before the solve of set 1 overwrites H
     if (D->nsets == 2) {
@@ -271,10 +290,14 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
       }
       std::memcpy(eval + eoff, w.hval.data(), sizeof(double) * nev);
       eoff += nev;
-      if (!D->nvec) continue;
+      if (!D->nvec) {
+        mark(set == 0 ? "solve1" : "solve2");
+        continue;
+      }
       const int mx = w.extend(0, nev);
       Z.ensure(sizeof(double) * (size_t)n * mx);
       w.vectors(std::vector<int>{mx}, Z.as<double>(), std::vector<long>{0}, s);
+      mark(set == 0 ? "solve1" : "solve2");
       DVR_HIP(hipMemcpy2DAsync(vec + voff, sizeof(double) * n, Z.p, sizeof(double) * n,
                                sizeof(double) * n, nev, hipMemcpyDeviceToHost, s));
       voff += (long)n * nev;
@@ -297,8 +320,15 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
         row += blk[b].mvib;
       }
       DVR_HIP(hipStreamSynchronize(s));
+      mark(set == 0 ? "dstore1" : "dstore2");
     }
     DVR_HIP(hipStreamSynchronize(s));
+    for (size_t i = 1; i < ev_t.size(); ++i) {
+      float ms = 0.0f;
+      DVR_HIP(hipEventElapsedTime(&ms, ev_t[i - 1], ev_t[i]));
+      fprintf(stderr, "dvr_rotlev3: %-9s %10.3f ms\n", ev_n[i], ms);
+    }
+    for (auto e : ev_t) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(s);
   } catch (const std::exception& e) {
     fprintf(stderr, "dvr_rotlev3: %s\n", e.what());
