@@ -30,7 +30,7 @@ EXPORTS = (
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
     "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_nfmain", "dvr_rotlev3",
-    "dvr_set_profiling",
+    "dvr_dipole3", "dvr_set_profiling",
     "dvr_get_profile",
     "dvr_get_solver_stats", "dvr_ldl_solve", "dvr_keep_hamiltonian", "dvr_set_preconditioner",
     "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
@@ -56,6 +56,20 @@ class RotDims(C.Structure):
 
 class RotBlock(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("maxleg", "idvr", "lincr", "mvib")]
+
+
+class DipDims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nrad", "ipot", "ne1", "ne2", "nblk1", "nblk2", "npair")]
+
+
+class DipBlock(C.Structure):
+    _fields_ = [("nin", C.c_int32), ("vec", C.POINTER(C.c_double)),
+                ("ang", C.POINTER(C.c_double))]
+
+
+class DipPair(C.Structure):
+    _fields_ = [("b1", C.c_int32), ("b2", C.c_int32), ("comp", C.c_int32),
+                ("factor", C.c_double)]
 
 
 class Dims(C.Structure):
@@ -109,6 +123,8 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                C.c_double, I, I, I, D, D]
     lib.dvr_rotlev3.argtypes = [C.POINTER(RotDims), C.POINTER(RotBlock), D, D, D, D, I, D, D,
                                 D]
+    lib.dvr_dipole3.argtypes = [C.POINTER(DipDims), C.POINTER(DipBlock), C.POINTER(DipBlock), D,
+                                C.POINTER(DipPair), D, D]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
@@ -479,3 +495,44 @@ def rotlev3(dims: dict, blocks, pleg, dvrvec, evib, rm2):
                 do += nk * nev
         out.append(r)
     return out, jn.reshape(nb, 2)
+
+
+def dipole3(nrad: int, ipot: int, ket, bra, dipol, pairs, tz=None, tx=None):
+    """dvr_dipole3: DIPOLE3's transition moments on the device.  ket, bra:
+    per k block (vec, ang) with vec an (ne, nin*nrad) array of the block's
+    vectors (index a + nin*mn) and ang the (ipot, nin) values of the input
+    functions at the integration points, or None for a block no pair reads;
+    dipol: (2, nrad*ipot) z and x dipole values with the weights; pairs:
+    (b1, b2, comp, factor) tuples.  Returns (tz, tx), (ne1, ne2) each, added
+    to the given start values."""
+    lib = load()
+    keep = []
+
+    def blocks(side):
+        out, ne = [], 0
+        for b in side:
+            if b is None:
+                out.append(DipBlock(0, None, None))
+                continue
+            vec, ang = b
+            v = _f64(np.asarray(vec, dtype=np.float64))
+            a = _f64(np.asarray(ang, dtype=np.float64).ravel(order="F"))
+            keep.extend([v, a])
+            nin = a.size // ipot
+            ne = v.size // (nin * nrad)
+            out.append(DipBlock(nin, _dp(v), _dp(a)))
+        return (DipBlock * len(out))(*out), ne
+
+    K, ne1 = blocks(ket)
+    B, ne2 = blocks(bra)
+    P = (DipPair * max(1, len(pairs)))(*[DipPair(int(b1), int(b2), int(c), float(f))
+                                          for b1, b2, c, f in pairs])
+    d = DipDims(nrad, ipot, ne1, ne2, len(ket), len(bra), len(pairs))
+    dp = _f64(np.asarray(dipol, dtype=np.float64).reshape(-1))
+    z = np.zeros((ne1, ne2), order="F") if tz is None else np.array(tz, dtype=np.float64, order="F")
+    x = np.zeros((ne1, ne2), order="F") if tx is None else np.array(tx, dtype=np.float64, order="F")
+    rc = lib.dvr_dipole3(C.byref(d), K, B, _dp(dp), P, _dp(z.reshape(-1, order="F")),
+                         _dp(x.reshape(-1, order="F")))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_dipole3 failed")
+    return z, x

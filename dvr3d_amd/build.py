@@ -4,6 +4,7 @@
   dvr3d_amd/libdvr3d_host.so     Fortran host set-up with bind(C) entry points
   dvr3d_amd/host/dvr3drjz_amd    drop-in executable (Fortran host + POTV)
   dvr3d_amd/host/rotlev3_amd     drop-in ROTLEV3 (Fortran host)
+  dvr3d_amd/host/dipole3_amd     drop-in DIPOLE3 (Fortran host + example DIPD)
 
 Every call recompiles.  The oracle (reference executable) is built separately
 by oracle/Makefile.
@@ -19,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 HIP_SOURCES = ["k_grid.hip", "k_jacobi.hip", "k_syrk.hip", "k_syev.hip", "k_syev_wg.hip", "k_gemm.hip", "k_ldl.hip", "k_dav.hip", "k_mlp.hip", "dvr_davidson.hip", "dvr_mlp.hip", "dvr_refine.hip",
-               "k_syev_lds.hip", "k_twostage.hip", "dvr_nfmain.hip", "dvr_rotlev3.hip", "dvr_noalloc.hip", "dvr_capi.hip"]
+               "k_syev_lds.hip", "k_twostage.hip", "dvr_nfmain.hip", "dvr_rotlev3.hip", "dvr_dipole3.hip", "dvr_noalloc.hip", "dvr_capi.hip"]
 ARCH = os.environ.get("DVR_OFFLOAD_ARCH", "gfx950")
 # MFMA accumulators in ordinary VGPRs: without this the compiler keeps them
 # in AGPRs and copies them to VGPRs and back around every k slice of the GEMM
@@ -87,6 +88,13 @@ def build_host(verbose=False):
     rot = os.path.join(HOST, "rotlev3_amd")
     _run(["amdflang", rot_o, api_o, "-L" + HERE, "-ldvr3d_amd", "-Wl,-rpath,$ORIGIN/..",
           "-Wl,-rpath," + rocm_lib, "-o", rot], verbose=verbose)
+    dip_o = os.path.join(BUILD, "dipole3_amd.o")
+    dipd_o = os.path.join(BUILD, "dipd_example.o")
+    _run(fc + ["-c", os.path.join(HOST, "dipole3_amd.f90"), "-o", dip_o], verbose=verbose)
+    _run(fc + ["-c", os.path.join(HOST, "dipd_example.f90"), "-o", dipd_o], verbose=verbose)
+    dip = os.path.join(HOST, "dipole3_amd")
+    _run(["amdflang", dip_o, api_o, dipd_o, "-L" + HERE, "-ldvr3d_amd", "-Wl,-rpath,$ORIGIN/..",
+          "-Wl,-rpath," + rocm_lib, "-o", dip], verbose=verbose)
     return lib, exe
 
 

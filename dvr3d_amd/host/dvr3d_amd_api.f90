@@ -35,7 +35,30 @@ module dvr3d_amd_api
     integer(c_int) :: maxleg, idvr, lincr, mvib
   end type
 
+  type, bind(C) :: dvr_dip_dims_t
+    integer(c_int) :: nrad, ipot, ne1, ne2, nblk1, nblk2, npair
+  end type
+
+  type, bind(C) :: dvr_dip_block_t
+    integer(c_int) :: nin
+    type(c_ptr) :: vec, ang
+  end type
+
+  type, bind(C) :: dvr_dip_pair_t
+    integer(c_int) :: b1, b2, comp
+    real(c_double) :: factor
+  end type
+
   interface
+    integer(c_int) function dvr_dipole3(dims, ket, bra, dipol, pairs, tz, tx) &
+        bind(C, name="dvr_dipole3")
+      import :: c_int, c_double, dvr_dip_dims_t, dvr_dip_block_t, dvr_dip_pair_t
+      type(dvr_dip_dims_t), intent(in) :: dims
+      type(dvr_dip_block_t), intent(in) :: ket(*), bra(*)
+      real(c_double), intent(in) :: dipol(*)
+      type(dvr_dip_pair_t), intent(in) :: pairs(*)
+      real(c_double), intent(inout) :: tz(*), tx(*)
+    end function
     integer(c_int) function dvr_rotlev3(dims, blocks, pleg, dvrvec, evib, rm2, jst_nang, &
                                         eval, vec, dvec) bind(C, name="dvr_rotlev3")
       import :: c_int, c_double, dvr_rot_dims_t, dvr_rot_block_t

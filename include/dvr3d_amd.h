@@ -286,6 +286,54 @@ int dvr_rotlev3(const dvr_rot_dims_t* dims, const dvr_rot_block_t* blocks, const
                 double* eval, double* vec, double* dvec);
 
 /*
+ * The transition moments of DIPOLE3 (source/dipole3_segmented.f90, cited as
+ * P:<line>), replacing the body of DMAIN (P:707-1105): DSRD's evaluation of
+ * every k block's wavefunctions at the angular integration points
+ * (P:1521-1545; for a J = 0 IWAVE file also JTRAN, P:1559-1640) and TRANS's
+ * sums over the radial and angular points (P:1642-1777, rank-one DGER
+ * updates in the reference).
+ *
+ * Block b of each side (ket: nblk1 blocks, bra: nblk2) gives
+ *   vec  ne vectors of nin*nrad values, vector ie at vec + ie*nin*nrad,
+ *        index a + nin*mn (a = angular function or DVR point, mn = radial
+ *        point, r1 fastest): the DSTORE record of a KVEC file, or the IWAVE
+ *        wavefunction records of a J = 0 run
+ *   ang  ipot x nin (column-major): input function a at integration point p,
+ *        i.e. the normalised associated Legendre values of ASLEG for FBR
+ *        input, or those times JTRAN's P_sel for DVR input
+ * Its values at the points are d_b[p + ipot*mn, ie] = sum_a ang[p, a]
+ * vec[a + nin*mn, ie].  dipol holds 2 * nrad*ipot values, the z component
+ * (nu = 0) then the x component (nu = +-1) of LAGPT (P:1187-1217: the
+ * surface times the quadrature weights, doubled for IDIA = 2), index
+ * p + ipot*mn.  Each pair (b1, b2, comp, factor) adds
+ *   t[ie1, ie2] += factor * sum_i dipol[comp][i] d1_b1[i, ie1] d2_b2[i, ie2]
+ * to tz (comp 0) or tx (comp 1), both ne1 x ne2 column-major, accumulated
+ * in place (zero them for a fresh run; a restart passes the stored sums).
+ * factor = threej * xfac * (-1)^kk1 as TRANS forms it (P:1690-1692); pairs
+ * with factor 0 are skipped.  Blocks no pair uses are never read.
+ */
+typedef struct dvr_dip_dims {
+  int32_t nrad;       /* radial points of a block, npnt1*npnt2 */
+  int32_t ipot;       /* angular integration points */
+  int32_t ne1, ne2;   /* ket and bra levels */
+  int32_t nblk1, nblk2;
+  int32_t npair;
+} dvr_dip_dims_t;
+typedef struct dvr_dip_block {
+  int32_t nin;        /* input functions per radial point */
+  const double* vec;
+  const double* ang;
+} dvr_dip_block_t;
+typedef struct dvr_dip_pair {
+  int32_t b1, b2;     /* ket block, bra block (0-based) */
+  int32_t comp;       /* 0: z (tz), 1: x (tx) */
+  double factor;
+} dvr_dip_pair_t;
+int dvr_dipole3(const dvr_dip_dims_t* dims, const dvr_dip_block_t* ket,
+                const dvr_dip_block_t* bra, const double* dipol, const dvr_dip_pair_t* pairs,
+                double* tz, double* tx);
+
+/*
  * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)
  * (D:2638-2659: dsyev('V','L',nham,ham,maxham,eig,...)).  On entry the lower
  * triangle of the column-major ham(maxham,nham) holds the symmetric matrix; on
