@@ -18,7 +18,8 @@ where the reference as shipped cannot finish).
 rotational loop (every level of each block), solved concurrently on one GPU
 (one context and stream per block) and, for comparison, one after another as
 the reference's ccmain loop does.  --config rot times the rotational step
-(ROTLEV3) on the c2 vibrational output.
+(ROTLEV3) on the c2 vibrational output; --config dip the line strengths
+(DIPOLE3) between the two parity sets that ROTLEV3 step produces.
 
 Prints ONE JSON line.  --gpus N with N > 1 is refused: this build runs on one
 GPU.
@@ -43,6 +44,7 @@ CONFIGS = {
     "c4": ("c1.job", "ab2_c0.par"),   # 256 seeded perturbations of c1, 300 levels each
     "c2": ("c2.job", "abc_c2.par"),   # J = 2 ABC, k = 0, 1, 2, all levels of every block
     "rot": ("c2.job", "abc_c2.par"),  # ROTLEV3 on the c2 vibrational output
+    "dip": ("c2.job", "abc_c2.par"),  # DIPOLE3 between the two parity sets of that output
 }
 C4_LEVELS = 300
 C4_TRAIN = 32        # c4 instances solved densely and used to train the learned preconditioner
@@ -512,6 +514,225 @@ def rotbench(args):
     print(json.dumps(line))
 
 
+def _gauss_legendre(n):
+    """n-point Gauss-Legendre rule, the positive points largest first and then
+    their mirror images (DIPOLE3's order, LAGPT dipole3_segmented.f90:1194-1204)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    h = n // 2
+    xp, wp = x[::-1][:h], w[::-1][:h]
+    return np.concatenate([xp, -xp[::-1]]), np.concatenate([wp, wp[::-1]])
+
+
+def _assoc_legendre(lmax, x, m):
+    """Normalised P_{m+l}^m(x), l = 0 .. lmax, columns (ASLEG's recurrence,
+    dipole3_segmented.f90:1234-1308)."""
+    out = np.zeros((len(x), lmax + 2))
+    pmm = np.ones_like(x)
+    s = np.sqrt((1.0 - x) * (1.0 + x))
+    fact = 1.0
+    for _ in range(m):
+        pmm = -pmm * fact * s
+        fact += 2.0
+    out[:, 0] = pmm
+    out[:, 1] = x * (2 * m + 1) * pmm
+    for ll in range(2, lmax + 1):
+        l = m + ll
+        out[:, ll] = (x * (2 * l - 1) * out[:, ll - 1] - (l + m - 1) * out[:, ll - 2]) / (l - m)
+    from math import factorial
+    norm = [np.sqrt((2 * (m + ll) + 1) / (2.0 * factorial(2 * m + ll) / factorial(ll)))
+            for ll in range(lmax + 1)]
+    return out[:, :lmax + 1] * np.array(norm)[None, :]
+
+
+def _threej(j1, j2, j3, m1, m2, m3):
+    """Wigner 3-j symbol (Racah's formula)."""
+    from math import factorial as f, sqrt
+    if m1 + m2 + m3 != 0 or j3 < abs(j1 - j2) or j3 > j1 + j2 or max(abs(m1) - j1, abs(m2) - j2,
+                                                                          abs(m3) - j3) > 0:
+        return 0.0
+    tri = f(j1 + j2 - j3) * f(j1 - j2 + j3) * f(-j1 + j2 + j3) / f(j1 + j2 + j3 + 1)
+    pre = sqrt(tri * f(j1 + m1) * f(j1 - m1) * f(j2 + m2) * f(j2 - m2) * f(j3 + m3) * f(j3 - m3))
+    s = 0.0
+    for t in range(0, j1 + j2 + j3 + 1):
+        d = [t, j3 - j2 + t + m1, j3 - j1 + t - m2, j1 + j2 - j3 - t, j1 - t - m1, j2 - t + m2]
+        if min(d) < 0:
+            continue
+        s += (-1) ** t / np.prod([float(f(v)) for v in d])
+    return (-1) ** (j1 - j2 - m3) * pre * s
+
+
+def dipbench(args):
+    """dip: the line strengths of DIPOLE3 (dvr_dipole3) between the e and f
+    parity sets of the ROTLEV3 step on the c2 output (J = 2: ket e over
+    k = 0, 1, 2, bra f over k = 1, 2, --rot-neval levels each; the k22
+    pairing of the parity tests at c2 size): five k -> k' blocks, nrad = 1800
+    radial points, --dip-npot angular integration points, the synthetic
+    dipole surface of the example DIPD (kind 1).  The vib and rot steps run
+    once, untimed; their DIPOLE3 vectors stay in HBM.  One step is one
+    dvr_dipole3 call reading them in place: every block evaluated at the
+    integration points (one GEMM each) and the moments of every block pair
+    (one dipole-weighted TN GEMM each).  The same call with the vectors in
+    host memory (the PCIe upload included, as the drop-in reads its files)
+    is timed beside it."""
+    import torch
+    from dvr3d_amd import _lib, host
+    jobs = os.path.join(REPO, "dvr3d_amd", "jobs")
+    os.environ["DVR_PES"] = os.path.join(jobs, CONFIGS["c2"][1])
+    job = host.read_job(os.path.join(jobs, CONFIGS["c2"][0]))
+    sz = host.sizes(job)
+    ks = list(range(sz.kd, sz.ku + 1))
+    M, ne = args.rot_nvib, args.rot_neval
+    progress(f"dip: c2 vibrational blocks k = {ks}, {M} levels with vectors, then ROTLEV3 "
+             f"(both parity sets, {ne} levels) (setup)")
+    ctxs, bases = [], []
+    for k in ks:
+        b = host.Basis(job, k)
+        na = b.sz.npnta
+        ctx = _lib.Context(b.dims(nvec=M, solver=_lib.DVR_SOLVER_DENSE))
+        ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, b.wterm,
+                      job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
+        ctx.set_potential(b.potential_grid())
+        ctxs.append(ctx)
+        bases.append(b)
+    _lib.run_blocks(ctxs)
+    evib = [c.eigenvalues()[:M].copy() for c in ctxs]
+    vecs = np.concatenate([c.wavefunctions(M).reshape(-1) for c in ctxs])
+    del ctxs
+    b0 = bases[0]
+    zembed = bool(b0.sz.zembed)
+    rm2 = (0.5 / (b0.r2 ** 2 * b0.consts.urr2) if zembed else
+           0.5 / (b0.r1 ** 2 * b0.consts.urr1))
+    blocks = [dict(maxleg=b.sz.maxleg, idvr=b.sz.idvr, lincr=k, mvib=M)
+              for b, k in zip(bases, ks)]
+    plegw = [np.asfortranarray(b.pleg * b.walf[None, :]) for b in bases]
+    dims = dict(nblk=len(ks), kfirst=0, jrot=sz.jrot, idia=sz.idia, ipar=sz.ipar,
+                npnt1=job.npnt1, npnt2=job.npnt2, zembed=int(zembed), zquad2=int(job.zquad2),
+                nlim=len(rm2), nsets=2, neval=(ne, ne), nvec=1)
+    out, jn = _lib.rotlev3(dims, blocks, plegw, vecs, evib, rm2)
+    del vecs
+    nang = [int(x) for x in jn[:, 1]]
+    nrad = job.npnt1 * job.npnt2
+    J = sz.jrot
+    # ket: e set (KMIN = 1, k = 0 .. J); bra: f set (KMIN = 0, k = 1 .. J)
+    kmin1, kmin2 = 1, 0
+    kets = [(np.ascontiguousarray(d.T), k, nang[b]) for b, (d, k) in enumerate(zip(out[0]["d"], ks))]
+    bras = [(np.ascontiguousarray(d.T), k, nang[b + 1])
+            for b, (d, k) in enumerate(zip(out[1]["d"], ks[1:]))]
+    e1, e2 = out[0]["eval"], out[1]["eval"]
+    npot = args.dip_npot + args.dip_npot % 2
+    ipot = npot if sz.idia < 2 else npot // 2
+    xd, wtd = _gauss_legendre(npot)
+    r1 = b0.r1 if job.zr2r1 else b0.r2
+    r2 = b0.r2 if job.zr2r1 else b0.r1
+    R2, R1, X = np.meshgrid(r2, r1, xd[:ipot], indexing="ij")
+    u1, u2 = R1 - 2.18, R2 - 3.20
+    dipz = 0.35 + 0.12 * u1 - 0.08 * u2 + 0.21 * X + 0.05 * u1 * X
+    dipx = np.sqrt(np.maximum(0.0, 1.0 - X * X)) * (0.18 + 0.07 * u1 + 0.04 * u2 - 0.09 * X)
+    jdia = max(1, sz.idia)
+    dipol = np.stack([(wtd[:ipot] * jdia * v).reshape(-1) for v in (dipz, dipx)])
+    angs = lambda side: [np.asfortranarray(_assoc_legendre(n - 1, xd[:ipot], k))
+                         for _, k, n in side]
+    ak, ab = angs(kets), angs(bras)
+    # the k -> k' blocks of DMAIN for e -> f (dipole3_segmented.f90:877-1049)
+    pairs = []
+    jk1, jk2 = J + kmin1, J + kmin2
+
+    def add(k1, k2, nu, xfac, comp):
+        kk1, kk2 = k1 - kmin1, k2 - kmin2
+        f = _threej(J, 1, J, kk1, nu, -kk2) * xfac * (-1 if kk1 % 2 else 1)
+        pairs.append((k1 - 1, k2 - 1, comp, f))
+    for k1 in range(2 if kmin1 > kmin2 else 1, jk1 + 1):
+        if k1 - kmin1 + kmin2 <= jk2:
+            add(k1, k1 - kmin1 + kmin2, 0, 1.0, 0)
+    for k1 in range(1, jk1 + 1):
+        kk1 = k1 - kmin1
+        if kk1 + 1 + kmin2 <= jk2:
+            add(k1, kk1 + 1 + kmin2, 1, -1.0 if kk1 == 0 else -1.0 / np.sqrt(2.0), 1)
+        if kk1 - 1 + kmin2 >= 1:
+            add(k1, kk1 - 1 + kmin2, -1, 1.0 if kk1 == 1 else 1.0 / np.sqrt(2.0), 1)
+    K = nrad * ipot
+    vec_bytes = sum(v.nbytes for v, _, _ in kets + bras)
+    progress(f"dip: {len(pairs)} block pairs, K = {K}, {len(e1)} x {len(e2)} levels, "
+             f"{vec_bytes / 1e9:.2f} GB of vectors")
+    dev = torch.device("cuda")
+    dk = [(torch.from_numpy(v).to(dev), torch.from_numpy(np.ravel(a, order="F")).to(dev))
+          for (v, _, _), a in zip(kets, ak)]
+    db = [(torch.from_numpy(v).to(dev), torch.from_numpy(np.ravel(a, order="F")).to(dev))
+          for (v, _, _), a in zip(bras, ab)]
+    ddip = torch.from_numpy(dipol).to(dev)
+    torch.cuda.synchronize()
+    run = lambda st=None: _lib.dipole3(nrad, ipot, dk, db, ddip, pairs, stats=st)
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = {}
+    for _ in range(args.steps):
+        tz, tx = run(st)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    hk = [(v, a) for (v, _, _), a in zip(kets, ak)]
+    hb = [(v, a) for (v, _, _), a in zip(bras, ab)]
+    t1 = time.perf_counter()
+    tzh, txh = _lib.dipole3(nrad, ipot, hk, hb, dipol, pairs)
+    dth = time.perf_counter() - t1
+    same = float(max(np.abs(tzh - tz).max(), np.abs(txh - tx).max()))
+    flops_trans = 2.0 * len(e1) * len(e2) * K
+    ach = flops_trans / (st["trans_ms"] / st["ntrans"] * 1e-3) / 1e12
+    nlines = len(e1) * len(e2)
+    line = {
+        "metric": "DIPOLE3 line strengths per second (c2 J=2 e -> f, inputs resident in HBM)",
+        "value": round(nlines / dt, 1), "unit": "line strengths/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"dip: DIPOLE3 between the e (k={ks}) and f (k={ks[1:]}) sets of "
+                               f"ROTLEV3 on c2, {len(e1)} x {len(e2)} levels, npot={npot}, "
+                               f"nrad={nrad}, {len(pairs)} block pairs",
+                   "nang": nang, "K": K, "vector_bytes": vec_bytes,
+                   "host_input_ms_per_step": round(dth * 1e3, 3),
+                   "host_vs_device_input_max_diff": same,
+                   "device_ms": {k: round(st[k], 3) for k in ("transform_ms", "scale_ms",
+                                                              "trans_ms")}},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 3), "peak": FP64_PEAK_TFS,
+                     "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFS, 4),
+                     "traffic": None, "kernel": "dip:trans",
+                     "launches": st["ntrans"],
+                     "avg_launch_us": round(st["trans_ms"] / st["ntrans"] * 1e3, 3)},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import dipole3_oracle as P
+        t2 = time.perf_counter()
+
+        def wave(side, kmin, ev):
+            recs = [v.tobytes() for v, _, _ in side]
+            return P.Wave(idia=sz.idia, ipar=sz.ipar, lmax=sz.idvr, npnt1=job.npnt1,
+                          npnt2=job.npnt2, jrot=J, kmin=kmin, neval=len(ev), zembed=zembed,
+                          zmors1=True, zmors2=True, xmass=np.zeros(3), g1=0.0, g2=0.0,
+                          znco=False, morse=np.zeros(6), jk=J + kmin,
+                          nbass=np.array([n * nrad for _, _, n in side]), r1=r1, r2=r2,
+                          energies=ev, recs=recs, first=0)
+
+        def surf(a, b, x, nu):
+            u, v = a - 2.18, b - 3.20
+            if nu == 0:
+                return 0.35 + 0.12 * u - 0.08 * v + 0.21 * x + 0.05 * u * x
+            return np.sqrt(np.maximum(0.0, 1.0 - x * x)) * (0.18 + 0.07 * u + 0.04 * v - 0.09 * x)
+        res = P.run(wave(kets, kmin1, e1), wave(bras, kmin2, e2), P.DipInput(npot), surf)
+        tc = time.perf_counter() - t2
+        scale = max(np.abs(res["tz"]).max(), np.abs(res["tx"]).max())
+        line["config"]["max_rel_err_vs_port"] = float(
+            max(np.abs(res["tz"] - tz).max(), np.abs(res["tx"] - tx).max()) / scale)
+        line["config"]["port_pairs_match"] = [(p.b1, p.b2, p.comp, round(p.factor, 12))
+                                              for p in res["pairs"]] == \
+            [(a, b, c, round(f, 12)) for a, b, c, f in pairs]
+        line["cpu_baseline"] = {"value": round(nlines / tc, 1), "unit": "line strengths/s",
+                                "cores": os.cpu_count(), "kind": "port",
+                                "sample": "one full step of the oracle (numpy/OpenBLAS GEMMs) "
+                                          "on the same input"}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -527,7 +748,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rot-nvib", type=int, default=1000,
                     help="rot: vibrational functions per k block")
-    ap.add_argument("--rot-neval", type=int, default=500, help="rot: levels per parity set")
+    ap.add_argument("--rot-neval", type=int, default=500,
+                    help="rot, dip: levels per parity set")
+    ap.add_argument("--dip-npot", type=int, default=100,
+                    help="dip: angular integration points")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 512 MiB three times (calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters against known bytes)")
@@ -567,6 +791,9 @@ def main():
         return
     if args.config == "rot":
         rotbench(args)
+        return
+    if args.config == "dip":
+        dipbench(args)
         return
     from dvr3d_amd import _lib, host
     cfg = args.config

@@ -72,6 +72,11 @@ class DipPair(C.Structure):
                 ("factor", C.c_double)]
 
 
+class DipStats(C.Structure):
+    _fields_ = [("transform_ms", C.c_double), ("scale_ms", C.c_double),
+                ("trans_ms", C.c_double), ("ntransform", C.c_int32), ("ntrans", C.c_int32)]
+
+
 class Dims(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "npnta", "npntb", "npntc", "max2d", "max3d", "neval", "zcut", "zall",
@@ -124,7 +129,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     lib.dvr_rotlev3.argtypes = [C.POINTER(RotDims), C.POINTER(RotBlock), D, D, D, D, I, D, D,
                                 D]
     lib.dvr_dipole3.argtypes = [C.POINTER(DipDims), C.POINTER(DipBlock), C.POINTER(DipBlock), D,
-                                C.POINTER(DipPair), D, D]
+                                C.POINTER(DipPair), D, D, C.POINTER(DipStats)]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
@@ -497,14 +502,26 @@ def rotlev3(dims: dict, blocks, pleg, dvrvec, evib, rm2):
     return out, jn.reshape(nb, 2)
 
 
-def dipole3(nrad: int, ipot: int, ket, bra, dipol, pairs, tz=None, tx=None):
+def _arr_ptr(x):
+    """(pointer, element count, keep-alive) of a float64 host array or of a
+    device tensor (anything with data_ptr()/numel(), read in place)."""
+    if hasattr(x, "data_ptr"):
+        return C.cast(C.c_void_p(x.data_ptr()), C.POINTER(C.c_double)), int(x.numel()), x
+    a = _f64(np.asarray(x, dtype=np.float64))
+    return _dp(a), int(a.size), a
+
+
+def dipole3(nrad: int, ipot: int, ket, bra, dipol, pairs, tz=None, tx=None, stats=None):
     """dvr_dipole3: DIPOLE3's transition moments on the device.  ket, bra:
-    per k block (vec, ang) with vec an (ne, nin*nrad) array of the block's
-    vectors (index a + nin*mn) and ang the (ipot, nin) values of the input
-    functions at the integration points, or None for a block no pair reads;
-    dipol: (2, nrad*ipot) z and x dipole values with the weights; pairs:
-    (b1, b2, comp, factor) tuples.  Returns (tz, tx), (ne1, ne2) each, added
-    to the given start values."""
+    per k block (vec, ang) with vec the block's ne vectors of nin*nrad values
+    (an (ne, nin*nrad) array, index a + nin*mn) and ang the (ipot, nin)
+    values of the nin input functions at the ipot integration points (a
+    numpy array, or a device tensor already holding them column-major), or
+    None for a block no pair reads; dipol: (2, nrad*ipot) z and x dipole values with the weights;
+    pairs: (b1, b2, comp, factor) tuples.  vec, ang and dipol may be numpy
+    arrays or device tensors (read in place).  Returns (tz, tx), (ne1, ne2)
+    each, added to the given start values; stats, if a dict, receives the
+    call's device phase times."""
     lib = load()
     keep = []
 
@@ -515,12 +532,14 @@ def dipole3(nrad: int, ipot: int, ket, bra, dipol, pairs, tz=None, tx=None):
                 out.append(DipBlock(0, None, None))
                 continue
             vec, ang = b
-            v = _f64(np.asarray(vec, dtype=np.float64))
-            a = _f64(np.asarray(ang, dtype=np.float64).ravel(order="F"))
-            keep.extend([v, a])
-            nin = a.size // ipot
-            ne = v.size // (nin * nrad)
-            out.append(DipBlock(nin, _dp(v), _dp(a)))
+            if not hasattr(ang, "data_ptr"):
+                ang = np.asarray(ang, dtype=np.float64).ravel(order="F")
+            pa, na, ka = _arr_ptr(ang)
+            pv, nv, kv = _arr_ptr(vec)
+            keep.extend([ka, kv])
+            nin = na // ipot
+            ne = nv // (nin * nrad)
+            out.append(DipBlock(nin, pv, pa))
         return (DipBlock * len(out))(*out), ne
 
     K, ne1 = blocks(ket)
@@ -528,11 +547,16 @@ def dipole3(nrad: int, ipot: int, ket, bra, dipol, pairs, tz=None, tx=None):
     P = (DipPair * max(1, len(pairs)))(*[DipPair(int(b1), int(b2), int(c), float(f))
                                           for b1, b2, c, f in pairs])
     d = DipDims(nrad, ipot, ne1, ne2, len(ket), len(bra), len(pairs))
-    dp = _f64(np.asarray(dipol, dtype=np.float64).reshape(-1))
+    if not hasattr(dipol, "data_ptr"):
+        dipol = np.asarray(dipol, dtype=np.float64).reshape(-1)
+    pd, _, kd = _arr_ptr(dipol)
     z = np.zeros((ne1, ne2), order="F") if tz is None else np.array(tz, dtype=np.float64, order="F")
     x = np.zeros((ne1, ne2), order="F") if tx is None else np.array(tx, dtype=np.float64, order="F")
-    rc = lib.dvr_dipole3(C.byref(d), K, B, _dp(dp), P, _dp(z.reshape(-1, order="F")),
-                         _dp(x.reshape(-1, order="F")))
+    st = DipStats()
+    rc = lib.dvr_dipole3(C.byref(d), K, B, pd, P, _dp(z.reshape(-1, order="F")),
+                         _dp(x.reshape(-1, order="F")), C.byref(st))
     if rc != DVR_OK:
         raise DvrError(rc, "dvr_dipole3 failed")
+    if isinstance(stats, dict):
+        stats.update({k: getattr(st, k) for k, _ in DipStats._fields_})
     return z, x

@@ -2,7 +2,7 @@
 // cited as P:<line>): given the wavefunctions of every k block of a ket and a
 // bra file, the z and x moments of every ket/bra level pair.
 //
-//   1. DSRD (P:1521-1545, with JTRAN P:1559-1640 for a J = 0 IWAVE file):
+//   1. DSRD (P:1511-1548, with JTRAN P:1564-1645 for a J = 0 IWAVE file):
 //      each block's vectors at the angular integration points.  The input
 //      of block b is nin functions per radial point (associated Legendre
 //      FBR functions, or the DVR points), and its values at the points are
@@ -10,9 +10,9 @@
 //      block over all of its levels at once, written straight into the
 //      layout the moment products read (K = nrad ipot rows, one column per
 //      level);
-//   2. TRANS (P:1642-1777): the reference adds nrad*ipot rank-one DGER
+//   2. TRANS (P:1648-1776): the reference adds nrad*ipot rank-one DGER
 //      updates per block pair; here the bra side is scaled by the dipole
-//      values on the grid (LAGPT P:1187-1217, weights included) and the
+//      values on the grid (LAGPT P:1206-1230, weights included) and the
 //      pair's whole sum is one TN GEMM t += factor d1^T (dip .* d2) with
 //      K = nrad ipot, split over work-groups when the output is small.
 // Transformed blocks stay resident until their last pair has used them.
@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <memory>
+#include <utility>
 #include <vector>
 
 namespace dvr {
@@ -41,6 +42,17 @@ k_dip_scale(const double* __restrict__ D, const double* __restrict__ w, long K, 
 
 int grid_rows(long n) { return (int)std::min<long>((n + 255) / 256, 16384); }
 
+// true for memory the device can read in place (hipMalloc'd); host memory,
+// registered or not, is staged through device buffers instead
+bool on_device(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
 }  // namespace
 
 }  // namespace dvr
@@ -49,7 +61,8 @@ using namespace dvr;
 
 extern "C" int dvr_dipole3(const dvr_dip_dims_t* D, const dvr_dip_block_t* ket,
                            const dvr_dip_block_t* bra, const double* dipol,
-                           const dvr_dip_pair_t* pairs, double* tz, double* tx) {
+                           const dvr_dip_pair_t* pairs, double* tz, double* tx,
+                           dvr_dip_stats_t* stats) {
   if (!D || !dipol || !tz || !tx || D->nrad < 1 || D->ipot < 1 || D->ne1 < 1 || D->ne2 < 1 ||
       D->nblk1 < 1 || D->nblk2 < 1 || D->npair < 0 || (D->npair > 0 && (!pairs || !ket || !bra)))
     return DVR_ERR_ARG;
@@ -74,20 +87,40 @@ extern "C" int dvr_dipole3(const dvr_dip_dims_t* D, const dvr_dip_block_t* ket,
       }
 
   hipStream_t s = nullptr;
-  const char* tv = getenv("DVR_DIP_TIMES");
-  const bool times = tv && atoi(tv) != 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // with stats: an event pair around every phase launch, summed at the end
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  auto tick = [&](int ph) -> hipEvent_t {
+    if (!stats) return nullptr;
+    hipEvent_t a;
+    DVR_HIP(hipEventCreate(&a));
+    DVR_HIP(hipEventRecord(a, s));
+    ev[ph].push_back({a, nullptr});
+    return a;
+  };
+  auto tock = [&](int ph) {
+    if (!stats) return;
+    hipEvent_t b;
+    DVR_HIP(hipEventCreate(&b));
+    DVR_HIP(hipEventRecord(b, s));
+    ev[ph].back().second = b;
+  };
+  auto drop_events = [&]() {
+    for (auto& v : ev)
+      for (auto& e : v) {
+        if (e.first) (void)hipEventDestroy(e.first);
+        if (e.second) (void)hipEventDestroy(e.second);
+      }
+  };
   try {
     DVR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    if (times) {
-      DVR_HIP(hipEventCreate(&e0));
-      DVR_HIP(hipEventCreate(&e1));
-      DVR_HIP(hipEventRecord(e0, s));
-    }
     DevBuf ddip, dt[2], E, X, A, work;
     const long nt = (long)ne[0] * ne[1];
-    ddip.ensure(sizeof(double) * 2 * K);
-    DVR_HIP(hipMemcpyAsync(ddip.p, dipol, sizeof(double) * 2 * K, hipMemcpyHostToDevice, s));
+    const double* dip = dipol;
+    if (!on_device(dipol)) {
+      ddip.ensure(sizeof(double) * 2 * K);
+      DVR_HIP(hipMemcpyAsync(ddip.p, dipol, sizeof(double) * 2 * K, hipMemcpyHostToDevice, s));
+      dip = ddip.as<double>();
+    }
     double* hts[2] = {tz, tx};
     for (int c = 0; c < 2; ++c) {
       dt[c].ensure(sizeof(double) * nt);
@@ -103,21 +136,33 @@ extern "C" int dvr_dipole3(const dvr_dip_dims_t* D, const dvr_dip_block_t* ket,
       val[sd][b].reset(new DevBuf());
       DevBuf& out = *val[sd][b];
       out.ensure(sizeof(double) * K * n);
-      A.ensure(sizeof(double) * ipot * nin);
-      DVR_HIP(hipMemcpyAsync(A.p, B.ang, sizeof(double) * ipot * nin, hipMemcpyHostToDevice, s));
+      const double* ang = B.ang;
+      if (!on_device(ang)) {
+        A.ensure(sizeof(double) * ipot * nin);
+        DVR_HIP(hipMemcpyAsync(A.p, B.ang, sizeof(double) * ipot * nin, hipMemcpyHostToDevice,
+                               s));
+        ang = A.as<double>();
+      }
+      const bool vdev = on_device(B.vec);
       // the product's second grid dimension counts 64-column tiles of
       // nrad * levels: levels in chunks that keep it under 65536
       const long lev_max = std::max<long>(1, (65535L * 64) / nrad);
       const long chunk = std::min<long>(n, lev_max);
-      X.ensure(sizeof(double) * nin * nrad * chunk);
+      if (!vdev) X.ensure(sizeof(double) * nin * nrad * chunk);
       for (long l0 = 0; l0 < n; l0 += chunk) {
         const long m = std::min(chunk, n - l0);
-        // the previous chunk's product reads X: the copy is ordered after it
-        DVR_HIP(hipMemcpyAsync(X.p, B.vec + nin * nrad * l0, sizeof(double) * nin * nrad * m,
-                               hipMemcpyHostToDevice, s));
+        const double* x = B.vec + nin * nrad * l0;
+        if (!vdev) {
+          // the previous chunk's product reads X: the copy is ordered after it
+          DVR_HIP(hipMemcpyAsync(X.p, x, sizeof(double) * nin * nrad * m, hipMemcpyHostToDevice,
+                                 s));
+          x = X.as<double>();
+        }
+        tick(0);
         launch_gemm_var(s, "dip:dsrd", false, false, (int)ipot, (int)(nrad * m), (int)nin, 1.0,
-                        A.as<double>(), ipot, X.as<double>(), nin, 0.0,
-                        out.as<double>() + K * l0, ipot, false, 16, false, false);
+                        ang, ipot, x, nin, 0.0, out.as<double>() + K * l0, ipot, false, 16,
+                        false, false);
+        tock(0);
       }
       // A and X are rewritten by the next block's copies, which the stream
       // orders after these products
@@ -128,14 +173,18 @@ extern "C" int dvr_dipole3(const dvr_dip_dims_t* D, const dvr_dip_block_t* ket,
       if (!val[0][p.b1]) transform(0, p.b1);
       if (!val[1][p.b2]) transform(1, p.b2);
       E.ensure(sizeof(double) * K * ne[1]);
+      tick(1);
       DVR_LAUNCH("dip:scale", 16.0 * K * ne[1] + 8.0 * K, 0.0, k_dip_scale,
                  dim3(grid_rows(K * ne[1])), dim3(256), 0, s, val[1][p.b2]->as<double>(),
-                 ddip.as<double>() + K * p.comp, K, (long)ne[1], E.as<double>());
+                 dip + K * p.comp, K, (long)ne[1], E.as<double>());
+      tock(1);
       const int ks = gemm_auto_ksplit(ne[0], ne[1], (int)std::min<long>(K, 1L << 30));
       work.ensure(sizeof(double) * std::max<size_t>(1, gemm_work_doubles(ne[0], ne[1], ks)));
+      tick(2);
       launch_gemm(s, "dip:trans", true, false, ne[0], ne[1], (int)K, p.factor,
                   val[0][p.b1]->as<double>(), K, E.as<double>(), K, 1.0, dt[p.comp].as<double>(),
                   ne[0], ks, work.as<double>(), false);
+      tock(2);
       // release what no later pair reads (the stream orders the free after
       // the products that read it only once it is synchronised)
       for (int sd = 0; sd < 2; ++sd) {
@@ -149,15 +198,21 @@ extern "C" int dvr_dipole3(const dvr_dip_dims_t* D, const dvr_dip_block_t* ket,
     for (int c = 0; c < 2; ++c)
       DVR_HIP(hipMemcpyAsync(hts[c], dt[c].p, sizeof(double) * nt, hipMemcpyDeviceToHost, s));
     DVR_HIP(hipStreamSynchronize(s));
-    if (times) {
-      float ms = 0.0f;
-      DVR_HIP(hipEventRecord(e1, s));
-      DVR_HIP(hipEventSynchronize(e1));
-      DVR_HIP(hipEventElapsedTime(&ms, e0, e1));
-      fprintf(stderr, "dvr_dipole3: %10.3f ms\n", ms);
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
+    if (stats) {
+      double ms3[3] = {0.0, 0.0, 0.0};
+      for (int ph = 0; ph < 3; ++ph)
+        for (auto& e : ev[ph]) {
+          float ms = 0.0f;
+          DVR_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+          ms3[ph] += ms;
+        }
+      stats->transform_ms = ms3[0];
+      stats->scale_ms = ms3[1];
+      stats->trans_ms = ms3[2];
+      stats->ntransform = (int32_t)ev[0].size();
+      stats->ntrans = (int32_t)ev[2].size();
     }
+    drop_events();
     (void)hipStreamDestroy(s);
   } catch (const std::exception& e) {
     fprintf(stderr, "dvr_dipole3: %s\n", e.what());
@@ -165,6 +220,7 @@ extern "C" int dvr_dipole3(const dvr_dip_dims_t* D, const dvr_dip_block_t* ket,
       (void)hipStreamSynchronize(s);
       (void)hipStreamDestroy(s);
     }
+    drop_events();
     return DVR_ERR_HIP;
   }
   return DVR_OK;

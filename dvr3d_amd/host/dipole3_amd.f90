@@ -6,12 +6,12 @@
 ! angular integration points and the transition-moment sums run on the GPU
 ! through dvr_dipole3 (libdvr3d_amd).
 !
-! The host does what the reference's host does: INSIZE's checks (P:196-614),
-! GENIND (P:658-705), the Gauss-Legendre points and the dipole surface on the
-! grid (LAGPT P:1151-1217), the associated Legendre values (ASLEG P:1220-1271)
-! and JTRAN's transformation matrix (P:1559-1640), the order and factors of
-! the k -> k' blocks (DMAIN P:707-1105, THREEJ P:1780-1823), the restart file
-! (RDSCR/WRSCR) and SPECT/OUTPT (P:1827-2008).  The dipole surface is the
+! The host does what the reference's host does: INSIZE's checks (P:198-620),
+! GENIND (P:656-704), the Gauss-Legendre points and the dipole surface on the
+! grid (LAGPT P:1141-1230), the associated Legendre values (ASLEG P:1234-1308)
+! and JTRAN's transformation matrix (P:1564-1645), the order and factors of
+! the k -> k' blocks (DMAIN P:708-1115, THREEJ P:1778-1822), the restart file
+! (RDSCR/WRSCR P:2022-2041) and SPECT/OUTPT (P:1825-2002).  The dipole surface is the
 ! user's SUBROUTINE DIPD(DIPC,R1,R2,XCOS,NU), linked as for the reference.
 ! Not supported (refused with a message): IDIA = -2 files (symmetrised Radau,
 ! the ROTLEV3B output) and pairs of non-Coriolis files.
@@ -20,7 +20,7 @@ program dipole3_amd
   use dvr3d_amd_api
   implicit none
 
-  ! namelist PRT (P:133-136) with the block data defaults (P:162-166); the
+  ! namelist PRT (P:145-146) with the block data defaults (P:192-194); the
   ! logical switches live in COMMON /LOGIC/ as in the reference, where a
   ! user's DIPD may declare them
   logical :: zmors1, znco1, znco2, zprint, zpmin, ztra, zstart, zmors2
@@ -100,7 +100,7 @@ contains
     read(iu) w%zemb, w%zmor1, w%zmor2, w%xm, w%g1, w%g2, w%znco
   end subroutine
 
-  ! INSIZE (P:196-614): the checks that bra and ket belong together, the
+  ! INSIZE (P:198-620): the checks that bra and ket belong together, the
   ! block counts and the cards of the integration
   subroutine insize()
     double precision, parameter :: toler = 1.0d-3
@@ -173,7 +173,7 @@ contains
       if (abs(ket%morse(i) - bra%morse(i)) > toler) &
         call fatal('radial basis parameters (re, diss, we) incompatible')
     end do
-    ! non-Coriolis files next to coupled ones (P:508-528)
+    ! non-Coriolis files next to coupled ones (P:485-503)
     if (znco1 .and. znco2) call fatal('non-Coriolis wavefunction pairs are not supported')
     if (znco1 .and. .not. znco2) then
       if (abs(jrot1) > 1) call fatal('attempt to mix no coriolis and coupled calcs: stop')
@@ -242,7 +242,7 @@ contains
     end if
   end subroutine
 
-  ! GENIND (P:658-705) and the rest of a file: the radial grids, energies
+  ! GENIND (P:656-704) and the rest of a file: the radial grids, energies
   ! and the vectors of levels ibase .. ibase + ne - 1 of every k block
   subroutine read_body(w, ne, ibase, jay_ipar, blk)
     type(wavefile), intent(inout) :: w
@@ -303,7 +303,7 @@ contains
   end subroutine read_body
 
   ! first l, offset and count of the FBR functions of a J = 0 IWAVE file
-  ! (JTRAN P:1576-1590)
+  ! (JTRAN P:1578-1600)
   subroutine jtran_matrix(w, jay_ipar, jstart, jj0, nang)
     type(wavefile), intent(in) :: w
     integer, intent(in) :: jay_ipar
@@ -321,7 +321,7 @@ contains
   end subroutine
 
   ! the values of a block's input functions at the integration points
-  ! (DSRD P:1521-1545): normalised associated Legendre functions of order
+  ! (DSRD P:1511-1548): normalised associated Legendre functions of order
   ! kz (every other one from jfirst for IDIA = 2), for a J = 0 IWAVE file
   ! multiplied by JTRAN's rows of pleg*w
   subroutine angular_values(w, b, kz, jay_ipar, blk)
@@ -333,7 +333,7 @@ contains
     m = kz
     if (w%jk <= 1) then
       call jtran_matrix(w, jay_ipar, jstart, jj0, nang)
-      m = w%kz                       ! the order DSRD reads from the file (P:1496)
+      m = w%kz                       ! the order DSRD reads from the file (P:1489)
     else
       nang = w%nbass(b) / nrad
     end if
@@ -366,7 +366,7 @@ contains
     end if
   end subroutine angular_values
 
-  ! ASLEG (P:1220-1271): normalised P_{m+ll}^m(x_i), ll = 0 .. lmax, with
+  ! ASLEG (P:1234-1308): normalised P_{m+ll}^m(x_i), ll = 0 .. lmax, with
   ! the (-1)^m phase of the recurrence
   subroutine asleg(pleg, lmax, x, n, m)
     integer, intent(in) :: lmax, n, m
@@ -408,7 +408,7 @@ contains
   end subroutine asleg
 
   ! npot-point Gauss-Legendre quadrature as LAGPT and JACOBI form it
-  ! (P:1151-1186, 1274-1403; Stroud and Secrest's start values for the
+  ! (P:1166-1204, 1311-1409; Stroud and Secrest's start values for the
   ! zeros, Newton steps, the weight from the derivative of the last step's
   ! starting point), then the mirror points
   subroutine gauss_legendre(n, x, a)
@@ -495,7 +495,7 @@ contains
     dpn = dp
   end subroutine newton
 
-  ! THREEJ (P:1780-1823) from the binomial table of SETFAC (P:1110-1130)
+  ! THREEJ (P:1778-1822) from the binomial table of SETFAC (P:1118-1138)
   double precision function threej(j1, j2, j3, m1, m2, m3, binom, nbin)
     integer, intent(in) :: j1, j2, j3, m1, m2, m3, nbin
     double precision, intent(in) :: binom(nbin, nbin)
@@ -530,9 +530,9 @@ contains
     if (mod(n3 + imin, 2) /= 0) threej = -threej
   end function threej
 
-  ! the k -> k' blocks of DMAIN in the reference's order (P:807-838 for
-  ! nu = 0, P:864-1047 for nu = +-1) with TRANS's factor threej * xfac *
-  ! (-1)^kk1 (P:1690-1692); factor 0 marks a block with an empty side
+  ! the k -> k' blocks of DMAIN in the reference's order (P:877-913 for
+  ! nu = 0, P:933-1049 for nu = +-1) with TRANS's factor threej * xfac *
+  ! (-1)^kk1 (P:1699-1700); factor 0 marks a block with an empty side
   subroutine schedule()
     integer :: ks1, k1, k2, kk1, kk2, i, j, n
     double precision :: xfac
@@ -600,7 +600,7 @@ contains
     pfac(npair) = f
   end subroutine add
 
-  ! LAGPT (P:1187-1217): dipol(j + ipot (i1 - 1 + npnt1 (i2 - 1)), nu + 1)
+  ! LAGPT (P:1206-1230): dipol(j + ipot (i1 - 1 + npnt1 (i2 - 1)), nu + 1)
   ! = wtd_j DIPD(r1_i1, r2_i2, x_j, nu), doubled for IDIA = 2
   subroutine lagpt(nu)
     integer, intent(in) :: nu
@@ -701,7 +701,7 @@ contains
     dims%nblk1 = jk1
     dims%nblk2 = jk2
     dims%npair = nrun
-    rc = dvr_dipole3(dims, ck, cb, dipol, cp, tz, tx)
+    rc = dvr_dipole3(dims, ck, cb, dipol, cp, tz, tx, c_null_ptr)
     if (rc /= DVR_OK) then
       write(6, '(/5x,a,i3)') 'dvr_dipole3 failed with code', rc
       stop 1
@@ -726,7 +726,7 @@ contains
     call spect()
   end subroutine dmain
 
-  ! SPECT (P:1827-1962) and OUTPT (P:1965-2008)
+  ! SPECT (P:1825-1953) and OUTPT (P:1956-2002)
   subroutine spect()
     double precision :: ezero, xf, xe1, dd, dd3, tzd, txd, t, sxd, a
     double precision, allocatable :: xe2(:), sint(:, :)

@@ -50,14 +50,15 @@ module dvr3d_amd_api
   end type
 
   interface
-    integer(c_int) function dvr_dipole3(dims, ket, bra, dipol, pairs, tz, tx) &
+    integer(c_int) function dvr_dipole3(dims, ket, bra, dipol, pairs, tz, tx, stats) &
         bind(C, name="dvr_dipole3")
-      import :: c_int, c_double, dvr_dip_dims_t, dvr_dip_block_t, dvr_dip_pair_t
+      import :: c_int, c_double, c_ptr, dvr_dip_dims_t, dvr_dip_block_t, dvr_dip_pair_t
       type(dvr_dip_dims_t), intent(in) :: dims
       type(dvr_dip_block_t), intent(in) :: ket(*), bra(*)
       real(c_double), intent(in) :: dipol(*)
       type(dvr_dip_pair_t), intent(in) :: pairs(*)
       real(c_double), intent(inout) :: tz(*), tx(*)
+      type(c_ptr), value :: stats
     end function
     integer(c_int) function dvr_rotlev3(dims, blocks, pleg, dvrvec, evib, rm2, jst_nang, &
                                         eval, vec, dvec) bind(C, name="dvr_rotlev3")

@@ -287,10 +287,10 @@ int dvr_rotlev3(const dvr_rot_dims_t* dims, const dvr_rot_block_t* blocks, const
 
 /*
  * The transition moments of DIPOLE3 (source/dipole3_segmented.f90, cited as
- * P:<line>), replacing the body of DMAIN (P:707-1105): DSRD's evaluation of
+ * P:<line>), replacing the body of DMAIN (P:708-1115): DSRD's evaluation of
  * every k block's wavefunctions at the angular integration points
- * (P:1521-1545; for a J = 0 IWAVE file also JTRAN, P:1559-1640) and TRANS's
- * sums over the radial and angular points (P:1642-1777, rank-one DGER
+ * (P:1511-1548; for a J = 0 IWAVE file also JTRAN, P:1564-1645) and TRANS's
+ * sums over the radial and angular points (P:1648-1776, rank-one DGER
  * updates in the reference).
  *
  * Block b of each side (ket: nblk1 blocks, bra: nblk2) gives
@@ -303,14 +303,18 @@ int dvr_rotlev3(const dvr_rot_dims_t* dims, const dvr_rot_block_t* blocks, const
  *        input, or those times JTRAN's P_sel for DVR input
  * Its values at the points are d_b[p + ipot*mn, ie] = sum_a ang[p, a]
  * vec[a + nin*mn, ie].  dipol holds 2 * nrad*ipot values, the z component
- * (nu = 0) then the x component (nu = +-1) of LAGPT (P:1187-1217: the
+ * (nu = 0) then the x component (nu = +-1) of LAGPT (P:1206-1230: the
  * surface times the quadrature weights, doubled for IDIA = 2), index
  * p + ipot*mn.  Each pair (b1, b2, comp, factor) adds
  *   t[ie1, ie2] += factor * sum_i dipol[comp][i] d1_b1[i, ie1] d2_b2[i, ie2]
  * to tz (comp 0) or tx (comp 1), both ne1 x ne2 column-major, accumulated
  * in place (zero them for a fresh run; a restart passes the stored sums).
- * factor = threej * xfac * (-1)^kk1 as TRANS forms it (P:1690-1692); pairs
- * with factor 0 are skipped.  Blocks no pair uses are never read.
+ * factor = threej * xfac * (-1)^kk1 as TRANS forms it (P:1699-1700); pairs
+ * with factor 0 are skipped.  Blocks no pair uses are never read.  vec, ang
+ * and dipol may be host memory (copied in) or device memory of the current
+ * device (read in place: a caller that already holds the vectors on the GPU,
+ * e.g. from dvr_rotlev3's products, moves nothing over PCIe); tz and tx are
+ * host memory.  Runs on a stream of its own and returns when done.
  */
 typedef struct dvr_dip_dims {
   int32_t nrad;       /* radial points of a block, npnt1*npnt2 */
@@ -329,9 +333,16 @@ typedef struct dvr_dip_pair {
   int32_t comp;       /* 0: z (tz), 1: x (tx) */
   double factor;
 } dvr_dip_pair_t;
+/* Device times of the call (HIP events on its stream; stats may be NULL):
+ * the blocks' evaluation at the points, the dipole scaling and the moment
+ * products (split-K reduction included), with their launch counts. */
+typedef struct dvr_dip_stats {
+  double transform_ms, scale_ms, trans_ms;
+  int32_t ntransform, ntrans;
+} dvr_dip_stats_t;
 int dvr_dipole3(const dvr_dip_dims_t* dims, const dvr_dip_block_t* ket,
                 const dvr_dip_block_t* bra, const double* dipol, const dvr_dip_pair_t* pairs,
-                double* tz, double* tx);
+                double* tz, double* tx, dvr_dip_stats_t* stats);
 
 /*
  * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)

@@ -7,16 +7,16 @@ checker of the device path, never by the product.
 
 What is restated (coordinates with IDIA > -2; the symmetrised Radau branches
 of IDIA = -2 belong to ROTLEV3B input and are not part of this build):
-  read_wave   INSIZE P:196-614, GENIND P:658-705    headers, block sizes
-  gauss_legendre  LAGPT P:1151-1186, JACOBI/ROOT/RECUR P:1274-1403
+  read_wave   INSIZE P:198-620, GENIND P:656-704    headers, block sizes
+  gauss_legendre  LAGPT P:1141-1204, JACOBI/ROOT/RECUR P:1311-1409
                                   npot-point Gauss-Legendre, Stroud-Secrest
-  dipole_grid LAGPT P:1187-1217   DIPD at (r1, r2, x) times the weights
-  asleg       ASLEG P:1220-1271   normalised associated Legendre functions
-  jtran       JTRAN P:1559-1640   DVR vectors of a J = 0 IWAVE -> FBR
-  transform   DSRD  P:1411-1556   FBR coefficients -> the integration points
-  threej      THREEJ P:1780-1823  3-j symbol from the binomial table (SETFAC)
-  run         DMAIN P:707-1105, TRANS P:1642-1777  the k -> k' block sums
-  spect       SPECT P:1827-1962, OUTPT P:1965-2008  frequencies, moments,
+  dipole_grid LAGPT P:1206-1230   DIPD at (r1, r2, x) times the weights
+  asleg       ASLEG P:1234-1308   normalised associated Legendre functions
+  jtran       JTRAN P:1564-1645   DVR vectors of a J = 0 IWAVE -> FBR
+  transform   DSRD  P:1412-1562   FBR coefficients -> the integration points
+  threej      THREEJ P:1778-1822  3-j symbol from the binomial table (SETFAC)
+  run         DMAIN P:708-1115, TRANS P:1648-1776  the k -> k' block sums
+  spect       SPECT P:1825-1953, OUTPT P:1956-2002  frequencies, moments,
                                   line strengths, Einstein A, the SPECTRA file
 
 Parity pinned by tests/test_dip_oracle.py against the reference DIPOLE3 built
@@ -33,9 +33,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-AUTOCM = 2.19474624e05      # P:1853
-AUTODE = 2.5417662          # P:1854
-DETOSEC = 3.136186e-07      # P:1855
+AUTOCM = 2.19474624e05      # P:1851
+AUTODE = 2.5417662          # P:1852
+DETOSEC = 3.136186e-07      # P:1853
 
 
 def read_records(path):
@@ -94,8 +94,8 @@ class Wave:
 
 
 def read_wave(path) -> Wave:
-    """Header records as INSIZE reads them (P:264-275, 521-522), the block
-    table of GENIND (P:680) and the energies (P:873-888)."""
+    """Header records as INSIZE reads them (P:267-275, 442-443), the block
+    table of GENIND (P:680) and the energies (P:817-828)."""
     r = read_records(path)
     h = _i(r[0])
     lg = _i(r[1][:12])
@@ -104,7 +104,7 @@ def read_wave(path) -> Wave:
     znco = bool(_i(r[1][52:56])[0])
     idia, ipar, lmax, n1, n2, jrot, kmin, neval = (int(x) for x in h[:8])
     # INSIZE's KMIN: 0 for a non-Coriolis file paired with a coupled one
-    # (P:514-528), 1 for J = 0 (P:538); jk = J + KMIN blocks (P:539), which
+    # (P:485-503), 1 for J = 0 (P:516); jk = J + KMIN blocks (P:517), which
     # is also the length of GENIND's block record
     t = _i(r[3])
     if (len(t) - 1) % 3 != 0:
@@ -129,8 +129,8 @@ def read_wave(path) -> Wave:
 
 @dataclass
 class DipInput:
-    """Namelist PRT (P:133-136, 162-166) and the cards of INSIZE/SPECT
-    (P:563-567, 1878-1880)."""
+    """Namelist PRT (P:145-146, 192-194) and the cards of INSIZE/SPECT
+    (P:564-567, 1884)."""
     npot: int
     nv1: int = 0
     nv2: int = 0
@@ -142,7 +142,7 @@ class DipInput:
 
 # ---------------------------------------------------------------- quadrature
 def _recur(x, nn, alf, bta, b, c):
-    """P:1383-1403."""
+    """P:1389-1409."""
     p1, p = 1.0, x + (alf - bta) / (alf + bta + 2.0)
     dp1, dp = 0.0, 1.0
     for j in range(2, nn + 1):
@@ -153,7 +153,7 @@ def _recur(x, nn, alf, bta, b, c):
 
 
 def _root(x, nn, alf, bta, b, c, eps=1e-12):
-    """Newton steps P:1365-1380; the derivative returned is the one of the
+    """Newton steps P:1370-1386; the derivative returned is the one of the
     last step's starting point."""
     it = 0
     while True:
@@ -177,7 +177,7 @@ def gauss_legendre(npot: int):
     """npot-point Gauss-Legendre points and weights as LAGPT and JACOBI
     form them (Stroud-Secrest start values, alf = bta = 0): the npot/2
     positive points largest first, then their mirror images
-    (P:1151-1186, 1274-1362).  Returns (xd, wtd, csa, tsa)."""
+    (P:1166-1204, 1311-1367).  Returns (xd, wtd, csa, tsa)."""
     alf = bta = 0.0
     nn = npot
     b = np.zeros(nn + 1)
@@ -221,7 +221,7 @@ def gauss_legendre(npot: int):
         x[i - 1] = xt
         a[i - 1] = cc / (dpn * pn1)
         csa += a[i - 1]
-    for i in range(1, nn2 + 1):                      # P:1179-1186
+    for i in range(1, nn2 + 1):                      # P:1194-1204
         j = npot + 1 - i
         x[j - 1] = -x[i - 1]
         a[j - 1] = a[i - 1]
@@ -229,12 +229,12 @@ def gauss_legendre(npot: int):
 
 
 def ipot_of(idia: int, npot: int) -> int:
-    """Angular integration points used (P:607-613)."""
+    """Angular integration points used (P:609-618)."""
     return npot if idia < 2 else npot // 2
 
 
 def dipole_grid(dipd, w: Wave, xd, wtd, nu: int, ipot: int) -> np.ndarray:
-    """LAGPT P:1187-1217: d0[j + ipot (i1 + npnt1 i2)] = wtd_j DIPD(r1_i1,
+    """LAGPT P:1206-1230: d0[j + ipot (i1 + npnt1 i2)] = wtd_j DIPD(r1_i1,
     r2_i2, x_j, nu), doubled for IDIA = 2.  dipd(r1, r2, x, nu) is the
     surface (vectorised over numpy arrays)."""
     jdia = max(1, w.idia)
@@ -246,7 +246,7 @@ def dipole_grid(dipd, w: Wave, xd, wtd, nu: int, ipot: int) -> np.ndarray:
 
 
 def asleg(lmax: int, x, m: int) -> np.ndarray:
-    """P:1220-1271: (len(x), lmax+1) array, column ll = normalised
+    """P:1234-1308: (len(x), lmax+1) array, column ll = normalised
     P_{m+ll}^m(x) including the (-1)^m phase of the recurrence."""
     x = np.asarray(x, dtype=float)
     out = np.zeros((len(x), lmax + 2))
@@ -276,7 +276,7 @@ def asleg(lmax: int, x, m: int) -> np.ndarray:
 
 # ------------------------------------------------------------ wavefunctions
 def _jstart(kz, jdia, ipar, maxleg):
-    """First l and count of the FBR functions of a block (JTRAN P:1576-1590)."""
+    """First l and count of the FBR functions of a block (JTRAN P:1578-1600)."""
     jstart, jj0 = kz, -jdia
     if jdia == 2 and jstart % 2 != ipar:
         jj0 += 1
@@ -288,7 +288,7 @@ def _jstart(kz, jdia, ipar, maxleg):
 def jtran(w: Wave, ne: int, ibase: int, jay_ipar: int, ipar1: int):
     """The first ne vectors (from ibase) of a J = 0 IWAVE file in the FBR
     basis: coef[ie, j + nang mn] = sum_a psi_ie[a, mn] pleg[jj_j, a]
-    (DSRD P:1492-1519, JTRAN P:1559-1640).  Returns (coef, kz, nang)."""
+    (DSRD P:1481-1507, JTRAN P:1564-1645).  Returns (coef, kz, nang)."""
     r = w.recs
     kz, maxleg, nidvr = (int(v) for v in _i(r[7])[:3])
     pleg = _d(r[8]).reshape(nidvr, maxleg + 1).T            # (0:maxleg, idvr)
@@ -307,7 +307,7 @@ def jtran(w: Wave, ne: int, ibase: int, jay_ipar: int, ipar1: int):
 
 def fbr_block(w: Wave, blk: int, ne: int, ibase: int):
     """Block blk (0-based) of a KVEC file: coef[ie, j] for the ne levels
-    from ibase (DSRD P:1485-1488)."""
+    from ibase (DSRD P:1478-1479)."""
     nb = int(w.nbass[blk])
     rec = _d(w.recs[w.first + blk])
     return rec.reshape(-1, nb)[ibase - 1:ibase - 1 + ne].copy()
@@ -316,7 +316,7 @@ def fbr_block(w: Wave, blk: int, ne: int, ibase: int):
 def transform(w: Wave, blk: int, ne: int, ibase: int, jay_ipar: int, ipar1: int,
               xd, ipot: int):
     """DSRD: the block's wavefunctions at the angular integration points,
-    d[ie, p + ipot mn] = sum_l coef[ie, l + nang mn] P_l(x_p) (P:1521-1545)."""
+    d[ie, p + ipot mn] = sum_l coef[ie, l + nang mn] P_l(x_p) (P:1511-1548)."""
     nrad = w.npnt1 * w.npnt2
     jdia = max(1, w.idia)
     kk = blk + 1 - w.kmin                          # the block's k
@@ -331,12 +331,12 @@ def transform(w: Wave, blk: int, ne: int, ibase: int, jay_ipar: int, ipar1: int,
         jfirst = (kk + ipar1) % 2 if w.zembed else (kk + jay_ipar) % 2
         pl = pl[:, jfirst::2][:, :nang]
     C = coef.reshape(ne, nrad, nang)
-    return np.einsum("emn,pn->emp", C, pl).reshape(ne, nrad * ipot)
+    return (C @ pl.T).reshape(ne, nrad * ipot)
 
 
 # --------------------------------------------------------------- 3-j symbol
 def binomials(nbin: int) -> np.ndarray:
-    """SETFAC P:1110-1130: binom[i, j] = C(i, j) (0-based)."""
+    """SETFAC P:1118-1138: binom[i, j] = C(i, j) (0-based)."""
     b = np.zeros((nbin, nbin))
     b[0, 0] = b[1, 0] = b[1, 1] = 1.0
     for i in range(2, nbin):
@@ -347,7 +347,7 @@ def binomials(nbin: int) -> np.ndarray:
 
 
 def threej(j1, j2, j3, m1, m2, m3, binom) -> float:
-    """P:1780-1823 (Racah's sum over the binomial table, 1-based indices
+    """P:1778-1822 (Racah's sum over the binomial table, 1-based indices
     of the reference mapped to 0-based rows/columns)."""
     B = lambda i, j: binom[i - 1, j - 1]   # noqa: E731
     if m1 + m2 + m3 != 0:
@@ -379,7 +379,7 @@ def threej(j1, j2, j3, m1, m2, m3, binom) -> float:
 class Pair:
     """One k -> k' block of DMAIN: ket block b1, bra block b2 (0-based),
     component (0: z, nu = 0; 1: x, nu = +-1), nu and the factor
-    threej * xfac * (-1)^kk1 that TRANS applies (P:1690-1692)."""
+    threej * xfac * (-1)^kk1 that TRANS applies (P:1699-1700)."""
     b1: int
     b2: int
     comp: int
@@ -389,7 +389,7 @@ class Pair:
 
 def check(ket: Wave, bra: Wave):
     """The consistency checks of INSIZE that decide what is run
-    (P:281-350, 523-544, 588-595)."""
+    (P:282-352, 485-520, 590-595)."""
     if ket.idia != bra.idia:
         raise ValueError("diatomic mismatch")
     if ket.idia <= -2:
@@ -407,7 +407,7 @@ def check(ket: Wave, bra: Wave):
     if ket.znco and bra.znco:
         raise ValueError("non-Coriolis pairs are not supported")
     if (ket.znco and abs(ket.jrot) > 1) or (bra.znco and abs(bra.jrot) > 1):
-        raise ValueError("attempt to mix no coriolis and coupled calcs")   # P:514-528
+        raise ValueError("attempt to mix no coriolis and coupled calcs")   # P:485-503
     itot = ket.jrot + bra.jrot + 1 + ket.kmin + bra.kmin
     if itot % 2 != 0 or abs(ket.jrot - bra.jrot) > 1:
         raise ValueError("selection rules violated")
@@ -415,10 +415,10 @@ def check(ket: Wave, bra: Wave):
 
 
 def schedule(ket: Wave, bra: Wave, zembed: bool, nblock: int = 1000):
-    """The k -> k' blocks of DMAIN in the reference's order (P:807-838 for
-    nu = 0, P:864-1047 for nu = +-1), each with its factor; blocks with an
+    """The k -> k' blocks of DMAIN in the reference's order (P:877-913 for
+    nu = 0, P:933-1049 for nu = +-1), each with its factor; blocks with an
     empty side are listed with factor 0 (the reference skips them but counts
-    them, P:814-817)."""
+    them, P:884-888)."""
     jk1, jk2 = ket.jk, bra.jk
     j1, j2 = jk1 - ket.kmin, jk2 - bra.kmin
     binom = binomials(ket.jrot + bra.jrot + 3)
@@ -496,7 +496,7 @@ def run(ket: Wave, bra: Wave, inp: DipInput, dipd):
 
 
 def spect(res, ezero: float = 0.0, zbisc: bool = False):
-    """SPECT P:1893-1944: per (ie1, ie2) the ket and bra energies and the
+    """SPECT P:1898-1946: per (ie1, ie2) the ket and bra energies and the
     frequency (cm^-1), z and x moments and |dipole| (debye), S(f-i)
     (debye^2), Einstein A (s^-1), and sint (a.u.) for OUTPT."""
     j1, j2 = res["j"]

@@ -157,7 +157,7 @@ def test_device_matches_oracle_on_the_same_input(tmp_path, waves, name):
 
 def test_restart_continues_the_sums(tmp_path, waves):
     """nblock = 2 then a restart (ires = 1) from the scratch file gives the
-    line strengths of one full run (RDSCR/WRSCR, P:2034-2055)."""
+    line strengths of one full run (RDSCR/WRSCR, P:2022-2041)."""
     job = _case_dir(tmp_path, waves, "k12")
     deck = open(job).read()
     first = deck.replace("zstart=.true.,", "zstart=.true., nblock=2,")
