@@ -30,7 +30,7 @@ EXPORTS = (
     "dvr_get_eigenvalues", "dvr_get_1d_eigenvalues", "dvr_get_2d_eigenvalues",
     "dvr_get_wavefunctions", "dvr_get_vectors", "dvr_get_stage_times",
     "dvr_diag", "dvr_syev_batch", "dvr_syev_batch_noalloc", "dvr_nfmain", "dvr_rotlev3",
-    "dvr_dipole3", "dvr_set_profiling",
+    "dvr_dipole3", "dvr_xpect3", "dvr_set_profiling",
     "dvr_get_profile",
     "dvr_get_solver_stats", "dvr_ldl_solve", "dvr_keep_hamiltonian", "dvr_set_preconditioner",
     "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
@@ -75,6 +75,15 @@ class DipPair(C.Structure):
 class DipStats(C.Structure):
     _fields_ = [("transform_ms", C.c_double), ("scale_ms", C.c_double),
                 ("trans_ms", C.c_double), ("ntransform", C.c_int32), ("ntrans", C.c_int32)]
+
+
+class XpDims(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("npts", "npot", "ne", "nblk", "nprop")]
+
+
+class XpBlock(C.Structure):
+    _fields_ = [("nin", C.c_int32), ("ldv", C.c_int64), ("vec", C.POINTER(C.c_double)),
+                ("ang", C.POINTER(C.c_double))]
 
 
 class Dims(C.Structure):
@@ -130,6 +139,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
                                 D]
     lib.dvr_dipole3.argtypes = [C.POINTER(DipDims), C.POINTER(DipBlock), C.POINTER(DipBlock), D,
                                 C.POINTER(DipPair), D, D, C.POINTER(DipStats)]
+    lib.dvr_xpect3.argtypes = [C.POINTER(XpDims), C.POINTER(XpBlock), D, D]
     lib.dvr_set_profiling.argtypes = [P, C.c_int32]
     lib.dvr_get_profile.argtypes = [P, C.c_int32, C.c_char_p, C.c_int32, I, D, D, D]
     lib.dvr_get_solver_stats.argtypes = [P, I, C.c_int32]
@@ -560,3 +570,41 @@ def dipole3(nrad: int, ipot: int, ket, bra, dipol, pairs, tz=None, tx=None, stat
     if isinstance(stats, dict):
         stats.update({k: getattr(st, k) for k, _ in DipStats._fields_})
     return z, x
+
+
+def xpect3(npts: int, npot: int, ne: int, blocks, prop, out=None):
+    """dvr_xpect3: XPECT3's expectation values on the device.  blocks: per k
+    block (nin, ldv, vec, ang) -- nin = 0: vec holds each level's values at
+    the npts grid points (level i at row i of an (ne, ldv) array), ang None;
+    nin > 0: vec the (ne, ldv = nin*nrad) FBR coefficients and ang the
+    (npot, nin) sqrt(w) P_l values -- or None for an empty block; prop: the
+    (npts, nprop) property values.  vec, ang and prop may be numpy arrays or
+    device tensors.  Returns (ne, nprop), added to the given start values."""
+    lib = load()
+    keep, bl = [], []
+    for b in blocks:
+        if b is None:
+            bl.append(XpBlock(0, 0, None, None))
+            continue
+        nin, ldv, vec, ang = b
+        pv, _, kv = _arr_ptr(vec)
+        keep.append(kv)
+        pa = None
+        if ang is not None:
+            if not hasattr(ang, "data_ptr"):
+                ang = np.asarray(ang, dtype=np.float64).ravel(order="F")
+            pa, _, ka = _arr_ptr(ang)
+            keep.append(ka)
+        bl.append(XpBlock(int(nin), int(ldv), pv, pa))
+    B = (XpBlock * len(bl))(*bl)
+    if not hasattr(prop, "data_ptr"):
+        prop = np.asarray(prop, dtype=np.float64).ravel(order="F")
+    pp, nprop_tot, kp = _arr_ptr(prop)
+    nprop = nprop_tot // npts
+    d = XpDims(int(npts), int(npot), int(ne), len(bl), int(nprop))
+    o = np.zeros((ne, nprop), order="F") if out is None else np.array(out, dtype=np.float64,
+                                                                      order="F")
+    rc = lib.dvr_xpect3(C.byref(d), B, pp, _dp(o.reshape(-1, order="F")))
+    if rc != DVR_OK:
+        raise DvrError(rc, "dvr_xpect3 failed")
+    return o

@@ -5,6 +5,7 @@
   dvr3d_amd/host/dvr3drjz_amd    drop-in executable (Fortran host + POTV)
   dvr3d_amd/host/rotlev3_amd     drop-in ROTLEV3 (Fortran host)
   dvr3d_amd/host/dipole3_amd     drop-in DIPOLE3 (Fortran host + example DIPD)
+  dvr3d_amd/host/xpect3_amd      drop-in XPECT3 (Fortran host + example PROP)
 
 Every call recompiles.  The oracle (reference executable) is built separately
 by oracle/Makefile.
@@ -20,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 HOST = os.path.join(HERE, "host")
 BUILD = os.path.join(HERE, "_build")
 HIP_SOURCES = ["k_grid.hip", "k_jacobi.hip", "k_syrk.hip", "k_syev.hip", "k_syev_wg.hip", "k_gemm.hip", "k_ldl.hip", "k_dav.hip", "k_mlp.hip", "dvr_davidson.hip", "dvr_mlp.hip", "dvr_refine.hip",
-               "k_syev_lds.hip", "k_twostage.hip", "dvr_nfmain.hip", "dvr_rotlev3.hip", "dvr_dipole3.hip", "dvr_noalloc.hip", "dvr_capi.hip"]
+               "k_syev_lds.hip", "k_twostage.hip", "dvr_nfmain.hip", "dvr_rotlev3.hip", "dvr_dipole3.hip", "dvr_xpect3.hip", "dvr_noalloc.hip", "dvr_capi.hip"]
 ARCH = os.environ.get("DVR_OFFLOAD_ARCH", "gfx950")
 # MFMA accumulators in ordinary VGPRs: without this the compiler keeps them
 # in AGPRs and copies them to VGPRs and back around every k slice of the GEMM
@@ -95,6 +96,13 @@ def build_host(verbose=False):
     dip = os.path.join(HOST, "dipole3_amd")
     _run(["amdflang", dip_o, api_o, dipd_o, "-L" + HERE, "-ldvr3d_amd", "-Wl,-rpath,$ORIGIN/..",
           "-Wl,-rpath," + rocm_lib, "-o", dip], verbose=verbose)
+    xp_o = os.path.join(BUILD, "xpect3_amd.o")
+    prop_o = os.path.join(BUILD, "prop_example.o")
+    _run(fc + ["-c", os.path.join(HOST, "xpect3_amd.f90"), "-o", xp_o], verbose=verbose)
+    _run(fc + ["-c", os.path.join(HOST, "prop_example.f90"), "-o", prop_o], verbose=verbose)
+    xp = os.path.join(HOST, "xpect3_amd")
+    _run(["amdflang", xp_o, api_o, prop_o, "-L" + HERE, "-ldvr3d_amd", "-Wl,-rpath,$ORIGIN/..",
+          "-Wl,-rpath," + rocm_lib, "-o", xp], verbose=verbose)
     return lib, exe
 
 

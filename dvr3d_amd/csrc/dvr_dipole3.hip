@@ -42,17 +42,6 @@ k_dip_scale(const double* __restrict__ D, const double* __restrict__ w, long K, 
 
 int grid_rows(long n) { return (int)std::min<long>((n + 255) / 256, 16384); }
 
-// true for memory the device can read in place (hipMalloc'd); host memory,
-// registered or not, is staged through device buffers instead
-bool on_device(const void* p) {
-  hipPointerAttribute_t a;
-  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return a.type == hipMemoryTypeDevice;
-}
-
 }  // namespace
 
 }  // namespace dvr

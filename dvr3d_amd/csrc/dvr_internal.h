@@ -40,6 +40,17 @@ struct HipError : std::runtime_error {
 
 #define DVR_LAUNCH_CHECK() DVR_HIP(hipGetLastError())
 
+// true for memory the device can read in place (hipMalloc'd); host memory,
+// registered or not, is staged through device buffers by the callers
+inline bool on_device(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice;
+}
+
 // ---------------------------------------------------------------------------
 // Optional per-launch timing: while a Profiler is installed (one profiled
 // dvr_run), every launch wrapper brackets its kernel with two HIP events on

@@ -49,7 +49,24 @@ module dvr3d_amd_api
     real(c_double) :: factor
   end type
 
+  type, bind(C) :: dvr_xp_dims_t
+    integer(c_int) :: npts, npot, ne, nblk, nprop
+  end type
+
+  type, bind(C) :: dvr_xp_block_t
+    integer(c_int) :: nin
+    integer(c_int64_t) :: ldv
+    type(c_ptr) :: vec, ang
+  end type
+
   interface
+    integer(c_int) function dvr_xpect3(dims, blocks, prop, out) bind(C, name="dvr_xpect3")
+      import :: c_int, c_double, dvr_xp_dims_t, dvr_xp_block_t
+      type(dvr_xp_dims_t), intent(in) :: dims
+      type(dvr_xp_block_t), intent(in) :: blocks(*)
+      real(c_double), intent(in) :: prop(*)
+      real(c_double), intent(inout) :: out(*)
+    end function
     integer(c_int) function dvr_dipole3(dims, ket, bra, dipol, pairs, tz, tx, stats) &
         bind(C, name="dvr_dipole3")
       import :: c_int, c_double, c_ptr, dvr_dip_dims_t, dvr_dip_block_t, dvr_dip_pair_t

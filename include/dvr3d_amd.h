@@ -345,6 +345,43 @@ int dvr_dipole3(const dvr_dip_dims_t* dims, const dvr_dip_block_t* ket,
                 double* tz, double* tx, dvr_dip_stats_t* stats);
 
 /*
+ * The expectation values of XPECT3 (source/xpect3.f90, cited as X:<line>),
+ * replacing the sums of X3MAIN (X:436-520): DSRD/REST's evaluation of each k
+ * block at the quadrature points (X:763-925) and TRANS's grid sums
+ * (X:998-1082):
+ *   out[i + ne*n] += sum_g prop[g + npts*n] * d_b[g, i]^2  over every block b
+ * (out: ne x nprop column-major, accumulated in place).  Block b gives
+ *   nin = 0:  vec holds each level's values at the npts grid points already
+ *             (the DVR amplitudes of a J = 0 IWAVE file: jhmain's
+ *             psi(idvr,npnt1,npnt2), or nfmain's kept angles per radial
+ *             pair), level i at vec + i*ldv, ldv >= npts;
+ *   nin > 0:  vec holds ne vectors of nin*nrad FBR coefficients (ldv =
+ *             nin*nrad, the DSTORE record of a KVEC block), ang the npot x nin
+ *             column-major values sqrt(w_p) P_l(x_p) of REST, and
+ *             d_b[p + npot*mn, i] = sum_a ang[p, a] vec[a + nin*mn + ldv*i]
+ *             with npts = nrad*npot;
+ *   vec = NULL: an empty block (skipped).
+ * prop holds the npts x nprop property values on the same grid (PROP at the
+ * points, averaged over the homonuclear exchange as X:470-500, 634-646 do).
+ * vec, ang and prop may be host or device memory; out is host memory.
+ */
+typedef struct dvr_xp_dims {
+  int32_t npts;       /* grid points of a block */
+  int32_t npot;       /* quadrature points per radial point (nin > 0 blocks) */
+  int32_t ne;         /* levels */
+  int32_t nblk;
+  int32_t nprop;
+} dvr_xp_dims_t;
+typedef struct dvr_xp_block {
+  int32_t nin;
+  int64_t ldv;
+  const double* vec;
+  const double* ang;
+} dvr_xp_block_t;
+int dvr_xpect3(const dvr_xp_dims_t* dims, const dvr_xp_block_t* blocks, const double* prop,
+               double* out);
+
+/*
  * Drop-in for the reference's SUBROUTINE diag(ham,maxham,nham,eig)
  * (D:2638-2659: dsyev('V','L',nham,ham,maxham,eig,...)).  On entry the lower
  * triangle of the column-major ham(maxham,nham) holds the symmetric matrix; on

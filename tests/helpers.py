@@ -101,3 +101,36 @@ def write_h3p_table(tmp, name):
         tab.astype("<f8").tofile(f)
     with open(os.path.join(tmp, "pes.par"), "w") as f:
         f.write("3\n" + os.path.join(str(tmp), "table.bin") + "\n")
+
+
+def write_water_potv_table(tmp):
+    """pes.par (kind 3 of the example POTV) and its table: the reference's
+    water POTV on the grid of the water J=0 deck (tests/golden/water_vgrid.npz)."""
+    from oracle import dvr3d_oracle as O
+    from oracle import nfmain_oracle as NF
+    b = NF.nf_basis(O.load_job(os.path.join(JOBS, "water_J0.job")))
+    s = b.sizes
+    V = np.load(os.path.join(GOLD, "water_vgrid.npz"))["v"]
+    G, B, A = np.meshgrid(np.arange(s.nalf), np.arange(s.npnt), np.arange(s.npnt), indexing="ij")
+    tab = np.stack([b.r[A.ravel()], b.r[B.ravel()], b.xalf[G.ravel()], V.ravel()], axis=1)
+    with open(os.path.join(tmp, "table.bin"), "wb") as f:
+        np.array([len(tab)], dtype="<f8").tofile(f)
+        tab.astype("<f8").tofile(f)
+    with open(os.path.join(tmp, "pes.par"), "w") as f:
+        f.write("3\n" + os.path.join(str(tmp), "table.bin") + "\n")
+
+
+def write_water_prop_table(tmp):
+    """prop.par (kind 3 of the example PROP) and its table: the reference's
+    water property routine (properties 1-4) at every (r_a, r_b, x) of the
+    water J=0 deck's grid (tests/golden/water_propgrid.npz)."""
+    t = np.load(os.path.join(GOLD, "water_propgrid.npz"))
+    r, x, v = t["r"], t["x"], t["v"]
+    R1, R2, X = np.meshgrid(r, r, x, indexing="ij")
+    tab = np.concatenate([np.stack([R1.ravel(), R2.ravel(), X.ravel()], axis=1),
+                          v.reshape(-1, v.shape[-1])], axis=1)
+    with open(os.path.join(tmp, "ptable.bin"), "wb") as f:
+        np.array([len(tab), v.shape[-1]], dtype="<f8").tofile(f)
+        tab.astype("<f8").tofile(f)
+    with open(os.path.join(tmp, "prop.par"), "w") as f:
+        f.write("3\n" + os.path.join(str(tmp), "ptable.bin") + "\n")
