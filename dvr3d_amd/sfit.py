@@ -2,22 +2,22 @@
 cited as S:<line>) in one process on the GPU.
 
 The reference's SFIT runs the DVR3DRJZ, ROTLEV3B and XPECT3 executables
-through system calls at every iteration and reads their files back (S:723-
-860); the energy derivatives with respect to the potential parameters are
+through system calls at every iteration and reads their files back (S:741-
+880); the energy derivatives with respect to the potential parameters are
 Hellmann-Feynman expectation values <psi_n | dV/dp_j | psi_n> computed by
-XPECT3 with a property routine that returns dV/dp_j (S:836-878).  Here one
+XPECT3 with a property routine that returns dV/dp_j (S:825-910).  Here one
 iteration is one dvr_run (potential, 1D/2D/3D stages, levels and
 wavefunctions, all in HBM) and one dvr_xpect3 call (every level against
 every parameter derivative on the DVR grid as one GEMM), and the normal
-equations of the weighted least-squares step are those of S:1035-1075:
+equations of the weighted least-squares step are those of S:1060-1130:
 
     eps_n  = obs_n - (E_n - E_0)                       (term values, cm^-1)
     J_nj   = d(E_n - E_0)/dp_j = <dV/dp_j>_n - <dV/dp_j>_0      (S:902)
     A = J^T W J,  b = J^T W eps,  dp = lstsq(A, b)  (dgelss, rcond 1e-12)
 
-with W the normalised weights (S:590-607).  The loop stops when the weighted
+with W the normalised weights (S:593-606).  The loop stops when the weighted
 standard deviation or its relative change falls below the reference's
-thresholds (stadev_best = 1e-4, stab_best = 1e-12, S:26) or after itmax
+thresholds (stadev_best = 1e-4, stab_best = 1e-12, S:23) or after itmax
 iterations.  J = 0 levels of one parity (the vibrational fit); the surface is
 any callable with parameter derivatives, e.g. AB2Surface.
 """
@@ -30,8 +30,8 @@ import numpy as np
 from dvr3d_amd import _lib, host
 
 AUTOCM = 2.19474624e05
-STADEV_BEST = 1e-4        # S:26
-STAB_BEST = 1e-12         # S:26
+STADEV_BEST = 1e-4        # S:23
+STAB_BEST = 1e-12         # S:23
 
 
 class AB2Surface:
@@ -90,7 +90,7 @@ class VibFit:
             raise ValueError("observed levels are term values above level 0")
         self.obs = np.array([obs[n] for n in self.levels], dtype=float)
         w = np.ones(len(self.levels)) if weights is None else np.asarray(weights, dtype=float)
-        self.w = w / w.sum()                                  # S:590-607
+        self.w = w / w.sum()                                  # S:593-606
         self.nvec = int(self.levels.max()) + 1
         sz = b.sz
         if sz.neval < self.nvec:
@@ -134,8 +134,8 @@ class VibFit:
         return FitStep(p.copy(), calc, eps, stadev, rms, J)
 
     def fit(self, itmax: int = 10):
-        """Gauss-Newton iterations of the weighted normal equations (S:1035-
-        1107); returns the list of FitStep (the last one at the final
+        """Gauss-Newton iterations of the weighted normal equations (S:1060-
+        1130); returns the list of FitStep (the last one at the final
         parameters, not followed by an update)."""
         p = self.params.copy()
         hist = []
