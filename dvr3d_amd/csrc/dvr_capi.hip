@@ -418,6 +418,7 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.ldl_xcd = env_int("DVR_LDL_XCD", 1) != 0;
     C->dav.ldl_sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
     C->dav.ldl_cp = env_int("DVR_LDL_CP", 1) != 0;
+    C->dav.ldl_wide = env_int("DVR_LDL_WIDE", 32);
     if (const char* t = getenv("DVR_DAV_EARLY")) C->dav.early_tol = atof(t);
     C->dav.first_overlap = std::max(0, env_int("DVR_DAV_FIRST", 0));
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
@@ -919,6 +920,7 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
       f.lookahead = env_int("DVR_LDL_LOOKAHEAD", 1) != 0;
       f.sb = std::max(64, env_int("DVR_LDL_SB", 2048) / 64 * 64);
       f.upd_cp = env_int("DVR_LDL_CP", 1) != 0;
+      f.upd_wide = env_int("DVR_LDL_WIDE", 32);
       f.ensure(n);
       const int nu = f.factor(H.as<double>(), sigma, s);
       if (inertia) *inertia = nu;

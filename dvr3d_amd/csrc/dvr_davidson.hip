@@ -241,6 +241,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
     f.panel_sub = ldl_sub;
     f.upd_xcd = ldl_xcd;
     f.upd_cp = ldl_cp;
+    f.upd_wide = ldl_wide;
     f.tuner = &tuner;
   };
   setup(ldl);

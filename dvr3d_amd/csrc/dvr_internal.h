@@ -549,6 +549,14 @@ void launch_gemm_var(hipStream_t s, const char* name, bool ta, bool tb, int M, i
                      double alpha, const double* A, long lda, const double* B, long ldb,
                      double beta, double* C, long ldc, bool lower, int kt, bool cp,
                      bool xcd_lower = false);
+// Lower-tile NT update C = alpha A B^T + beta C (M x M, lower 64 x 64 tiles)
+// on the persistent wavefront-tile kernel with wgs_per_xcd work-groups per
+// XCD (one wavefront per SIMD each); M < 64 or K other than 64, 128, 256 on
+// the 64 x 64 kernel (launch name + "_small").  Same bits as launch_gemm_var
+// with tb and lower.
+void launch_gemm_lower_nt(hipStream_t s, const char* name, int M, int K, double alpha,
+                          const double* A, long lda, const double* B, long ldb, double beta,
+                          double* C, long ldc, int wgs_per_xcd, bool xcd_lower);
 // split of K that gives about a thousand work-groups for small outputs
 int gemm_auto_ksplit(int M, int N, int K);
 
@@ -604,6 +612,8 @@ struct LdlFactor {
   int upd_kt = 16;              // trailing-update GEMM: k-slice depth, C read at the start,
   bool upd_cp = true;           // XCD-aware tile order
   bool upd_xcd = true;
+  int upd_wide = 32;            // trailing update on the persistent wavefront-tile kernel with
+                                // this many work-groups per XCD; 0: the 64 x 64 kernel
   GemmTuner* tuner = nullptr;   // per-shape GEMM choices (nullptr: fixed rule)
   hipStream_t aux = nullptr;    // trailing updates overlapped with the next panel
   hipEvent_t ev_panel = nullptr, ev_trail = nullptr;
@@ -692,6 +702,7 @@ class DavidsonSolver {
   bool ldl_xcd = true;         // XCD-aware tile order of the trailing updates
   int ldl_sb = 2048;           // super-block of the shift-and-invert triangular solves
   bool ldl_cp = true;          // trailing updates read C before the k loop
+  int ldl_wide = 32;           // trailing updates on the persistent kernel (work-groups per XCD; 0 off)
   // residual norm below which the Ritz pairs up to one past the top of a
   // slice fix the next shift: its factorisation then starts on a second
   // stream while the slice converges (0: after the slice, on the same stream)

@@ -411,9 +411,13 @@ void LdlFactor::factor_begin(const double* H, double sigma, hipStream_t s) {
       hipStream_t ts = lookahead ? aux : s;
       DVR_HIP(hipEventRecord(ev_panel, s));
       DVR_HIP(hipStreamWaitEvent(aux, ev_panel, 0));
-      launch_gemm_var(ts, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda,
-                      lda, W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, true, upd_kt, upd_cp,
-                      upd_xcd);
+      if (upd_wide > 0)
+        launch_gemm_lower_nt(ts, "ldl_update", m2, K, -1.0, a + r3 + (long)p0 * lda, lda, W + r3,
+                             n, 1.0, a + r3 + (long)r3 * lda, lda, upd_wide, upd_xcd);
+      else
+        launch_gemm_var(ts, "ldl_update", false, true, m2, m2, K, -1.0, a + r3 + (long)p0 * lda,
+                        lda, W + r3, n, 1.0, a + r3 + (long)r3 * lda, lda, true, upd_kt, upd_cp,
+                        upd_xcd);
       DVR_HIP(hipEventRecord(ev_trail, aux));
       trail_pending = true;
     }

@@ -3,17 +3,20 @@
 Reads the CSV output of five rocprofv3 runs (made with `--output-format csv`):
 
   fetch, write    --pmc FETCH_SIZE / --pmc WRITE_SIZE over `bench.py --ldl-probe N`
-                  (one LDL^T factorisation of an N x N matrix), kernels k_gemm only
+                  (one LDL^T factorisation of an N x N matrix), kernel
+                  k_gemm_lower_w64 only
   copy_fetch,     the same counters over `bench.py --copy-probe` (three 512 MiB
   copy_write      device copies): FETCH_SIZE / WRITE_SIZE of a copy of known size
                   give the factors that turn each counter into bytes
   trace           --kernel-trace over the same ldl probe (durations)
 
-and writes the per-launch figures of the trailing update (the lower-tile
-k_gemm launches) as JSON: algorithmic bytes (the two panels read,
-8 * 2 * m * K, plus the lower triangle of A22 read and written, 8 * m^2, with
-m = N - 2 PW - PW i for launch i and K = PW = 256), measured bytes, their
-ratio and the average duration.
+and writes the per-launch figures of the trailing update (every dispatch of
+the persistent lower-tile kernel k_gemm_lower_w64 is one, in factorisation
+order; the last, under 64 rows, runs on the 64 x 64 kernel) as JSON:
+algorithmic bytes (the two panels read, 8 * 2 * m * K, plus the lower
+triangle of A22 read and written, 8 * m^2, with m = N - 2 PW - PW i for
+launch i and K = PW = 256), measured bytes, their ratio and the average
+duration.
 
 usage: python -m dvr3d_amd.traffic --n 20000 --fetch D1 --write D2 \\
            --copy-fetch D3 --copy-write D4 --trace D5 --out profiles/ldl_update_traffic.json
@@ -38,48 +41,24 @@ def _rows(d, pattern):
         return list(csv.DictReader(f))
 
 
-def _expected_wgs(n, i):
-    """Work-groups of the i-th lower-tile update (m = n - 2 PW - PW i rows):
-    one per lower tile, or, in the XCD-aware order, 64 per 8 x 8 super-tile
-    rounded up to a multiple of 8 super-tiles."""
-    m = n - 2 * PW - PW * i
-    if m <= 0:
-        return (-1, -1)
-    nt = (m + TILE - 1) // TILE
-    ns = (nt + 7) // 8
-    return (nt * (nt + 1) // 2, 8 * 64 * ((ns * (ns + 1) // 2 + 7) // 8))
+UPDATE_KERNEL = "k_gemm_lower_w64"
 
 
-def _updates(dispatches, n):
-    """Pick the lower-tile updates out of the NT GEMM dispatches (in dispatch
-    order): the trailing updates and the next-panel updates share the kernel,
-    the work-group count tells them apart.  A run of several factorisations
-    restarts the sequence."""
-    out, i = [], 0
-    for did, wg, val in dispatches:
-        if wg in _expected_wgs(n, i):
-            out.append((did, val))
-            i += 1
-        elif wg in _expected_wgs(n, 0):
-            out.append((did, val))
-            i = 1
-    return out
+def _rows_m(n, nl):
+    """Rows of launch i's trailing block."""
+    return [n - 2 * PW - PW * i for i in range(nl)]
 
 
-UPDATE_KERNEL = "k_gemm<false, true, 16, true>"
-
-
-def _counter(d, counter, n):
+def _counter(d, counter):
     """Per-dispatch values of one counter for the trailing updates, in
     dispatch order."""
-    vals, wgs = {}, {}
+    vals = {}
     for r in _rows(d, "*counter_collection.csv"):
         if r["Counter_Name"] != counter or UPDATE_KERNEL not in r["Kernel_Name"]:
             continue
         did = int(r["Dispatch_Id"])
         vals[did] = vals.get(did, 0.0) + float(r["Counter_Value"])
-        wgs[did] = int(r["Grid_Size"]) // 256
-    return [v for _, v in _updates([(k, wgs[k], vals[k]) for k in sorted(vals)], n)]
+    return [vals[k] for k in sorted(vals)]
 
 
 def _copy_kib(d, counter):
@@ -99,20 +78,15 @@ def main(argv=None):
     ap.add_argument("--n", type=int, required=True)
     ap.add_argument("--copy-bytes", type=int, default=512 << 20)
     a = ap.parse_args(argv)
-    fetch = _counter(a.fetch, "FETCH_SIZE", a.n)
-    write = _counter(a.write, "WRITE_SIZE", a.n)
+    fetch = _counter(a.fetch, "FETCH_SIZE")
+    write = _counter(a.write, "WRITE_SIZE")
     # counters are in KiB; a copy of B bytes reads and writes B bytes
     f_scale = a.copy_bytes / 1024.0 / _copy_kib(a.copy_fetch, "FETCH_SIZE")
     w_scale = a.copy_bytes / 1024.0 / _copy_kib(a.copy_write, "WRITE_SIZE")
-    tr = []
-    for r in _rows(a.trace, "*kernel_trace.csv"):
-        if UPDATE_KERNEL in r["Kernel_Name"]:
-            wg = int(r["Grid_Size_X"]) * int(r.get("Grid_Size_Y") or 1) // 256
-            tr.append((int(r["Dispatch_Id"]), wg,
-                       int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
-    durs = [v for _, v in _updates(sorted(tr), a.n)]
+    durs = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            for r in _rows(a.trace, "*kernel_trace.csv") if UPDATE_KERNEL in r["Kernel_Name"]]
     nl = min(len(fetch), len(write))
-    ms = [a.n - 2 * PW - PW * i for i in range(nl)]
+    ms = _rows_m(a.n, nl)
     alg = [8.0 * 2 * m * PW + 8.0 * m * m for m in ms]
     fb = [v * 1024.0 * f_scale for v in fetch[:nl]]
     wb = [v * 1024.0 * w_scale for v in write[:nl]]

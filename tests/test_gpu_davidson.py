@@ -71,6 +71,32 @@ def test_ldl_panel_widths_agree(monkeypatch):
     assert nu0 == nu1 == 601 and np.array_equal(tx0, tx1)
 
 
+@pytest.mark.parametrize("n", [200, 1700, 2203, 4133])
+def test_ldl_persistent_update_same_bits(monkeypatch, n):
+    """The trailing update on the persistent wavefront-tile kernel (32, 28 and
+    5 work-groups per XCD; K = 256, 128, 64 from panels of 4, 2, 1
+    sub-panels; its M % 64 ragged rows on the 64 x 64 kernel) sums in the
+    order of the 64 x 64 kernel: inertia (LAPACK's count) and solves
+    bit-identical to DVR_LDL_WIDE=0.  (The shift sits mid-spectrum, where
+    the unpivoted factorisation has element growth: the accuracy of the
+    solves is tested on the cases above and below.)"""
+    from dvr3d_amd import _lib
+    rng = np.random.default_rng(n + 3)
+    a = _spd_like(n, rng)
+    w = np.linalg.eigvalsh(a)
+    sigma = 0.5 * (w[n // 2] + w[n // 2 + 1])
+    r = rng.standard_normal((n, 3))
+    for sub in ("4", "2", "1"):
+        monkeypatch.setenv("DVR_LDL_SUB", sub)
+        monkeypatch.setenv("DVR_LDL_WIDE", "0")
+        nu0, t0 = _lib.ldl_solve(a, sigma, r)
+        assert nu0 == n // 2 + 1
+        for wgs in ("32", "28", "5"):
+            monkeypatch.setenv("DVR_LDL_WIDE", wgs)
+            nu, t = _lib.ldl_solve(a, sigma, r)
+            assert nu == nu0 and np.array_equal(t, t0), (sub, wgs)
+
+
 @pytest.mark.parametrize("sb", ["512", "1024", "2048"])
 def test_ldl_super_block_sizes(monkeypatch, sb):
     """Triangular solves over super-blocks of 512 / 1024 / 2048 rows (the
