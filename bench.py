@@ -48,6 +48,10 @@ CONFIGS = {
 }
 C4_LEVELS = 300
 C4_TRAIN = 32        # c4 instances solved densely and used to train the learned preconditioner
+C4_ABL = 8           # learned-part instances also solved without the network
+ABL_TRAIN = 2        # perturbed potentials the ablation's network is trained on
+ABL_NVEC = 300       # eigenpairs of each (training pairs)
+ABL_FIRST = 6        # iterations beside the first factorisation in the ablation
 TOL_CM = 1e-3
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
@@ -254,6 +258,83 @@ def roofline_of(kprof):
     return roof
 
 
+def learned_ablation(cfg, job, b, xa, par, gold, main_run, args):
+    """The learned part of the chosen form, with and without: the default
+    solve (shift-and-invert from the first iteration, the first factorisation
+    before it) against the same solve whose first ABL_FIRST iterations run
+    beside the first factorisation with (a) the classical diagonal correction
+    and (b) the learned one (DVR_PREC_SI_LEARNED), the network trained on the
+    GPU on the dense path's eigenpairs of ABL_TRAIN seeded perturbations of
+    this configuration's potential (every parameter scaled by 1 + N(0, 0.01)).
+    One warm-up and one timed solve per variant; levels against the same
+    reference as the main line."""
+    import math
+    from dvr3d_amd import _lib, host
+    na = b.sz.npnta
+
+    def context(solver, nvec):
+        ctx = _lib.Context(b.dims(nvec=nvec, solver=solver))
+        ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, None,
+                      job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
+        return ctx
+
+    # potential parameters back in degrees for the perturbation
+    base = par.copy()
+    base[4] = math.acos(par[4]) * 180.0 / math.pi
+    progress(f"{cfg} ablation: training the preconditioner on {ABL_TRAIN} perturbed potentials")
+    dense = context(_lib.DVR_SOLVER_DENSE, min(b.sz.neval, ABL_NVEC))
+    dense.keep_hamiltonian(True)
+    mlp = _lib.mlp_init(0)
+    t0 = time.perf_counter()
+    for i in range(ABL_TRAIN):
+        p = base * perturbed_par(base, 2000 + i)
+        p[4] = math.cos(p[4] * math.pi / 180.0)
+        dense.eval_potential_ab2(b.rc, b.rb, xa, p)
+        dense.run()
+        mlp, hist = dense.mlp_train(mlp, steps=args.train_steps, batch=32, seed=i, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    del dense
+    out = {"first_iterations": ABL_FIRST, "training_instances": ABL_TRAIN,
+           "training_s": round(t_train, 2), "final_training_loss": float(hist[-1])}
+
+    def row(run):
+        st = run["solver_stats"]
+        return {"ms_per_solve": round(run["ms_per_step"], 3),
+                "converged": run["converged"], "max_err_cm": run["max_err_cm"],
+                "iterations": st["iterations"], "matvecs": st["matvecs"],
+                "factor_exposed_ms": round(st["factor_us"] / 1e3, 3),
+                "dense_fallback": st["dense_fallback"]}
+
+    out["shift_invert"] = row(main_run)
+    for name, kind in (("diagonal_first", _lib.PREC_SHIFT_INVERT),
+                       ("learned_first", _lib.PREC_SI_LEARNED)):
+        progress(f"{cfg} ablation: {name}")
+        old = os.environ.get("DVR_DAV_FIRST")
+        os.environ["DVR_DAV_FIRST"] = str(ABL_FIRST)
+        try:
+            ctx = context(_lib.DVR_SOLVER_DAVIDSON, 0)
+            ctx.set_preconditioner(kind, mlp if kind == _lib.PREC_SI_LEARNED else None)
+            for _ in range(2):
+                t0 = time.perf_counter()
+                ctx.eval_potential_ab2(b.rc, b.rb, xa, par)
+                ctx.run()
+                dt = time.perf_counter() - t0
+        finally:
+            if old is None:
+                del os.environ["DVR_DAV_FIRST"]
+            else:
+                os.environ["DVR_DAV_FIRST"] = old
+        ev = ctx.eigenvalues()
+        run = {"ms_per_step": dt * 1e3, "solver_stats": ctx.solver_stats(),
+               "converged": 0, "max_err_cm": None}
+        if gold is not None and len(gold) == len(ev):
+            err = np.abs(ev * host.AUTOCM - gold)
+            run["converged"], run["max_err_cm"] = int((err <= TOL_CM).sum()), float(err.max())
+        out[name] = row(run)
+        del ctx
+    return out
+
+
 def perturbed_par(base, seed):
     """PES parameters (De, a, re, kb, theta_e, c) each scaled by 1 + N(0, 0.01)."""
     rng = np.random.default_rng(seed)
@@ -300,9 +381,12 @@ def sweep(args):
     dense path with eigenvectors, and each one's Hamiltonian and eigenpairs
     train the learned preconditioner on the GPU (dvr_mlp_train, continuing
     from the previous instance's parameters); the remaining instances run
-    block-Davidson with that learned preconditioner (with the dense path as
-    fallback when it stalls).  Levels of two instances of each part are
-    checked against the FP64 oracle's (tests/golden/c4_oracle.json)."""
+    the chosen form, block-Davidson with shift-and-invert whose first
+    iterations use the learned correction while the first shift is factored
+    (DVR_PREC_SI_LEARNED; the dense path as fallback when it stalls), and the
+    first C4_ABL of them again with shift-and-invert alone (the ablation).
+    Levels of two instances of each part are checked against the FP64
+    oracle's (tests/golden/c4_oracle.json)."""
     from dvr3d_amd import _lib, host
     os.environ.setdefault("DVR_DAV_MAXIT", "100")
     b, xa, context, params = c4_setup()
@@ -329,7 +413,7 @@ def sweep(args):
             errs[str(i)] = float(np.abs(dense.eigenvalues() - np.array(gold[str(i)])).max()
                                  * host.AUTOCM)
     learned = context(_lib.DVR_SOLVER_DAVIDSON, 0)
-    learned.set_preconditioner(_lib.PREC_LEARNED, mlp)
+    learned.set_preconditioner(_lib.PREC_SI_LEARNED, mlp)
     t_dav, stats = 0.0, []
     for i in range(ntrain, n):
         t0 = time.perf_counter()
@@ -341,6 +425,18 @@ def sweep(args):
             errs[str(i)] = float(np.abs(learned.eigenvalues() - np.array(gold[str(i)])).max()
                                  * host.AUTOCM)
     napp = n - ntrain
+    # ablation: the first C4_ABL instances of the learned part again with
+    # shift-and-invert alone (no network), same contexts' settings
+    plain_si = context(_lib.DVR_SOLVER_DAVIDSON, 0)
+    nabl = min(C4_ABL, napp)
+    t_si, stats_si = 0.0, []
+    for i in range(ntrain, ntrain + nabl):
+        t0 = time.perf_counter()
+        plain_si.eval_potential_ab2(b.rc, b.rb, xa, params(i))
+        plain_si.run()
+        t_si += time.perf_counter() - t0
+        stats_si.append(plain_si.solver_stats())
+    stats_abl = stats[:nabl]
     print(json.dumps({
         "metric": "solves/s (c4 sweep)",
         "value": round(n / (t_solve + t_train + t_dav), 3), "unit": "solves/s",
@@ -351,13 +447,36 @@ def sweep(args):
                   "train_steps_per_instance": args.train_steps,
                   "final_training_loss": losses[-1] if losses else None},
         "learned_davidson": {
+            "preconditioner": "shift-and-invert, learned corrections in the first "
+                              f"{os.environ.get('DVR_DAV_FIRST', '6')} iterations beside the "
+                              "first factorisation",
             "instances": napp,
             "ms_per_solve": round(t_dav / napp * 1e3, 3) if napp else None,
             "converged_without_fallback": sum(1 for st in stats if st["dense_fallback"] == 0),
             "dense_fallbacks": sum(1 for st in stats if st["dense_fallback"] != 0),
             "mean_iterations": (round(float(np.mean([st["iterations"] for st in stats])), 1)
                                 if stats else None),
-            "iteration_cap": int(os.environ["DVR_DAV_MAXIT"])},
+            "mean_matvecs": (round(float(np.mean([st["matvecs"] for st in stats])), 1)
+                             if stats else None),
+            "iteration_cap": int(os.environ["DVR_DAV_MAXIT"]),
+            # (instance, solver status: 2 stalled at the cap, 3 certification
+            # failed, 4 error; iterations) of every instance handed to dense
+            "fallbacks": [[ntrain + k, st["status"], st["iterations"]]
+                          for k, st in enumerate(stats) if st["dense_fallback"] != 0]},
+        "ablation_shift_invert_only": {
+            "instances": nabl,
+            "ms_per_solve": round(t_si / nabl * 1e3, 3) if nabl else None,
+            "mean_iterations": (round(float(np.mean([st["iterations"] for st in stats_si])), 1)
+                                if stats_si else None),
+            "mean_matvecs": (round(float(np.mean([st["matvecs"] for st in stats_si])), 1)
+                             if stats_si else None),
+            "learned_mean_iterations_same_instances": (
+                round(float(np.mean([st["iterations"] for st in stats_abl])), 1)
+                if stats_abl else None),
+            "learned_mean_matvecs_same_instances": (
+                round(float(np.mean([st["matvecs"] for st in stats_abl])), 1)
+                if stats_abl else None),
+            "dense_fallbacks": sum(1 for st in stats_si if st["dense_fallback"] != 0)},
         "max_err_cm_per_checked_instance": errs}))
 
 
@@ -746,6 +865,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain FP64 dense path run beside the chosen form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the chosen form's learned-preconditioner ablation")
     ap.add_argument("--rot-nvib", type=int, default=1000,
                     help="rot: vibrational functions per k block")
     ap.add_argument("--rot-neval", type=int, default=500,
@@ -861,6 +982,9 @@ def main():
     progress(f"{cfg}: {args.solver} solve, {args.warmup} warm-up + {args.steps} timed steps")
     main_run = measure(args.solver, args.steps, args.warmup, True)
     progress(f"{cfg}: {args.solver} {main_run['ms_per_step']:.1f} ms per step")
+    ablation = None
+    if args.solver == "davidson" and not args.no_ablation:
+        ablation = learned_ablation(cfg, job, b, xa, par, gold, main_run, args)
     plain = None
     if args.solver == "davidson" and args.plain_steps > 0:
         plain = measure("dense", args.plain_steps, 1, False)
@@ -889,6 +1013,8 @@ def main():
         "kernels": {k: {"launches": v["launches"], "ms": round(v["seconds"] * 1e3, 3)}
                     for k, v in sorted(kprof.items(), key=lambda kv: -kv[1]["seconds"])[:25]},
     }
+    if ablation is not None:
+        line["config"]["ablation_learned"] = ablation
     if plain is not None:
         line["config"]["plain_fp64"] = {
             "ms_per_step": round(plain["ms_per_step"], 3),

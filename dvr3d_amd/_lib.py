@@ -36,7 +36,7 @@ EXPORTS = (
     "dvr_mlp_train", "dvr_mlp_forward", "dvr_mlp_backward",
 )
 MLP_NPARAM = 17537
-PREC_SHIFT_INVERT, PREC_DIAGONAL, PREC_LEARNED = 0, 1, 2
+PREC_SHIFT_INVERT, PREC_DIAGONAL, PREC_LEARNED, PREC_SI_LEARNED = 0, 1, 2, 3
 SOLVER_STATS = ("status", "iterations", "matvecs", "solves", "factorizations", "slices",
                 "certified", "missing", "dense_fallback", "rr_us", "si_us", "expand_us",
                 "factor_us", "start_us", "twostage")

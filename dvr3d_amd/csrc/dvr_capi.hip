@@ -420,13 +420,16 @@ void run_pipeline(dvr_ctx_t* C) {
     C->dav.ldl_cp = env_int("DVR_LDL_CP", 1) != 0;
     C->dav.ldl_wide = env_int("DVR_LDL_WIDE", 32);
     if (const char* t = getenv("DVR_DAV_EARLY")) C->dav.early_tol = atof(t);
-    C->dav.first_overlap = std::max(0, env_int("DVR_DAV_FIRST", 0));
+    C->dav.force_ambiguous = std::max(0, env_int("DVR_DAV_FORCE_AMBIG", 0));
+    // kind 3 needs iterations beside the first factorisation to be used in
+    C->dav.first_overlap =
+        std::max(0, env_int("DVR_DAV_FIRST", C->prec_kind == DVR_PREC_SI_LEARNED ? 6 : 0));
     P.slice = env_int("DVR_DAV_SLICE", P.slice);
     P.mmax = env_int("DVR_DAV_MMAX", P.mmax);
     P.keep = env_int("DVR_DAV_KEEP", P.keep);
     P.max_iter = env_int("DVR_DAV_MAXIT", 2000);
     P.prec = C->prec_kind;
-    if (P.prec == 2) {
+    if (P.prec == DVR_PREC_LEARNED || P.prec == DVR_PREC_SI_LEARNED) {
       if (!C->mlp_set) throw std::runtime_error("learned preconditioner selected without parameters");
       P.mlp = C->mlpP.as<float>();
       P.st = C->st3.as<double>();
@@ -786,11 +789,12 @@ int dvr_keep_hamiltonian(dvr_ctx_t* c, int32_t on) {
 }
 
 int dvr_set_preconditioner(dvr_ctx_t* c, int32_t kind, const float* params) {
-  if (!c || kind < 0 || kind > 2 || (kind == 2 && !params)) return DVR_ERR_ARG;
+  const bool learned = kind == DVR_PREC_LEARNED || kind == DVR_PREC_SI_LEARNED;
+  if (!c || kind < 0 || kind > 3 || (learned && !params)) return DVR_ERR_ARG;
   try {
     DeviceGuard g(c->device);
     c->prec_kind = kind;
-    if (kind == 2) {
+    if (learned) {
       c->mlpP.ensure(sizeof(float) * MLP_NPARAM);
       DVR_HIP(hipMemcpy(c->mlpP.p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
       c->mlp_set = true;

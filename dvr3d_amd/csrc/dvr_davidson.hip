@@ -298,15 +298,24 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
     DVR_HIP(hipStreamSynchronize(s));
   };
 
-  const bool si = P.prec == 0;
-  if (!si) {
+  // kind 3: shift-and-invert, with learned corrections in the iterations
+  // that run while the first factorisation is on the second stream
+  const bool si = P.prec == 0 || P.prec == 3;
+  const bool learned = P.prec == 2 || P.prec == 3;
+  if (learned) {
     // Dd already holds the diagonal
-    if (P.prec == 2) {
-      if (!P.mlp || !P.st) throw HipError("davidson: learned preconditioner without parameters");
-      Fd.ensure(sizeof(float) * (size_t)N * b * MLP_NF);
-      Wd.ensure(sizeof(float) * (size_t)N * b);
-    }
+    if (!P.mlp || !P.st) throw HipError("davidson: learned preconditioner without parameters");
+    Fd.ensure(sizeof(float) * (size_t)N * b * MLP_NF);
+    Wd.ensure(sizeof(float) * (size_t)N * b);
   }
+  // inertia of a factorisation as the solver sees it; with force_ambiguous
+  // = k > 0 the k-th count of the solve is reported ambiguous (-1), to test
+  // the retries at nudged shifts
+  int ncount = 0;
+  auto inertia = [&](int nu) {
+    ++ncount;
+    return (force_ambiguous > 0 && ncount == force_ambiguous) ? -1 : nu;
+  };
   // first shift (shift-and-invert only): the lowest Ritz value of the start
   // block; the diagonal preconditioners factor only to certify slices
   if (si) {
@@ -327,14 +336,28 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
       clock.mark(PH_FACTOR, s);
       // a zero or non-finite pivot (shift on an eigenvalue): the solves
       // would not be usable
-      if (cur->factor(H, sigma, s) < 0) return 3;
+      // a near-zero pivot (ambiguous inertia): the shift a little lower
+      int nu0 = inertia(cur->factor(H, sigma, s));
+      for (int t = 0; nu0 < 0 && t < 2; ++t) {
+        sigma -= 1e-6 * (std::fabs(sigma) + 1.0);
+        nu0 = inertia(cur->factor(H, sigma, s));
+        stats.factorizations++;
+      }
+      if (nu0 < 0) return 3;
     }
   }
   // every later use of the first factor on s comes after this
   bool first_bad = false;
   auto first_done = [&]() {
     if (!first_pending) return;
-    first_bad = cur->factor_end(s2) < 0;
+    int nu0 = inertia(cur->factor_end(s2));
+    // ambiguous inertia (near-zero pivot): the shift a little lower, on s
+    for (int t = 0; nu0 < 0 && t < 2; ++t) {
+      sigma -= 1e-6 * (std::fabs(sigma) + 1.0);
+      nu0 = inertia(cur->factor(H, sigma, s));
+      stats.factorizations++;
+    }
+    first_bad = nu0 < 0;
     first_pending = false;
   };
 
@@ -346,7 +369,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
     // then takes every level below it and ends when their number equals the
     // inertia nu_e
     bool early = false, launch_e = false;
-    double sig_e = 0.0;
+    double sig_e = 0.0, gap_e = 0.0;
     int nu_e = -1;
     auto found_below = [&](double sg) {
       int f = 0;
@@ -377,7 +400,15 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
         if (!(nl < m && th[nl] < sig_e) || nlocked >= nlmax) {
           if (nu_e < 0) {
             clock.mark(PH_FACTOR, s);
-            nu_e = nxt->factor_end(s2);
+            nu_e = inertia(nxt->factor_end(s2));
+            // a near-zero pivot makes the count ambiguous: the shift a
+            // quarter of the gap higher, still between the same two Ritz
+            // values as when it was placed, factored on s
+            for (int t = 0; nu_e < 0 && t < 2; ++t) {
+              sig_e += 0.25 * gap_e;
+              nu_e = inertia(nxt->factor(H, sig_e, s));
+              stats.factorizations++;
+            }
             first_pending = false;   // s2 has drained
             // no trustworthy inertia at this shift: certification failed
             if (nu_e < 0) return 3;
@@ -396,6 +427,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
           for (int j = nl; j <= nl + r; ++j) ok = ok && nrm[j] <= early_tol;
           if (ok) {
             sig_e = 0.5 * (th[nl + r - 1] + th[nl + r]);
+            gap_e = th[nl + r] - th[nl + r - 1];
             early = launch_e = true;
           }
         }
@@ -443,7 +475,7 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
         for (int c = 0; c < na; ++c) ta[c] = th[act[c]];
         const double* thp = upload(tha, ta, s);
         const float* wp = nullptr;
-        if (P.prec == 2) {
+        if (learned) {
           launch_mlp_features(s, Dd.as<double>(), P.st, N, na, thp, thp, Fd.as<float>());
           launch_mlp_infer_bf16(s, P.mlp, (long)N * na, Fd.as<float>(), Wd.as<float>());
           wp = Wd.as<float>();
@@ -490,11 +522,17 @@ int DavidsonSolver::solve_impl(const double* H, int n, const DavParams& P, hipSt
     }
     if (!early) {
       const double top = *std::max_element(thL.begin(), thL.end());
-      const double sig_hi = have_next ? 0.5 * (top + th_next)
+      double sig_hi = have_next ? 0.5 * (top + th_next)
                                       : top + 1e-6 * (std::fabs(top) + 1.0);
       clock.mark(PH_FACTOR, s);
       first_done();
-      nu = cur->factor(H, sig_hi, s);
+      nu = inertia(cur->factor(H, sig_hi, s));
+      // ambiguous inertia (near-zero pivot): a quarter of the gap higher
+      for (int t = 0; nu < 0 && t < 2; ++t) {
+        sig_hi += have_next ? 0.25 * (th_next - top) : 1e-6 * (std::fabs(top) + 1.0);
+        nu = inertia(cur->factor(H, sig_hi, s));
+        stats.factorizations++;
+      }
       stats.factorizations++;
       if (nu < 0) return 3;
       sigma = sig_hi;

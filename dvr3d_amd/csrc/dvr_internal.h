@@ -710,6 +710,9 @@ class DavidsonSolver {
   // iterations with diagonal corrections while the first factorisation runs
   // on the second stream (needs early_tol > 0; 0: factored before them)
   int first_overlap = 0;
+  // k > 0: the k-th inertia count of a solve is reported ambiguous (tests of
+  // the retries at nudged shifts; 0 in use)
+  int force_ambiguous = 0;
   ~DavidsonSolver();
 
  private:

@@ -176,12 +176,16 @@ int dvr_ldl_solve(const double* a, int32_t n, double sigma, const double* r, int
 #define DVR_PREC_SHIFT_INVERT 0  /* (H - sigma I)^-1 r, LDL^T per slice (default) */
 #define DVR_PREC_DIAGONAL 1      /* r / (D - theta)                             */
 #define DVR_PREC_LEARNED 2       /* w(features) r / (D - theta), BF16 inference */
+#define DVR_PREC_SI_LEARNED 3    /* shift-and-invert; the first DVR_DAV_FIRST (default
+                                    6) iterations run the learned correction while the
+                                    first shift is factored on a second stream */
 
 /* Keep a copy of H3 after the next dvr_run (needed by dvr_mlp_train). */
 int dvr_keep_hamiltonian(dvr_ctx_t* ctx, int32_t on);
 
 /* Correction of the Davidson solver (solver DVR_SOLVER_DAVIDSON); params
- * (DVR_MLP_NPARAM floats, copied) for DVR_PREC_LEARNED. */
+ * (DVR_MLP_NPARAM floats, copied) for DVR_PREC_LEARNED and
+ * DVR_PREC_SI_LEARNED. */
 int dvr_set_preconditioner(dvr_ctx_t* ctx, int32_t kind, const float* params);
 
 /* Trains params in place (Adam, steps x batch training pairs) on the

@@ -270,6 +270,34 @@ def test_davidson_context_rerun_with_smaller_basis():
         assert np.abs(ev - ref["eval"][: len(ev)]).max() * O.AUTOCM <= 1e-4
 
 
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+@pytest.mark.parametrize("first", ["0", "6"])
+def test_davidson_recovers_from_ambiguous_inertia(k, first, monkeypatch):
+    """A factorisation whose inertia count is ambiguous (a near-zero pivot:
+    the shift almost on a level) is repeated at a shift nudged within the
+    same gap instead of handing the problem to the dense path.  The k-th
+    count of the c1 solve is forced ambiguous from the host
+    (DVR_DAV_FORCE_AMBIG = k; k = 1 the first shift, later ones the next
+    shifts and the certification): the solve still certifies every level,
+    with one factorisation more, and matches the reference's levels; with the
+    first shift factored beside the first iterations (DVR_DAV_FIRST = 6) as
+    well."""
+    monkeypatch.setenv("DVR_DAV_FIRST", first)
+    job, pes = H.job_and_pes("c1")
+    monkeypatch.setenv("DVR_DAV_FORCE_AMBIG", "0")
+    base = H.run_device(H.device_inputs(job, pes, solver=DAVIDSON)).solver_stats()
+    monkeypatch.setenv("DVR_DAV_FORCE_AMBIG", str(k))
+    ctx = H.run_device(H.device_inputs(job, pes, solver=DAVIDSON))
+    st = ctx.solver_stats()
+    print(k, first, "factorisations", base["factorizations"], "->", st["factorizations"])
+    assert st["status"] == 0 and st["dense_fallback"] == 0, st
+    ev = ctx.eigenvalues() * O.AUTOCM
+    gold = np.array(H.golden("c1")["eval_cm"])
+    assert len(ev) == len(gold) and np.abs(ev - gold).max() <= 1e-4
+    if k <= base["factorizations"]:
+        assert st["factorizations"] == base["factorizations"] + 1, (base, st)
+
+
 def test_ldl_solve_zero_pivot_reported():
     """A shift on an eigenvalue gives a zero pivot: the inertia is ambiguous
     and dvr_ldl_solve reports DVR_ERR_NOCONV instead of a count."""
