@@ -10,6 +10,7 @@
 #include "dvr_internal.h"
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -52,7 +53,7 @@ struct dvr_ctx {
   std::string dav_error;   // message of an exception inside the Davidson solver
   bool keep_h = false, h_kept = false, split32 = false;
   double refine_resid = 0.0;
-  DevBuf Zr;
+  DevBuf Zr, permd;
   RefineWork rw;
   DevBuf H3keep, st3, mlpP;
   int prec_kind = 0;
@@ -376,10 +377,12 @@ void run_pipeline(dvr_ctx_t* C) {
     desc[0].diag = upload(C->e2sel, e2sel, s);
     desc[0].tile0 = 0;
     desc[0].ntile = nt * (nt + 1) / 2;
-    // split-precision FP32 assembly (with FP64 refinement after the
-    // Davidson solve) when DVR_ASSEMBLY=split32 and the chosen form runs
+    // split-precision FP32 assembly when DVR_ASSEMBLY=split32, with FP64
+    // refinement after the solve: a Rayleigh-Ritz step with the exact
+    // operator after the Davidson solve, a Rayleigh quotient per level after
+    // the dense one
     const char* asmenv = getenv("DVR_ASSEMBLY");
-    C->split32 = asmenv && std::string(asmenv) == "split32" && D.solver == DVR_SOLVER_DAVIDSON;
+    C->split32 = asmenv && std::string(asmenv) == "split32";
     if (C->split32)
       launch_syrk_split32(s, upload(C->sdesc3, desc, s), 1, desc[0].ntile, C->xkc.as<double>(),
                           nc, 8.0 * ((double)G * n3 + (double)n3 * n3),
@@ -478,24 +481,54 @@ void run_pipeline(dvr_ctx_t* C) {
     C->h_kept = true;
   }
   C->twostage_used = false;
+  std::vector<int> rq_perm;   // dense split32: level order after the refinement
   if (dense) {
     const int want = std::min(n3, meval + 16);
     // eigenvalues only: two-stage reduction (band reduction on MFMA, bulge
-    // chasing) of a copy of H3; with eigenvectors, or for small N3, the
-    // one-stage reduction whose reflectors the back-transformation uses
-    if (nvec == 0 && n3 >= 4 * twostage_bandwidth() && env_int("DVR_TWOSTAGE", 1)) {
+    // chasing) of a copy of H3; with eigenvectors (also those the split32
+    // refinement needs), or for small N3, the one-stage reduction whose
+    // reflectors the back-transformation uses
+    if (nvec == 0 && !C->split32 && n3 >= 4 * twostage_bandwidth() &&
+        env_int("DVR_TWOSTAGE", 1)) {
       C->twostage_used = twostage_values(C->H3.as<double>(), n3, want, s, C->ts, C->w3, C->H3b);
     } else {
       C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
       C->w3.values(std::vector<int>{want}, s);
     }
     C->h_eval.assign(C->w3.hval.begin(), C->w3.hval.begin() + meval);
+    if (C->split32) {
+      // every wanted level of the split-precision H3 refined by its Rayleigh
+      // quotient with the exact FP64 operator
+      prof_phase("refine");
+      const int mext = C->w3.extend(0, meval);
+      C->Zr.ensure(sizeof(double) * (size_t)n3 * mext);
+      C->w3.vectors(std::vector<int>{mext}, C->Zr.as<double>(), std::vector<long>{0}, s);
+      std::vector<int> col0(nc), iv2v(nc);
+      for (int c = 0; c < nc; ++c) {
+        col0[c] = cb[c].col0;
+        iv2v[c] = cb[c].iv2;
+      }
+      std::vector<double> ev;
+      C->refine_resid = refine_rq(s, C->cint.as<double>(), G, col0, iv2v, C->xkc.as<double>(),
+                                  C->e2sel.as<double>(), n3, C->Zr.as<double>(), meval, ev, C->rw);
+      rq_perm.resize(meval);
+      std::iota(rq_perm.begin(), rq_perm.end(), 0);
+      std::stable_sort(rq_perm.begin(), rq_perm.end(), [&](int x, int y) { return ev[x] < ev[y]; });
+      for (int k = 0; k < meval; ++k) C->h_eval[k] = ev[rq_perm[k]];
+      prof_phase("3d");
+    }
   }
   DVR_HIP(hipEventRecord(C->ev[6], s));
 
   // ---- eigenvectors and wavefunctions (trans, D:3650-3845)
   if (nvec > 0) {
-    if (dense) {
+    if (dense && C->split32) {
+      // the refined levels' vectors, in the refined order
+      std::vector<int> p(rq_perm.begin(), rq_perm.begin() + nvec);
+      C->Z3.ensure(sizeof(double) * (size_t)n3 * nvec);
+      launch_gather_cols(s, C->Zr.as<double>(), n3, upload(C->permd, p, s), nvec,
+                         C->Z3.as<double>(), n3, n3);
+    } else if (dense) {
       const int mext = C->w3.extend(0, nvec);
       C->Z3.ensure(sizeof(double) * (size_t)n3 * mext);
       C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);

@@ -802,5 +802,10 @@ void apply_h_exact(hipStream_t s, const double* cint, long G, const std::vector<
 double refine_pairs(hipStream_t s, const double* cint, long G, const std::vector<int>& col0,
                     const std::vector<int>& iv2, const double* xk, const double* e2, int N3,
                     double* X, int nv, std::vector<double>& eval, RefineWork& w);
+// Rayleigh quotient of every column of X (orthonormal) with the exact
+// operator (all levels wanted); returns the largest residual norm.
+double refine_rq(hipStream_t s, const double* cint, long G, const std::vector<int>& col0,
+                 const std::vector<int>& iv2, const double* xk, const double* e2, int N3,
+                 const double* X, int nv, std::vector<double>& eval, RefineWork& w);
 
 }  // namespace dvr

@@ -89,4 +89,30 @@ double refine_pairs(hipStream_t s, const double* cint, long G, const std::vector
   return mx;
 }
 
+// Rayleigh quotients with the exact operator, one per pair, for eigenvectors
+// X (N3 x nv, orthonormal) of an approximate H3 when every level is wanted
+// (a Rayleigh-Ritz over their span would be a second full eigensolve):
+// eval_i = x_i^T H x_i, whose error is of the order of the squared residual
+// over the gap to the next level; X is left as it is.  Returns the largest
+// residual norm ||H x_i - eval_i x_i||.
+double refine_rq(hipStream_t s, const double* cint, long G, const std::vector<int>& col0,
+                 const std::vector<int>& iv2, const double* xk, const double* e2, int N3,
+                 const double* X, int nv, std::vector<double>& eval, RefineWork& w) {
+  w.Y.ensure(sizeof(double) * (size_t)N3 * nv);
+  w.th.ensure(sizeof(double) * nv);
+  w.nrm.ensure(sizeof(double) * nv);
+  double* Y = w.Y.as<double>();
+  apply_h_exact(s, cint, G, col0, iv2, xk, e2, N3, X, nv, Y, w);
+  launch_coldot(s, X, Y, N3, nv, w.th.as<double>());
+  launch_resid_norms(s, X, Y, N3, w.th.as<double>(), nv, N3, w.nrm.as<double>());
+  eval.assign(nv, 0.0);
+  std::vector<double> nr(nv);
+  DVR_HIP(hipMemcpyAsync(eval.data(), w.th.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  DVR_HIP(hipMemcpyAsync(nr.data(), w.nrm.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+  DVR_HIP(hipStreamSynchronize(s));
+  double mx = 0.0;
+  for (double r : nr) mx = std::max(mx, r);
+  return mx;
+}
+
 }  // namespace dvr

@@ -213,6 +213,37 @@ def test_split32_assembly_refined_in_fp64(cfg, monkeypatch):
     assert np.abs(ev - gold).max() <= 1e-4
 
 
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_split32_dense_refined_by_rayleigh_quotients(cfg, monkeypatch):
+    """Every level wanted (dense solver): H3 assembled with FP32 split
+    products, solved densely, each level refined by its Rayleigh quotient
+    with the exact FP64 operator: levels within 1e-4 cm^-1 of the reference,
+    and the eigenvectors (in the refined order) span the FP64 dense run's:
+    |overlap| of each pair 1 to 1e-6 where the level is 0.01 cm^-1 away from
+    its neighbours."""
+    job, pes = H.job_and_pes(cfg)
+    gold = np.array(H.golden(cfg)["eval_cm"])
+    nvec = 20
+    monkeypatch.setenv("DVR_ASSEMBLY", "fp64")
+    ref = H.run_device(H.device_inputs(job, pes, nvec=nvec, solver=0))
+    monkeypatch.setenv("DVR_ASSEMBLY", "split32")
+    ctx = H.run_device(H.device_inputs(job, pes, nvec=nvec, solver=0))
+    ev = ctx.eigenvalues() * O.AUTOCM
+    err = np.abs(ev - gold)
+    print(cfg, "max |split32 + Rayleigh quotients - reference| cm^-1:", err.max())
+    assert len(ev) == len(gold) and err.max() <= 1e-4
+    assert np.all(np.diff(ev) >= 0)
+    # the wavefunctions are an isometric image of the eigenvectors
+    za = ref.wavefunctions(nvec).reshape(nvec, -1)
+    zb = ctx.wavefunctions(nvec).reshape(nvec, -1)
+    ov = np.abs(np.einsum("ij,ij->i", za, zb))
+    e = gold[:nvec + 1]
+    iso = np.ones(nvec, bool)
+    iso[1:] &= np.diff(e[:nvec]) > 1e-2
+    iso &= np.diff(e) > 1e-2
+    assert np.abs(ov[iso] - 1.0).max() <= 1e-6, ov
+
+
 def test_graph_replay_and_tuner_match_eager(monkeypatch):
     """The shift-and-invert solves replayed from HIP graphs give the same levels
     as eager launches (same kernels, bit for bit), and the tuned GEMM shapes
