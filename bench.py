@@ -882,10 +882,22 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a device-to-device copy of 512 MiB three times (calibrates "
                          "the FETCH_SIZE/WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--mlp-probe", action="store_true",
+                    help="only run the learned preconditioner's BF16 inference three times on "
+                         "the rows of one c3 correction block (20000 x 64 rows of 6 features)")
     ap.add_argument("--ldl-probe", type=int, default=0, metavar="N",
                     help="only factor one N x N symmetric matrix with the shift-and-invert "
                          "LDL^T (counter runs of its trailing-update GEMMs)")
     args = ap.parse_args()
+    if args.mlp_probe:
+        from dvr3d_amd import _lib
+        rng = np.random.default_rng(0)
+        params = _lib.mlp_init(0)
+        feats = rng.standard_normal((20000 * 64, 6)).astype(np.float32)
+        for _ in range(3):
+            w = _lib.mlp_forward(params, feats, bf16=True)
+        print(json.dumps({"mlp_probe_rows": feats.shape[0], "mean_weight": float(np.mean(w))}))
+        return
     if args.ldl_probe:
         from dvr3d_amd import _lib
         n = args.ldl_probe
