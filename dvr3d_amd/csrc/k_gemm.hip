@@ -332,7 +332,9 @@ __device__ inline void lower64_tile(long idx, int nt, int& ti, int& tj) {
 template <int NSTEPS, int D>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_gemm_lower_w64(GemmArgs g) {
-  static_assert(NSTEPS % 16 == 0 && NSTEPS <= 64 && D <= NSTEPS, "k steps");
+  // the ring runs on into the next tile, whose step t must find its
+  // fragments in slot t % D: D divides NSTEPS
+  static_assert(NSTEPS % 16 == 0 && NSTEPS <= 64 && NSTEPS % D == 0, "k steps");
   constexpr int CPS = 64 / NSTEPS;   // C loads per k step
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
