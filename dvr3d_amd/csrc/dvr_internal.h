@@ -236,9 +236,16 @@ struct EigBatch {
   double* gu;
   double* tnrm;
   double* pivmin;
+  double* pan;        // panel buffer of the blocked reduction (3 tri_panel_width() x sum n)
 };
-// Householder tridiagonalisation A = Q T Q^T (lower), column by column.
-void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n);
+// Householder tridiagonalisation A = Q T Q^T (lower): in panels when a
+// problem has over 2048 rows (needs b.pan; DVR_TRI=blocked: over 512,
+// DVR_TRI=column: never), column by column otherwise; h_aoff and h_voff are
+// the host copies of aoff and voff.
+void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n, const long* h_aoff,
+                    const long* h_voff);
+bool tri_blocked_enabled(int nmax);
+int tri_panel_width();
 // Gershgorin bounds + e^2.
 void launch_tridiag_bounds(hipStream_t s, const EigBatch& b);
 // Eigenvalues k in [il_p, il_p + cnt_p) of each problem, by Sturm-count
@@ -324,7 +331,7 @@ T* upload(DevBuf& d, const std::vector<T>& h, hipStream_t s) {
 struct EigWork {
   DevBuf n, aoff, voff, d, e, e2, tau, vb0, vb1, wb, pb, gl, gu, tn, piv;
   DevBuf il, kofs, eoff, eval;
-  DevBuf zoff, m, clus, scr, tbuf, toff, btofs;
+  DevBuf zoff, m, clus, scr, tbuf, toff, btofs, pan;
   EigBatch b{};
   std::vector<int> hn;
   std::vector<long> hvoff;
@@ -379,7 +386,12 @@ struct EigWork {
     b.tnrm = tn.as<double>();
     b.pivmin = piv.as<double>();
     if (!wg && reduce) {
-      launch_tridiag(s, b, hn.data());
+      b.pan = nullptr;
+      if (tri_blocked_enabled(nmax)) {
+        pan.ensure(sizeof(double) * 3 * tri_panel_width() * (size_t)(tot + 1));
+        b.pan = pan.as<double>();
+      }
+      launch_tridiag(s, b, hn.data(), aoff_.data(), hvoff.data());
       launch_tridiag_bounds(s, b);
     }
   }

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(TC) k_tri_col(EigBatch b, int j) {
 // wave, so no atomics are needed.
 constexpr int UC = 8;   // columns per work-group (2 per wave)
 
-__global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
+__global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j, int pending) {
   const int bid = blockIdx.x;
   int p = 0, base = 0;
   for (; p < b.nprob; ++p) {
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   double* cb = two ? A + (long)(k0 + 1) * n : ca;
   const int i0 = j + 1;
   double sa0 = 0.0, sa1 = 0.0, sb0 = 0.0, sb1 = 0.0;
-  if (j > 0) {
+  if (j > 0 && pending) {
     const double wa = w[k0], pa = vprev[k0];
     const double wb = two ? w[k0 + 1] : 0.0, pb = two ? vprev[k0 + 1] : 0.0;
     int i = i0 + lane;
@@ -250,6 +250,151 @@ __global__ void __launch_bounds__(TB) k_tri_upd(EigBatch b, int j) {
   if (lane == 0) {
     sv[k0] = ra;
     if (two) sv[k0 + 1] = rb;
+  }
+}
+
+// Blocked reduction of the large problems (the form of LAPACK's dsytrd /
+// dlatrd): the trailing square stays as it was at the start of a panel of
+// NBP columns, whose reflectors v and vectors w are kept in the panel buffer
+// [V | W | V] (n_p rows, 3 NBP columns per problem).  Per column j (slot
+// c = j - j0 of the panel starting at j0) two launches:
+//   k_tri_pcol : w of column j-1 from s = A_panel v_{j-1} (k_tri_upd without
+//                its update) with the panel's corrections,
+//                  w = tau p + alpha v,  p = s - V (W^T v) - W (V^T v),
+//                  alpha = -(tau^2 / 2) p.v;
+//                column j updated by the panel's reflectors so far, and its
+//                reflector (v_j, tau_j) formed as in k_tri_col;
+//   k_tri_upd  : s_j = A_panel v_j over the trailing square (read only).
+// After the panel, k_tri_pcol<true> finishes the last w, and one NT GEMM per
+// problem applies A -= [V W] [W V]^T to the trailing square (both
+// triangles).  The trailing square is read once per column and written once
+// per panel, where the column-by-column form reads and writes it per column.
+// Problems whose columns end inside the panel (the LDS tail's start) get zero
+// slots, so their GEMM adds nothing for them.
+constexpr int NBP = 32;
+int tri_panel_width() { return NBP; }
+
+template <bool FIN>
+__global__ void __launch_bounds__(TC) k_tri_pcol(EigBatch b, int j, int j0) {
+  __shared__ double red[TC / 64];
+  __shared__ double part2[2 * NBP][TC / 64];
+  __shared__ double ys[2 * NBP];   // W^T v, V^T v of the previous column
+  __shared__ double cj[2 * NBP];   // W(j, c'), V(j, c') of this column
+  const int p = blockIdx.x, tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  const int ts = tail_start(n);
+  double* A = b.A + b.aoff[p];
+  double* V = b.pan + 3L * NBP * vo;
+  double* W = V + (long)NBP * n;
+  double* V2 = V + 2L * NBP * n;
+  const int c = j - j0;
+  // the last real column of this problem in the panel is min(j0 + NBP, ts) - 1;
+  // its w is finished at column min(j0 + NBP, ts) (or by the FIN launch)
+  const int jreal = min(j0 + NBP, ts);
+  const bool finish = c > 0 && j - 1 < jreal && j <= jreal;
+  if (finish) {
+    // w of column j - 1 (slot c - 1); v_{j-1} lives in rows >= j
+    const int cp = c - 1;
+    const double* v = V + (long)cp * n;
+    const double* sv = b.pbuf + vo;
+    const double taup = b.tau[vo + j - 1];
+    // W(:, c')^T v and V(:, c')^T v for c' < cp
+    for (int q = 0; q < cp; ++q) {
+      double a1 = 0.0, a2 = 0.0;
+      for (int i = j + tid; i < n; i += TC) {
+        const double vi = v[i];
+        a1 += W[i + (long)q * n] * vi;
+        a2 += V[i + (long)q * n] * vi;
+      }
+      a1 = wave_sum(a1);
+      a2 = wave_sum(a2);
+      if (lane == 0) {
+        part2[q][wave] = a1;
+        part2[NBP + q][wave] = a2;
+      }
+    }
+    __syncthreads();
+    if (tid < NBP && tid < cp) {
+      double y1 = 0.0, y2 = 0.0;
+      for (int w = 0; w < TC / 64; ++w) {
+        y1 += part2[tid][w];
+        y2 += part2[NBP + tid][w];
+      }
+      ys[tid] = y1;
+      ys[NBP + tid] = y2;
+    }
+    __syncthreads();
+    // p = s - V y1 - W y2 (kept in the W column), K = p.v
+    double* wcol = W + (long)cp * n;
+    double kp = 0.0;
+    for (int i = j + tid; i < n; i += TC) {
+      double pi = sv[i];
+      for (int q = 0; q < cp; ++q)
+        pi -= V[i + (long)q * n] * ys[q] + W[i + (long)q * n] * ys[NBP + q];
+      wcol[i] = pi;
+      kp += pi * v[i];
+    }
+    const double K = taup * block_sum_n(kp, red, TC);
+    const double alpha = -0.5 * taup * K;
+    for (int i = tid; i < n; i += TC)
+      wcol[i] = i < j ? 0.0 : taup * wcol[i] + alpha * v[i];
+    __syncthreads();
+  }
+  if (FIN) return;
+  if (j >= ts || j > n - 2) {
+    // no column here for this problem: an empty slot
+    if (c < NBP)
+      for (int i = tid; i < n; i += TC) {
+        V[i + (long)c * n] = 0.0;
+        V2[i + (long)c * n] = 0.0;
+        W[i + (long)c * n] = 0.0;
+      }
+    return;
+  }
+  double* col = A + (long)j * n;
+  // column j with the panel's updates: col -= V W(j, :)^T + W V(j, :)^T
+  if (tid < c) {
+    cj[tid] = W[j + (long)tid * n];
+    cj[NBP + tid] = V[j + (long)tid * n];
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int i = j + tid; i < n; i += TC) {
+    double ci = col[i];
+    for (int q = 0; q < c; ++q)
+      ci -= V[i + (long)q * n] * cj[q] + W[i + (long)q * n] * cj[NBP + q];
+    col[i] = ci;
+    if (i >= j + 2) part += ci * ci;
+  }
+  const double sigma = block_sum_n(part, red, TC);   // column j visible
+  const double alpha = col[j + 1];
+  double tau, beta, scal;
+  if (sigma == 0.0) {
+    tau = 0.0;
+    beta = alpha;
+    scal = 0.0;
+  } else {
+    const double mu = sqrt(alpha * alpha + sigma);
+    beta = (alpha <= 0.0) ? mu : -mu;
+    tau = (beta - alpha) / beta;
+    scal = 1.0 / (alpha - beta);
+  }
+  double* vcur = (j & 1 ? b.vbuf1 : b.vbuf0) + vo;
+  for (int i = tid; i < n; i += TC) {
+    const double vi = i <= j ? 0.0 : (i == j + 1 ? 1.0 : col[i] * scal);
+    V[i + (long)c * n] = vi;
+    V2[i + (long)c * n] = vi;
+    if (i > j) vcur[i] = vi;
+    if (i > j + 1) col[i] = vi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    b.d[vo + j] = col[j];
+    b.e[vo + j] = beta;
+    b.tau[vo + j] = tau;
+    col[j + 1] = beta;
   }
 }
 
@@ -442,7 +587,13 @@ __global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b, int fused) {
     B[i + k * LDB] = A[(j0 + i) + (long)(j0 + k) * n];
   }
   double taup = 0.0;
-  if (j0 > 0) {
+  if (j0 > 0 && fused == 2) {
+    // blocked reduction: every update up to column j0 already applied
+    for (int i = tid; i < t; i += TT) {
+      vp[i] = 0.0;
+      ss[i] = 0.0;
+    }
+  } else if (j0 > 0) {
     const double* vprev = ((j0 - 1) & 1 ? b.vbuf1 : b.vbuf0) + vo;
     // s_{j0-1}: pbuf, or (one-launch columns) the buffer of that parity
     const double* sprev = (fused && ((j0 - 1) & 1) ? b.wbuf : b.pbuf) + vo;
@@ -544,7 +695,8 @@ __global__ void __launch_bounds__(TT) k_tri_tail(EigBatch b, int fused) {
   }
 }
 
-void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
+void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n, const long* h_aoff,
+                    const long* h_voff) {
   int jend = 0, nmax = 0;
   for (int p = 0; p < b.nprob; ++p) {
     jend = std::max(jend, tail_start(h_n[p]));
@@ -564,7 +716,40 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
     }
     DVR_LAUNCH("tri_fused", bytes, flops, k_tri_fused, dim3(grid), dim3(TB), 0, s, b, j);
   }
-  for (int j = 0; j < jend && !fused; ++j) {
+  const bool blocked = !fused && tri_blocked_enabled(nmax);
+  for (int j0 = 0; j0 < jend && blocked; j0 += NBP) {
+    for (int j = j0; j < j0 + NBP; ++j) {
+      double cbytes = 0.0, ubytes = 0.0, uflops = 0.0;
+      int grid = 0;
+      for (int p = 0; p < b.nprob; ++p) {
+        const double m = (double)(h_n[p] - j);
+        cbytes += 8.0 * 6.0 * NBP * std::max(m, 0.0);
+        if (j >= tail_start(h_n[p]) || j > h_n[p] - 3) continue;
+        grid += (h_n[p] - j - 1 + UC - 1) / UC;
+        const double t = m - 1.0;
+        ubytes += 8.0 * t * t + 8.0 * 2.0 * t;
+        uflops += 2.0 * t * t;
+      }
+      DVR_LAUNCH("tri_pcol", cbytes, 0.0, k_tri_pcol<false>, dim3(b.nprob), dim3(TC), 0, s, b, j,
+                 j0);
+      if (grid > 0)
+        DVR_LAUNCH("tri_symv", ubytes, uflops, k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j, 0);
+    }
+    DVR_LAUNCH("tri_pcol", 0.0, 0.0, k_tri_pcol<true>, dim3(b.nprob), dim3(TC), 0, s, b, j0 + NBP,
+               j0);
+    // the panel's rank-2 NBP update of every problem's trailing square
+    for (int p = 0; p < b.nprob; ++p) {
+      const int np = h_n[p], ts = tail_start(np);
+      if (j0 >= ts) continue;
+      const int js = std::min(j0 + NBP, ts), m = np - js;
+      if (m <= 0) continue;
+      double* V = b.pan + 3L * NBP * h_voff[p];
+      launch_gemm(s, "tri_pupd", false, true, m, m, 2 * NBP, -1.0, V + js, np,
+                  V + (long)NBP * np + js, np, 1.0, b.A + h_aoff[p] + js + (long)js * np, np,
+                  1, nullptr, false);
+    }
+  }
+  for (int j = 0; j < jend && !fused && !blocked; ++j) {
     double cbytes = 0.0, ubytes = 0.0, uflops = 0.0;
     int grid = 0;
     for (int p = 0; p < b.nprob; ++p) {
@@ -579,7 +764,7 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
     }
     DVR_LAUNCH("tri_col", cbytes, 0.0, k_tri_col, dim3(b.nprob), dim3(TC), 0, s, b, j);
     if (grid > 0) {
-      DVR_LAUNCH("tri_upd", ubytes, uflops, k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j);
+      DVR_LAUNCH("tri_upd", ubytes, uflops, k_tri_upd, dim3(grid), dim3(TB), 0, s, b, j, 1);
     }
   }
   for (const void* f : {(const void*)k_tri_tail<256>, (const void*)k_tri_tail<512>,
@@ -591,15 +776,16 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n) {
     tbytes += 8.0 * 2.0 * t * t;
   }
   const int tt = tail_threads();
+  const int mode = fused ? 1 : (blocked ? 2 : 0);
   if (tt == 256)
     DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail<256>, dim3(b.nprob), dim3(256), TAIL_LDS, s, b,
-               fused ? 1 : 0);
+               mode);
   else if (tt == 512)
     DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail<512>, dim3(b.nprob), dim3(512), TAIL_LDS, s, b,
-               fused ? 1 : 0);
+               mode);
   else
     DVR_LAUNCH("tri_tail", tbytes, 0.0, k_tri_tail<1024>, dim3(b.nprob), dim3(1024), TAIL_LDS, s,
-               b, fused ? 1 : 0);
+               b, mode);
   DVR_LAUNCH_CHECK();
 }
 
@@ -784,6 +970,19 @@ constexpr int LDS_BYTES_MAX = 160 * 1024;
 bool tri_fused_enabled() {
   const char* v = getenv("DVR_TRI");
   return !(v && std::string(v) == "twolaunch");
+}
+
+// The blocked (panel) reduction for batches with a problem over 2048 rows
+// (c2's 2D batch, 30 problems up to 3629 rows: 1090 -> 695 ms per block;
+// c3's, 64 problems up to 1116 rows: 41 ms column by column, 61 ms in
+// panels, where the per-problem panel GEMMs are short launches);
+// DVR_TRI=blocked takes it for every batch over FMAX rows, DVR_TRI=column
+// (or twolaunch) never
+bool tri_blocked_enabled(int nmax) {
+  const char* v = getenv("DVR_TRI");
+  const std::string t = v ? v : "";
+  if (t == "column" || t == "twolaunch") return false;
+  return nmax > (t == "blocked" ? FMAX : 2048);
 }
 
 // DVR_BT=global keeps the back-transformation on the global-memory kernels

@@ -89,6 +89,30 @@ def test_syev_small_problems_column_launches(monkeypatch, tri):
         _check(a, w, z, a.shape[0])
 
 
+@pytest.mark.parametrize("sizes", [[700], [1500, 600, 513, 100], [2203, 640, 129, 1]])
+def test_syev_blocked_reduction(monkeypatch, sizes, solver_kind):
+    """The blocked (panel) reduction of problems over 512 rows
+    (DVR_TRI=blocked; the default above 2048 rows: the trailing square
+    updated once per 32-column panel by an MFMA GEMM, read once per column):
+    eigenpairs against LAPACK in batches that mix sizes above and below 512,
+    whose LDS tails start inside different panels, and against the
+    column-by-column reduction (DVR_TRI=column).  Multi-
+    launch solver only: the single-work-group one has no column launches."""
+    if solver_kind != "multi":
+        pytest.skip("the blocked reduction belongs to the multi-launch solver")
+    from dvr3d_amd import _lib
+    rng = np.random.default_rng(sum(sizes) + 5)
+    mats = [_sym(nn, rng) for nn in sizes]
+    monkeypatch.setenv("DVR_TRI", "blocked")
+    ws, zs = _lib.syev_batch(mats)
+    for a, w, z in zip(mats, ws, zs):
+        _check(a, w, z, a.shape[0])
+    monkeypatch.setenv("DVR_TRI", "column")
+    ws0, _ = _lib.syev_batch(mats)
+    for w, w0, a in zip(ws, ws0, mats):
+        assert np.abs(w - w0).max() <= 1e-12 * max(1.0, np.abs(w0).max()) * a.shape[0] ** 0.5
+
+
 def test_syev_batch_partial_and_clusters():
     from dvr3d_amd import _lib
     rng = np.random.default_rng(7)
