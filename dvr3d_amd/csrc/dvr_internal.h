@@ -238,13 +238,14 @@ struct EigBatch {
   double* pivmin;
   double* pan;        // panel buffer of the blocked reduction (3 tri_panel_width() x sum n)
 };
-// Householder tridiagonalisation A = Q T Q^T (lower): in panels when a
-// problem has over 2048 rows (needs b.pan; DVR_TRI=blocked: over 512,
-// DVR_TRI=column: never), column by column otherwise; h_aoff and h_voff are
-// the host copies of aoff and voff.
+// Householder tridiagonalisation A = Q T Q^T (lower): in panels for batches
+// of at least 8 problems over 1024 rows with one over 2048 (needs b.pan;
+// DVR_TRI=blocked: any batch with a problem over 512, DVR_TRI=column: never),
+// column by column otherwise; h_aoff and h_voff are the host copies of aoff
+// and voff.
 void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n, const long* h_aoff,
                     const long* h_voff);
-bool tri_blocked_enabled(int nmax);
+bool tri_blocked_enabled(const int* hn, int np);
 int tri_panel_width();
 // Gershgorin bounds + e^2.
 void launch_tridiag_bounds(hipStream_t s, const EigBatch& b);
@@ -387,7 +388,7 @@ struct EigWork {
     b.pivmin = piv.as<double>();
     if (!wg && reduce) {
       b.pan = nullptr;
-      if (tri_blocked_enabled(nmax)) {
+      if (tri_blocked_enabled(hn.data(), np)) {
         pan.ensure(sizeof(double) * 3 * tri_panel_width() * (size_t)(tot + 1));
         b.pan = pan.as<double>();
       }

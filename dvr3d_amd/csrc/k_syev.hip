@@ -716,7 +716,7 @@ void launch_tridiag(hipStream_t s, const EigBatch& b, const int* h_n, const long
     }
     DVR_LAUNCH("tri_fused", bytes, flops, k_tri_fused, dim3(grid), dim3(TB), 0, s, b, j);
   }
-  const bool blocked = !fused && tri_blocked_enabled(nmax);
+  const bool blocked = !fused && tri_blocked_enabled(h_n, b.nprob);
   for (int j0 = 0; j0 < jend && blocked; j0 += NBP) {
     for (int j = j0; j < j0 + NBP; ++j) {
       double cbytes = 0.0, ubytes = 0.0, uflops = 0.0;
@@ -972,17 +972,25 @@ bool tri_fused_enabled() {
   return !(v && std::string(v) == "twolaunch");
 }
 
-// The blocked (panel) reduction for batches with a problem over 2048 rows
-// (c2's 2D batch, 30 problems up to 3629 rows: 1090 -> 695 ms per block;
-// c3's, 64 problems up to 1116 rows: 41 ms column by column, 61 ms in
-// panels, where the per-problem panel GEMMs are short launches);
-// DVR_TRI=blocked takes it for every batch over FMAX rows, DVR_TRI=column
-// (or twolaunch) never
-bool tri_blocked_enabled(int nmax) {
+// The blocked (panel) reduction for batches of at least 8 problems over 1024
+// rows with one over 2048: c2's 2D batch (30 problems up to 3629 rows) went
+// 1090 -> 695 ms per block; where the batch is small the one-work-group
+// column kernel of each panel step is the critical path -- one 4000-row
+// problem with 300 vectors (c4's dense solves) 165 -> 277 ms, c3's 2D batch
+// (64 problems up to 1116 rows, short per-problem panel GEMMs) 41 -> 61 ms.
+// DVR_TRI=blocked takes it for every batch with a problem over FMAX rows,
+// DVR_TRI=column (or twolaunch) never.
+bool tri_blocked_enabled(const int* hn, int np) {
   const char* v = getenv("DVR_TRI");
   const std::string t = v ? v : "";
   if (t == "column" || t == "twolaunch") return false;
-  return nmax > (t == "blocked" ? FMAX : 2048);
+  int nmax = 0, big = 0;
+  for (int p = 0; p < np; ++p) {
+    nmax = std::max(nmax, hn[p]);
+    big += hn[p] > 1024;
+  }
+  if (t == "blocked") return nmax > FMAX;
+  return nmax > 2048 && big >= 8;
 }
 
 // DVR_BT=global keeps the back-transformation on the global-memory kernels

@@ -92,7 +92,8 @@ def test_syev_small_problems_column_launches(monkeypatch, tri):
 @pytest.mark.parametrize("sizes", [[700], [1500, 600, 513, 100], [2203, 640, 129, 1]])
 def test_syev_blocked_reduction(monkeypatch, sizes, solver_kind):
     """The blocked (panel) reduction of problems over 512 rows
-    (DVR_TRI=blocked; the default above 2048 rows: the trailing square
+    (DVR_TRI=blocked; the default for batches of at least 8 problems over
+    1024 rows with one over 2048: the trailing square
     updated once per 32-column panel by an MFMA GEMM, read once per column):
     eigenpairs against LAPACK in batches that mix sizes above and below 512,
     whose LDS tails start inside different panels, and against the
