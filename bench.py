@@ -868,6 +868,8 @@ def main():
                     help="c4: training steps of the learned preconditioner per dense instance")
     ap.add_argument("--solver", default="davidson", choices=["dense", "davidson"])
     ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
+    ap.add_argument("--no-vectors-leg", action="store_true",
+                    help="skip the chosen-vs-plain comparison with every level's eigenvector")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="timed steps of the plain FP64 dense path run beside the chosen form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -951,9 +953,9 @@ def main():
             gold = np.array(json.load(open(gold_path))["eval_cm"])
             break
 
-    def measure(solver_name, steps, warmup, profile):
+    def measure(solver_name, steps, warmup, profile, nvec=0):
         solver = _lib.DVR_SOLVER_DENSE if solver_name == "dense" else _lib.DVR_SOLVER_DAVIDSON
-        ctx = _lib.Context(b.dims(nvec=0, solver=solver))
+        ctx = _lib.Context(b.dims(nvec=nvec, solver=solver))
         ctx.set_basis(b.xlmatr[:na, :na], b.xkb, b.xkc, b.wsum, None,
                       job.emax1 / host.AUTOCM, job.emax2 / host.AUTOCM)
 
@@ -1007,6 +1009,23 @@ def main():
     if args.solver == "davidson" and args.plain_steps > 0:
         plain = measure("dense", args.plain_steps, 1, False)
         progress(f"{cfg}: plain dense {plain['ms_per_step']:.1f} ms per step")
+    vec_leg = None
+    if args.solver == "davidson" and args.plain_steps > 0 and not args.no_vectors_leg:
+        # both forms also delivering the eigenvectors of every level (the
+        # reference's dsyev('V') and the wavefunction records): the Davidson
+        # solve has them already, the dense path back-transforms them
+        nv = b.sz.neval
+        cv = measure("davidson", args.plain_steps, 1, False, nvec=nv)
+        pv = measure("dense", args.plain_steps, 1, False, nvec=nv)
+        vec_leg = {"eigenvectors": nv,
+                   "chosen_ms_per_step": round(cv["ms_per_step"], 3),
+                   "chosen_converged": cv["converged"],
+                   "plain_ms_per_step": round(pv["ms_per_step"], 3),
+                   "plain_converged": pv["converged"], "plain_max_err_cm": pv["max_err_cm"],
+                   "plain_stage_ms": pv["stage_ms"],
+                   "speedup_vs_plain": round(pv["ms_per_step"] / cv["ms_per_step"], 3)}
+        progress(f"{cfg}: with {nv} eigenvectors: chosen {cv['ms_per_step']:.1f} ms, "
+                 f"plain {pv['ms_per_step']:.1f} ms per step")
     kprof = main_run.pop("kprof")
     line = {
         "metric": "converged levels/s at <=1e-3 cm^-1 vs FP64 oracle: chosen form vs plain FP64 GPU",
@@ -1040,6 +1059,8 @@ def main():
             "converged": plain["converged"], "max_err_cm": plain["max_err_cm"],
             "stage_ms": plain["stage_ms"]}
         line["config"]["speedup_vs_plain"] = round(plain["ms_per_step"] / main_run["ms_per_step"], 3)
+    if vec_leg is not None:
+        line["config"]["with_eigenvectors"] = vec_leg
     if not args.no_cpu_baseline:
         progress("CPU baseline: the reference program on host cores")
         ref = cpu_baseline_reference(cfg)

@@ -56,6 +56,7 @@ struct dvr_ctx {
   DevBuf Zr, permd;
   RefineWork rw;
   DevBuf H3keep, st3, mlpP;
+  DevBuf ptmp;                // per-block wavefunction products before the scatter
   int prec_kind = 0;
   bool mlp_set = false;
   MlpWork mw;
@@ -484,12 +485,13 @@ void run_pipeline(dvr_ctx_t* C) {
   std::vector<int> rq_perm;   // dense split32: level order after the refinement
   if (dense) {
     const int want = std::min(n3, meval + 16);
-    // eigenvalues only: two-stage reduction (band reduction on MFMA, bulge
-    // chasing) of a copy of H3; with eigenvectors (also those the split32
-    // refinement needs), or for small N3, the one-stage reduction whose
-    // reflectors the back-transformation uses
-    if (nvec == 0 && !C->split32 && n3 >= 4 * twostage_bandwidth() &&
-        env_int("DVR_TWOSTAGE", 1)) {
+    // two-stage reduction (band reduction on MFMA, bulge chasing) of a copy
+    // of H3, its reflectors kept when eigenvectors are wanted (also those the
+    // split32 refinement needs); for small N3, or with DVR_TWOSTAGE_VEC=0
+    // and eigenvectors, the one-stage reduction
+    const bool wantvec = nvec > 0 || C->split32;
+    if (n3 >= 4 * twostage_bandwidth() && (wantvec ? env_twostage_vec() : env_twostage())) {
+      C->ts.keep = wantvec;
       C->twostage_used = twostage_values(C->H3.as<double>(), n3, want, s, C->ts, C->w3, C->H3b);
     } else {
       C->w3.tridiag(std::vector<int>{n3}, C->H3.as<double>(), std::vector<long>{0}, s);
@@ -502,7 +504,10 @@ void run_pipeline(dvr_ctx_t* C) {
       prof_phase("refine");
       const int mext = C->w3.extend(0, meval);
       C->Zr.ensure(sizeof(double) * (size_t)n3 * mext);
-      C->w3.vectors(std::vector<int>{mext}, C->Zr.as<double>(), std::vector<long>{0}, s);
+      if (C->twostage_used)
+        twostage_vectors(n3, mext, C->Zr.as<double>(), s, C->ts, C->w3);
+      else
+        C->w3.vectors(std::vector<int>{mext}, C->Zr.as<double>(), std::vector<long>{0}, s);
       std::vector<int> col0(nc), iv2v(nc);
       for (int c = 0; c < nc; ++c) {
         col0[c] = cb[c].col0;
@@ -531,7 +536,10 @@ void run_pipeline(dvr_ctx_t* C) {
     } else if (dense) {
       const int mext = C->w3.extend(0, nvec);
       C->Z3.ensure(sizeof(double) * (size_t)n3 * mext);
-      C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);
+      if (C->twostage_used)
+        twostage_vectors(n3, mext, C->Z3.as<double>(), s, C->ts, C->w3);
+      else
+        C->w3.vectors(std::vector<int>{mext}, C->Z3.as<double>(), std::vector<long>{0}, s);
     } else if (C->split32) {
       C->Z3.ensure(sizeof(double) * (size_t)n3 * nvec);
       DVR_HIP(hipMemcpyAsync(C->Z3.p, C->Zr.p, sizeof(double) * (size_t)n3 * nvec,
@@ -543,9 +551,8 @@ void run_pipeline(dvr_ctx_t* C) {
     const size_t npsi = (size_t)D.idvr * D.npnt1 * D.npnt2;
     C->psi.ensure(sizeof(double) * npsi * nvec);
     DVR_HIP(hipMemsetAsync(C->psi.p, 0, sizeof(double) * npsi * nvec, s));
-    launch_trans(s, C->cint.as<double>(), C->Z3.as<double>(),
-                 C->psi.as<double>(), d_cb, nc, na, nb, n3, nvec, D.zr2r1,
-                 D.idvr, D.npnt1, D.npnt2);
+    launch_trans(s, C->cint.as<double>(), C->Z3.as<double>(), C->psi.as<double>(), d_cb, cb, nc,
+                 na, nb, n3, nvec, D.zr2r1, D.idvr, D.npnt1, D.npnt2, C->ptmp);
   }
   DVR_HIP(hipEventRecord(C->ev[7], s));
   DVR_HIP(hipStreamSynchronize(s));
@@ -1042,18 +1049,31 @@ int dvr_syev_batch(int32_t nprob, const int32_t* n, const double* a,
       nmx = std::max(nmx, x);
       nmn = std::min(nmn, x);
     }
-    // eigenvalues only, every problem of order >= 256: two-stage reduction
-    // of each problem in turn
-    if (!z && nmn >= 4 * twostage_bandwidth() && env_int("DVR_TWOSTAGE", 1)) {
+    // every problem of order >= 256: two-stage reduction of each problem in
+    // turn, eigenvectors back-transformed through both stages
+    if (nmn >= 4 * twostage_bandwidth() && (z ? env_twostage_vec() : env_twostage())) {
       TwoStage ts;
-      DevBuf cp;
-      long wo = 0;
+      ts.keep = z != nullptr;
+      DevBuf cp, Zp;
+      long wo = 0, zo = 0;
       for (int p = 0; p < nprob; ++p) {
         const int want = std::min(hn[p], cntv[p] + (cntv[p] < hn[p] ? 16 : 0));
         EigWork Wp;
-        twostage_values(A.as<double>() + aoff[p], hn[p], want, s, ts, Wp, cp);
+        const bool two = twostage_values(A.as<double>() + aoff[p], hn[p], want, s, ts, Wp, cp);
         std::memcpy(w + wo, Wp.hval.data(), sizeof(double) * cntv[p]);
         wo += cntv[p];
+        if (z && cntv[p] > 0) {
+          const int mext = Wp.extend(0, cntv[p]);
+          Zp.ensure(sizeof(double) * (size_t)hn[p] * mext);
+          if (two)
+            twostage_vectors(hn[p], mext, Zp.as<double>(), s, ts, Wp);
+          else
+            Wp.vectors(std::vector<int>{mext}, Zp.as<double>(), std::vector<long>{0}, s);
+          DVR_HIP(hipMemcpyAsync(z + zo, Zp.p, sizeof(double) * (size_t)hn[p] * cntv[p],
+                                 hipMemcpyDeviceToHost, s));
+          DVR_HIP(hipStreamSynchronize(s));
+          zo += (long)hn[p] * cntv[p];
+        }
       }
       DVR_HIP(hipStreamSynchronize(s));
       (void)hipStreamDestroy(s);

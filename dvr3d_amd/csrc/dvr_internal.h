@@ -157,12 +157,14 @@ void launch_cint(hipStream_t s, const double* v1, const double* c2,
                  double* cint, const CBlock* cb, const int* blk3,
                  const int* iv1off, int na, int nb, int n3);
 
+struct DevBuf;
 // Wavefunction amplitudes of nvec levels on the DVR grid, in the IWAVE
 // record layout psi(idvr, npnt1, npnt2) per level.
-void launch_trans(hipStream_t s, const double* cint, const double* c3,
-                  double* psi, const CBlock* cb, int nc, int na, int nb,
-                  int n3, int nvec, int zr2r1, int idvr, int npnt1,
-                  int npnt2);
+// hcb: the blocks on the host; tmp: staging of the per-block products
+// (G nc nvec doubles).
+void launch_trans(hipStream_t s, const double* cint, const double* c3, double* psi,
+                  const CBlock* cb, const std::vector<CBlock>& hcb, int nc, int na, int nb,
+                  int n3, int nvec, int zr2r1, int idvr, int npnt1, int npnt2, DevBuf& tmp);
 
 // ---------------------------------------------------------------------------
 // 1D angular eigenproblems (k_jacobi.hip)
@@ -441,8 +443,10 @@ struct EigWork {
   }
 
   // Eigenvectors 0..m[p]-1 of every problem into Z (n_p x m_p at zoff[p]).
+  // back = false leaves the eigenvectors of the tridiagonal matrices (the
+  // caller back-transforms them itself, as the two-stage reduction does).
   void vectors(const std::vector<int>& mv, double* Z,
-               const std::vector<long>& zoff_, hipStream_t s) {
+               const std::vector<long>& zoff_, hipStream_t s, bool back = true) {
     const int np = b.nprob;
     if (wg) {
       std::vector<long> so(np, 0);
@@ -482,6 +486,7 @@ struct EigWork {
     const long* dz = upload(zoff, zoff_, s);
     launch_stein(s, b, clus.as<Cluster>(), ncl, eval.as<double>(),
                  eoff.as<long>(), Z, dz, scr.as<double>(), T);
+    if (!back) return;
     // WY factors
     const int NBk = backtransform_nb();
     std::vector<long> ht(np, 0);
@@ -505,9 +510,20 @@ struct EigWork {
 struct TwoStage {
   DevBuf work, vyv, wb, qa, qb, G, dl, Linv, sc, M, Uinv, Rb, Zs, Mm, Sv, Tst, zero, flags, taul;
   DevBuf AB, vs, taus;
+  // kept for the back-transformation when keep is set: every reflector of
+  // the bulge chase (V2: BW doubles per (task, sweep), tau2), the reduced
+  // matrix (its first-stage reflectors below the band) and the panel count
+  bool keep = false;
+  DevBuf V2, tau2, Zt, Vp, bw1, bw2;
+  double* Ared = nullptr;
+  long lda_red = 0;
+  int n_red = 0, npanel = 0, jlast = 0, mlast = 0;
   // A (n x n, ld lda, both triangles) is destroyed; d[n], e[n-1] on the
   // device.  Returns 0, or 1 when a panel was rank deficient.
   int reduce(double* A, long lda, int n, double* d, double* e, hipStream_t s);
+  // Z (n x nv, ld ldz): eigenvectors of the tridiagonal matrix of the last
+  // reduce (with keep set) -> eigenvectors of its A, in place.
+  void back(double* Z, long ldz, int nv, hipStream_t s);
 };
 int twostage_bandwidth();
 // DVR_TWOSTAGE=0 switches the two-stage reduction off (one-stage instead)
@@ -515,12 +531,21 @@ inline bool env_twostage() {
   const char* v = getenv("DVR_TWOSTAGE");
   return !v || atoi(v) != 0;
 }
+// DVR_TWOSTAGE_VEC=0 keeps eigenvector solves on the one-stage reduction
+inline bool env_twostage_vec() {
+  const char* v = getenv("DVR_TWOSTAGE_VEC");
+  return env_twostage() && (!v || atoi(v) != 0);
+}
 // Lowest `want` eigenvalues of the symmetric A (device, n x n, ld n, both
 // triangles; left intact) through the two-stage reduction of a copy in
 // `copy`, with the one-stage reduction of A as fallback; the values end in
 // W.hval (W.heoff[0] = 0).  Returns true when the two-stage path was used.
 bool twostage_values(const double* A, int n, int want, hipStream_t s, TwoStage& ts, EigWork& W,
                      DevBuf& copy);
+// After twostage_values returned true with ts.keep set: eigenvectors 0..m-1
+// of A into Z (n x m, ld n): inverse iteration on the tridiagonal matrix,
+// then back through the bulge chase and the band reduction.
+void twostage_vectors(int n, int m, double* Z, hipStream_t s, TwoStage& ts, EigWork& W);
 
 // ---------------------------------------------------------------------------
 // General FP64 GEMM on MFMA (k_gemm.hip)

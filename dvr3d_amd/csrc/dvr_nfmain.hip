@@ -275,8 +275,10 @@ extern "C" int dvr_nfmain(const dvr_nf_dims_t* D, const double* hr, const double
     TwoStage ts;
     DevBuf cp, Z3;
     const int want = std::min(n3, meval + 16);
-    if (nv3 == 0 && n3 >= 4 * twostage_bandwidth() && env_twostage()) {
-      twostage_values(H3.as<double>(), n3, want, s, ts, w3, cp);
+    bool two = false;
+    if (n3 >= 4 * twostage_bandwidth() && (nv3 > 0 ? env_twostage_vec() : env_twostage())) {
+      ts.keep = nv3 > 0;
+      two = twostage_values(H3.as<double>(), n3, want, s, ts, w3, cp);
     } else {
       w3.tridiag(std::vector<int>{n3}, H3.as<double>(), std::vector<long>{0}, s);
       w3.values(std::vector<int>{want}, s);
@@ -285,7 +287,10 @@ extern "C" int dvr_nfmain(const dvr_nf_dims_t* D, const double* hr, const double
     if (nv3 > 0) {
       const int mx = w3.extend(0, nv3);
       Z3.ensure(sizeof(double) * (size_t)n3 * mx);
-      w3.vectors(std::vector<int>{mx}, Z3.as<double>(), std::vector<long>{0}, s);
+      if (two)
+        twostage_vectors(n3, mx, Z3.as<double>(), s, ts, w3);
+      else
+        w3.vectors(std::vector<int>{mx}, Z3.as<double>(), std::vector<long>{0}, s);
       DevBuf out, dgs, dc0, dnv;
       const long nb = (long)np * iang;
       out.ensure(sizeof(double) * nb * nv3);

@@ -282,8 +282,10 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
       TwoStage ts;
       DevBuf cp, Z, dd;
       const int want = std::min(n, nev + 16);
-      if (!D->nvec && n >= 4 * twostage_bandwidth() && env_twostage()) {
-        twostage_values(Hs, n, want, s, ts, w, cp);
+      bool two = false;
+      if (n >= 4 * twostage_bandwidth() && (D->nvec ? env_twostage_vec() : env_twostage())) {
+        ts.keep = D->nvec != 0;
+        two = twostage_values(Hs, n, want, s, ts, w, cp);
       } else {
         w.tridiag(std::vector<int>{n}, Hs, std::vector<long>{0}, s);
         w.values(std::vector<int>{want}, s);
@@ -296,7 +298,10 @@ extern "C" int dvr_rotlev3(const dvr_rot_dims_t* D, const dvr_rot_block_t* blk,
       }
       const int mx = w.extend(0, nev);
       Z.ensure(sizeof(double) * (size_t)n * mx);
-      w.vectors(std::vector<int>{mx}, Z.as<double>(), std::vector<long>{0}, s);
+      if (two)
+        twostage_vectors(n, mx, Z.as<double>(), s, ts, w);
+      else
+        w.vectors(std::vector<int>{mx}, Z.as<double>(), std::vector<long>{0}, s);
       mark(set == 0 ? "solve1" : "solve2");
       DVR_HIP(hipMemcpy2DAsync(vec + voff, sizeof(double) * n, Z.p, sizeof(double) * n,
                                sizeof(double) * n, nev, hipMemcpyDeviceToHost, s));

@@ -112,50 +112,62 @@ void launch_cint(hipStream_t s, const double* v1, const double* c2,
   DVR_LAUNCH_CHECK();
 }
 
-// psi_l(a, r1, r2) = sum_{j in block c} cint(g, col0+j) * c3(col0+j, l),
-// g = a*nb + b.  theta-first: zr2r1 -> (c=r1, b=r2), else (c=r2, b=r1).
-// One thread per (l, c, g); the inner product runs over the block's columns.
+// Wavefunction amplitudes on the DVR grid (trans, D:3650-3845): for every
+// c block, the (a, b) grid values of the levels are cint_c (G x iv2_c,
+// g = a nb + b) times the block's rows of the 3D eigenvectors, one FP64 MFMA
+// GEMM per block into T[g + G (c + nc l)]; a tiled scatter then writes them
+// in the IWAVE record layout psi(idvr, npnt1, npnt2) per level:
+// theta-first with zr2r1 -> (c = r1, b = r2), else (c = r2, b = r1).
+// T (G x nc) x l: 64 x 64 tiles of (a, b) through LDS, read along b and
+// written along a; blocks without chosen functions stay zero.
 __global__ void __launch_bounds__(256)
-k_trans(const double* __restrict__ cint, const double* __restrict__ c3,
-        double* __restrict__ psi, const CBlock* __restrict__ cb, int nc,
-        int na, int nb, int n3, int nvec, int zr2r1, int idvr, int npnt1,
-        int npnt2) {
+k_trans_scatter(const double* __restrict__ T, int na, int nb, int nc, long nlev,
+                const CBlock* __restrict__ cb, double* __restrict__ psi, long npsi, long sb,
+                long sc) {
+  __shared__ double tl[64][65];
   const long G = (long)na * nb;
-  const long per = G * nc;
-  const long total = per * nvec;
-  const long npsi = (long)idvr * npnt1 * npnt2;
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    const int l = (int)(idx / per);
-    const long rem = idx % per;
-    const int c = (int)(rem / G);
-    const int g = (int)(rem % G);
-    const int a = g / nb;
-    const int b = g % nb;
-    const CBlock B = cb[c];
-    double sum = 0.0;
-    const double* cp = cint + g + G * B.col0;
-    const double* vp = c3 + B.col0 + (long)n3 * l;
-    for (int j = 0; j < B.iv2; ++j) sum += cp[G * j] * vp[j];
-    long out;
-    if (zr2r1)
-      out = a + (long)idvr * (c + (long)npnt1 * b);
-    else
-      out = a + (long)idvr * (b + (long)npnt1 * c);
-    psi[npsi * l + out] = sum;
+  const int nta = (na + 63) / 64;
+  const int a0 = (blockIdx.x % nta) * 64, b0 = (blockIdx.x / nta) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (long y = blockIdx.y; y < (long)nc * nlev; y += gridDim.y) {
+    const int c = (int)(y % nc);
+    const long l = y / nc;
+    if (cb[c].iv2 <= 0) continue;       // uniform over the work-group
+    const double* src = T + G * y;
+    for (int i = ty; i < 64; i += 4) {
+      const int a = a0 + i, b = b0 + tx;
+      if (a < na && b < nb) tl[i][tx] = src[(long)a * nb + b];
+    }
+    __syncthreads();
+    double* dst = psi + npsi * l + sc * c;
+    for (int j = ty; j < 64; j += 4) {
+      const int a = a0 + tx, b = b0 + j;
+      if (a < na && b < nb) dst[a + sb * b] = tl[tx][j];
+    }
+    __syncthreads();
   }
 }
 
-void launch_trans(hipStream_t s, const double* cint, const double* c3,
-                  double* psi, const CBlock* cb, int nc, int na, int nb,
-                  int n3, int nvec, int zr2r1, int idvr, int npnt1,
-                  int npnt2) {
-  const long total = (long)na * nb * nc * nvec;
-  if (total <= 0) return;
-  int grid = (int)((total + 255) / 256);
-  if (grid > 65535 * 16) grid = 65535 * 16;
-  DVR_LAUNCH("trans", 8.0 * total, 2.0 * (double)na * nb * n3 * nvec, k_trans, dim3(grid), dim3(256), 0, s, cint, c3, psi, cb,
-                   nc, na, nb, n3, nvec, zr2r1, idvr, npnt1, npnt2);
+void launch_trans(hipStream_t s, const double* cint, const double* c3, double* psi,
+                  const CBlock* cb, const std::vector<CBlock>& hcb, int nc, int na, int nb,
+                  int n3, int nvec, int zr2r1, int idvr, int npnt1, int npnt2, DevBuf& tmp) {
+  const long G = (long)na * nb;
+  if (G <= 0 || nvec <= 0 || nc <= 0) return;
+  tmp.ensure(sizeof(double) * (size_t)G * nc * nvec);
+  double* T = tmp.as<double>();
+  for (int c = 0; c < nc; ++c) {
+    const CBlock& B = hcb[c];
+    if (B.iv2 <= 0) continue;
+    launch_gemm(s, "trans", false, false, (int)G, nvec, B.iv2, 1.0, cint + G * B.col0, G,
+                c3 + B.col0, n3, 0.0, T + G * c, G * nc, 1, nullptr, false);
+  }
+  const long npsi = (long)idvr * npnt1 * npnt2;
+  const long sb = zr2r1 ? (long)idvr * npnt1 : (long)idvr;
+  const long sc = zr2r1 ? (long)idvr : (long)idvr * npnt1;
+  const unsigned gx = (unsigned)(((na + 63) / 64) * ((nb + 63) / 64));
+  const unsigned gy = (unsigned)std::min<long>((long)nc * nvec, 65535);
+  DVR_LAUNCH("trans_scatter", 16.0 * G * nc * nvec, 0.0, k_trans_scatter, dim3(gx, gy),
+             dim3(256), 0, s, (const double*)T, na, nb, nc, (long)nvec, cb, psi, npsi, sb, sc);
   DVR_LAUNCH_CHECK();
 }
 

@@ -25,9 +25,26 @@
 // block in registers and the block below it in LDS.
 // A panel whose Cholesky QR breaks down (a rank-deficient panel) is reported,
 // and the caller then uses the one-stage reduction.
+//
+// Eigenvectors (dsyev with JOBZ = 'V', the reference's call): inverse
+// iteration gives those of the tridiagonal T for the wanted levels only,
+// and they are carried back through both stages:
+//   * stage 2: the chase keeps every reflector H(s, k) (sweep s, task k,
+//     rows s + 1 + k BW .. s + k BW + BW).  A later sweep's reflector that
+//     overlaps an earlier one has a task index no larger, so the reflectors
+//     of BT consecutive sweeps at one task form a block (g, k), applied in
+//     descending sweep order, and blocks on one wavefront L = (G-1-g) + k
+//     touch disjoint rows: one launch per wavefront.  The vectors are held
+//     transposed (one lane per vector, rows contiguous across lanes), and a
+//     lane keeps the BW rows of the current reflector in a register ring
+//     that slides up one row per reflector;
+//   * stage 1: per panel, in reverse, Y <- Y - V T^T V^T Y in the transposed
+//     layout as three FP64 MFMA GEMMs, after the reflectors of the last
+//     short panel.
 #include "dvr_internal.h"
 
 #include <cmath>
+#include <utility>
 
 namespace dvr {
 
@@ -333,10 +350,11 @@ k_band_extract(const double* __restrict__ A, long lda, int n, double* __restrict
 // 256 threads: thread (row r = tid / 4, quarter q = tid % 4) handles the
 // 16 columns q, q + 4, ... of row r (or the 16 rows of column r); the
 // quarter sums meet through lane shuffles.  All global loads are issued at
-// the start.
+// the start.  With V2 set, the reflector is also kept for the
+// back-transformation at V2[(k nsw + s) BW ...], its factor at tau2[k nsw + s].
 __global__ void __launch_bounds__(256)
 k_sb2st_step(double* __restrict__ AB, int n, int t, int s0, double* __restrict__ Vs,
-             double* __restrict__ taus) {
+             double* __restrict__ taus, double* __restrict__ V2, double* __restrict__ tau2) {
   extern __shared__ double sm[];
   double* O = sm;                // off-diagonal block (ld LD)
   double* vb = O + BW * LD;      // v
@@ -449,6 +467,11 @@ k_sb2st_step(double* __restrict__ AB, int n, int t, int s0, double* __restrict__
   }
   if (q == 0) Vs[(long)s * BW + r] = v;
   if (tid == 0) taus[s] = tau;
+  if (V2) {
+    const long ri = (long)k * (n - 2) + s;
+    if (q == 0) V2[ri * BW + r] = v;
+    if (tid == 0) tau2[ri] = tau;
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -457,6 +480,157 @@ k_band_to_tri(const double* __restrict__ AB, int n, double* __restrict__ d, doub
     d[i] = AB[(long)i * LDB];
     if (i + 1 < n) e[i] = AB[1 + (long)i * LDB];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Back-transformation kernels
+// ---------------------------------------------------------------------------
+
+constexpr int BT = 64;   // sweeps per stage-2 back-transformation block
+constexpr int PF = 4;    // window rows loaded ahead of the ring
+
+static_assert(BW == 64, "the register ring of the back-transformation holds BW = 64 rows");
+
+// Explicit address spaces: row pointers that pass through an empty asm stay
+// global (vector memory instructions), and reflector reads through the
+// constant space stay scalar loads.
+typedef __attribute__((address_space(1))) double gdouble;
+typedef const __attribute__((address_space(4))) double kdouble;
+
+// Reflector of sweep s0 + T on window rows T .. T + BW - 1 (ring slots
+// (T + j) & 63).  Before it, the ring's bottom row T + BW leaves (stored at
+// sp) and row T enters from the prefetch buffer, which then loads row
+// T - PF (at lp).  sp and lp walk up one row per step; they pass through an
+// empty asm so that the compiler keeps one pointer each in VGPRs instead of
+// materialising every row's address.  The reflector is read twice (scalar
+// loads, once for the dot product, once for the update) so that its 64
+// values need not stay in SGPRs across the reduction.
+template <int T>
+__device__ __forceinline__ void bt2_step(double (&z)[BW], double (&pre)[PF], gdouble*& sp,
+                                         gdouble*& lp, long ldt, int s0, int nsw, long krow,
+                                         kdouble* V2, kdouble* tau2) {
+  if (T < BT - 1) {
+    *sp = z[T & (BW - 1)];
+    sp -= ldt;
+    asm volatile("" : "+v"(sp));
+    z[T & (BW - 1)] = pre[T % PF];
+    if (T - PF >= 0) {
+      pre[T % PF] = *lp;
+      lp -= ldt;
+      asm volatile("" : "+v"(lp));
+    }
+  }
+  // slots of sweeps without task k hold v = 0, tau = 0 (zero-filled); past
+  // the last sweep the factor is 0 on a valid slot
+  const int s = s0 + T;
+  const long ri = krow + (s < nsw ? s : nsw - 1);
+  const double tau = tau2[ri] * (s < nsw ? 1.0 : 0.0);
+  kdouble* v = V2 + ri * BW;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+  for (int j = 0; j < BW; j += 4) {
+    a0 = fma(v[j], z[(T + j) & (BW - 1)], a0);
+    a1 = fma(v[j + 1], z[(T + j + 1) & (BW - 1)], a1);
+    a2 = fma(v[j + 2], z[(T + j + 2) & (BW - 1)], a2);
+    a3 = fma(v[j + 3], z[(T + j + 3) & (BW - 1)], a3);
+  }
+  const double f = -tau * ((a0 + a1) + (a2 + a3));
+  asm volatile("" : "+s"(v));
+#pragma unroll
+  for (int j = 0; j < BW; ++j) z[(T + j) & (BW - 1)] = fma(f, v[j], z[(T + j) & (BW - 1)]);
+}
+
+template <int... I>
+__device__ __forceinline__ void bt2_steps(std::integer_sequence<int, I...>, double (&z)[BW],
+                                          double (&pre)[PF], gdouble*& sp, gdouble*& lp, long ldt,
+                                          int s0, int nsw, long krow, kdouble* V2, kdouble* tau2) {
+  (bt2_step<BT - 1 - I>(z, pre, sp, lp, ldt, s0, nsw, krow, V2, tau2), ...);
+}
+
+// One wavefront L of the stage-2 back-transformation on Zt (vector c's row r
+// at Zt[c + r ldt]; ldt a multiple of 64 and at least BT + BW - 1 rows below
+// row n - 1, all zero, as are the columns nv .. ldt - 1).  Work-group
+// (x, y): block (g, k) with k = klo + x, g = G - 1 - (L - k), vectors
+// 64 y .. 64 y + 63.  The block's window is rows R0 = g BT + 1 + k BW ..
+// R0 + BT + BW - 2; sweep s0 + T of the block acts on window rows
+// T .. T + BW - 1.  Window rows past n - 1 meet only zero reflector entries
+// (a reflector ends at row n - 1), so the padding stays zero and no lane or
+// row needs a guard.
+__global__ void __launch_bounds__(64)
+k_bt2(double* __restrict__ Zt, long ldt, int n, int L, int G, int klo,
+      const double* __restrict__ V2, const double* __restrict__ tau2) {
+  const int nsw = n - 2;
+  const int k = klo + blockIdx.x;
+  const int g = G - 1 - (L - k);
+  const int s0 = g * BT;
+  // the group's first sweep has the most tasks
+  if (g < 0 || s0 >= nsw || k > (n - 2 - s0) / BW) return;
+  const long R0 = (long)s0 + 1 + (long)k * BW;
+  double* zb = Zt + (long)(blockIdx.y * 64 + threadIdx.x) + R0 * ldt;
+  double z[BW], pre[PF];
+#pragma unroll
+  for (int r = 0; r < BW; ++r) z[(BT - 1 + r) & (BW - 1)] = zb[(BT - 1 + r) * ldt];
+#pragma unroll
+  for (int q = 1; q <= PF; ++q) pre[(BT - 1 - q) % PF] = zb[(BT - 1 - q) * ldt];
+  gdouble* sp = (gdouble*)(zb + (BT - 2 + BW) * ldt);   // row leaving at the second step
+  gdouble* lp = (gdouble*)(zb + (BT - 2 - PF) * ldt);   // row loaded at the second step
+  bt2_steps(std::make_integer_sequence<int, BT>{}, z, pre, sp, lp, ldt, s0, nsw, (long)k * nsw,
+            (kdouble*)V2, (kdouble*)tau2);
+#pragma unroll
+  for (int r = 0; r < BW; ++r) zb[r * ldt] = z[r & (BW - 1)];
+}
+
+// V of the panel at column j0 (m x BW, ld m) from the reduced A: below the
+// diagonal of the panel's top block and all rows under it as stored, unit
+// diagonal, zero above.
+__global__ void __launch_bounds__(256)
+k_panel_v(const double* __restrict__ A, long lda, int j0, int m, double* __restrict__ V) {
+  const long tot = (long)m * BW;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < tot; t += (long)gridDim.x * 256) {
+    const int i = (int)(t % m), c = (int)(t / m);
+    V[t] = i < c ? 0.0 : (i == c ? 1.0 : A[(j0 + BW + i) + (long)(j0 + c) * lda]);
+  }
+}
+
+// The reflectors of the last short panel (m < BW rows below it, column j0),
+// last one first, on Zt: thread = vector.  Reflector c acts on rows
+// j0 + BW + c .. j0 + BW + m - 1, v = (1, A(below)), factor tau[c].
+__global__ void __launch_bounds__(256)
+k_bt1_last(double* __restrict__ Zt, long ldt, int nv, const double* __restrict__ A, long lda,
+           int j0, int m, const double* __restrict__ tau) {
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  if (col >= nv) return;
+  const long r0 = j0 + BW;
+  double* zc = Zt + col;
+  for (int c = m - 2; c >= 0; --c) {
+    const double tv = tau[c];
+    if (tv == 0.0) continue;
+    const double* v = A + r0 + (long)(j0 + c) * lda;
+    double d = zc[(r0 + c) * ldt];
+    for (int t = c + 1; t < m; ++t) d = fma(v[t], zc[(r0 + t) * ldt], d);
+    d *= tv;
+    zc[(r0 + c) * ldt] -= d;
+    for (int t = c + 1; t < m; ++t) zc[(r0 + t) * ldt] = fma(-d, v[t], zc[(r0 + t) * ldt]);
+  }
+}
+
+// B (cols x rows, ld ldb) = A^T, A rows x cols (ld lda), in 64 x 64 tiles
+// through LDS.
+__global__ void __launch_bounds__(256)
+k_transpose(const double* __restrict__ A, long lda, int rows, int cols, double* __restrict__ B,
+            long ldb) {
+  __shared__ double tl[64][65];
+  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int c = ty; c < 64; c += 4)
+    if (r0 + tx < rows && c0 + c < cols) tl[c][tx] = A[(r0 + tx) + (long)(c0 + c) * lda];
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4)
+    if (c0 + tx < cols && r0 + r < rows) B[(c0 + tx) + (long)(r0 + r) * ldb] = tl[tx][r];
+}
+
+__global__ void __launch_bounds__(256) k_zero(double* __restrict__ x, long n) {
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256) x[t] = 0.0;
 }
 
 int grid_of(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
@@ -572,6 +746,12 @@ int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream
   if (mlast > 0)
     DVR_LAUNCH("s1_last", 8.0 * 2 * (double)(mlast + BW) * (mlast + BW), 0.0, k_last_panel, dim3(1),
                dim3(256), last_panel_lds(), s, A, lda, j0, mlast, taul.as<double>());
+  Ared = A;
+  lda_red = lda;
+  n_red = n;
+  npanel = p;
+  jlast = j0;
+  this->mlast = mlast;
   // ---- stage 2
   AB.ensure(sizeof(double) * (size_t)n * LDB);
   vs.ensure(sizeof(double) * (size_t)n * BW);
@@ -580,6 +760,20 @@ int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream
              dim3(256), 0, s, A, lda, n, AB.as<double>());
   const int nsweep = n - 2;   // sweeps 0 .. n - 3
   auto kmax = [&](int sw) { return (n - 2 - sw) / BW; };   // last task with r0 <= n - 1
+  double* v2 = nullptr;
+  double* t2 = nullptr;
+  if (keep) {
+    // one slot per (task, sweep); slots without a task stay zero
+    const size_t nslot = (size_t)(kmax(0) + 1) * nsweep;
+    V2.ensure(sizeof(double) * nslot * BW);
+    tau2.ensure(sizeof(double) * nslot);
+    v2 = V2.as<double>();
+    t2 = tau2.as<double>();
+    DVR_LAUNCH("s2_zero", 8.0 * nslot * BW, 0.0, k_zero, dim3(grid_of((long)nslot * BW)),
+               dim3(256), 0, s, v2, (long)(nslot * BW));
+    DVR_LAUNCH("s2_zero", 8.0 * nslot, 0.0, k_zero, dim3(grid_of((long)nslot)), dim3(256), 0, s,
+               t2, (long)nslot);
+  }
   const int tmax = LAG * (nsweep - 1) + kmax(0);
   int slo = 0;
   for (int t = 0; t <= tmax; ++t) {
@@ -587,7 +781,7 @@ int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream
     while (slo <= shi && t - LAG * slo > kmax(slo)) ++slo;
     if (slo > shi) continue;
     DVR_LAUNCH("s2_chase", 0.0, 0.0, k_sb2st_step, dim3(shi - slo + 1), dim3(256), sb2st_lds(), s,
-               AB.as<double>(), n, t, slo, vs.as<double>(), taus.as<double>());
+               AB.as<double>(), n, t, slo, vs.as<double>(), taus.as<double>(), v2, t2);
   }
   DVR_LAUNCH("s2_tri", 16.0 * n, 0.0, k_band_to_tri, dim3(grid_of(n)), dim3(256), 0, s,
              AB.as<double>(), n, d, e);
@@ -596,6 +790,93 @@ int TwoStage::reduce(double* A, long lda, int n, double* d, double* e, hipStream
   DVR_HIP(hipMemcpyAsync(&hf, flags.p, sizeof(int), hipMemcpyDeviceToHost, s));
   DVR_HIP(hipStreamSynchronize(s));
   return hf ? 1 : 0;
+}
+
+void TwoStage::back(double* Z, long ldz, int nv, hipStream_t s) {
+  if (!keep || !Ared || !V2.p) throw HipError("two-stage back-transformation: reflectors not kept");
+  const int n = n_red;
+  if (nv <= 0) return;
+  // transposed vectors, columns padded to a multiple of 64 and BT + BW rows
+  // of zeros below the last row (the wavefront kernel's windows run into them)
+  const long ldt = (nv + 63) / 64 * 64;
+  const long zrows = (long)n + BT + BW;
+  Zt.ensure(sizeof(double) * (size_t)(ldt * zrows));
+  double* zt = Zt.as<double>();
+  DVR_LAUNCH("bt_zero", 8.0 * ldt * zrows, 0.0, k_zero, dim3(grid_of(ldt * zrows)), dim3(256), 0, s,
+             zt, ldt * zrows);
+  const dim3 tg((n + 63) / 64, (nv + 63) / 64);
+  DVR_LAUNCH("bt_transpose", 16.0 * (double)n * nv, 0.0, k_transpose, tg, dim3(256), 0, s,
+             (const double*)Z, ldz, n, nv, zt, ldt);
+  // ---- stage 2: wavefronts L = 0 .. G - 1 + K, blocks (g, k) with
+  // g = G - 1 - (L - k) and k <= kmax(g BT)
+  const int nsw = n - 2;
+  const int K = (n - 2) / BW;
+  const int G = (nsw + BT - 1) / BT;
+  const unsigned ny = (unsigned)(ldt / 64);
+  for (int L = 0; L <= G - 1 + K; ++L) {
+    const int klo = std::max(0, L - (G - 1));
+    int khi = std::min(K, L);
+    // last k whose block is not empty (kmax falls as g, and so k, grows)
+    while (khi >= klo && khi > (n - 2 - (G - 1 - (L - khi)) * BT) / BW) --khi;
+    if (khi < klo) continue;
+    const double nblk = (double)(khi - klo + 1);
+    DVR_LAUNCH("bt2", nblk * ldt * 16.0 * (BT + BW - 1), nblk * ldt * BT * 4.0 * BW, k_bt2,
+               dim3((unsigned)(khi - klo + 1), ny), dim3(64), 0, s, zt, ldt, n, L, G, klo,
+               (const double*)V2.as<double>(), (const double*)tau2.as<double>());
+  }
+  // ---- stage 1: the last short panel, then the panels in reverse
+  if (mlast > 1)
+    DVR_LAUNCH("bt1_last", 16.0 * mlast * nv, 2.0 * mlast * mlast * nv, k_bt1_last, dim3((nv + 255) / 256),
+               dim3(256), 0, s, zt, ldt, nv, (const double*)Ared, lda_red, jlast, mlast,
+               (const double*)taul.as<double>());
+  if (npanel > 0) {
+    const int m0 = n - BW;
+    size_t wneed = 1;
+    for (int p = 0; p < npanel; ++p) {
+      const int m = n - p * BW - BW;
+      const int shp[3][3] = {{nv, BW, m}, {nv, BW, BW}, {nv, m, BW}};
+      for (const auto& x : shp)
+        wneed = std::max(wneed, gemm_work_doubles(x[0], x[1], gemm_auto_ksplit(x[0], x[1], x[2])));
+    }
+    work.ensure(sizeof(double) * wneed);
+    Vp.ensure(sizeof(double) * (size_t)m0 * BW);
+    bw1.ensure(sizeof(double) * (size_t)nv * BW);
+    bw2.ensure(sizeof(double) * (size_t)nv * BW);
+    double* W = work.as<double>();
+    auto gemm = [&](const char* name, bool ta, bool tb, int M_, int N_, int K_, double alpha,
+                    const double* A_, long lda_, const double* B_, long ldb_, double beta,
+                    double* C_, long ldc_) {
+      const int ks = gemm_auto_ksplit(M_, N_, K_);
+      if (gemm_work_doubles(M_, N_, ks) > wneed)
+        throw HipError("two-stage back-transformation: split-K workspace too small");
+      launch_gemm(s, name, ta, tb, M_, N_, K_, alpha, A_, lda_, B_, ldb_, beta, C_, ldc_, ks,
+                  ks > 1 ? W : nullptr, false);
+    };
+    for (int p = npanel - 1; p >= 0; --p) {
+      const int j0 = p * BW;
+      const int m = n - j0 - BW;
+      double* V = Vp.as<double>();
+      DVR_LAUNCH("bt1_v", 16.0 * m * BW, 0.0, k_panel_v, dim3(grid_of((long)m * BW)), dim3(256), 0,
+                 s, (const double*)Ared, lda_red, j0, m, V);
+      double* Y = zt + (long)(j0 + BW) * ldt;
+      const double* T = Tst.as<double>() + (size_t)p * BW * BW;
+      // Y <- Y - (Y V) T^T V^T   (Yt = rows of the transposed vectors)
+      gemm("bt1_yv", false, false, nv, BW, m, 1.0, Y, ldt, V, m, 0.0, bw1.as<double>(), nv);
+      gemm("bt1_t", false, true, nv, BW, BW, 1.0, bw1.as<double>(), nv, T, BW, 0.0,
+           bw2.as<double>(), nv);
+      gemm("bt1_upd", false, true, nv, m, BW, -1.0, bw2.as<double>(), nv, V, m, 1.0, Y, ldt);
+    }
+  }
+  const dim3 tb((nv + 63) / 64, (n + 63) / 64);
+  DVR_LAUNCH("bt_transpose", 16.0 * (double)n * nv, 0.0, k_transpose, tb, dim3(256), 0, s,
+             (const double*)zt, ldt, nv, n, Z, ldz);
+  DVR_LAUNCH_CHECK();
+}
+
+void twostage_vectors(int n, int m, double* Z, hipStream_t s, TwoStage& ts, EigWork& W) {
+  W.vectors(std::vector<int>{m}, Z, std::vector<long>{0}, s, false);
+  ts.back(Z, n, m, s);
+  DVR_HIP(hipStreamSynchronize(s));
 }
 
 bool twostage_values(const double* A, int n, int want, hipStream_t s, TwoStage& ts, EigWork& W,

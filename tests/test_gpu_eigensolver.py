@@ -104,6 +104,7 @@ def test_syev_blocked_reduction(monkeypatch, sizes, solver_kind):
     from dvr3d_amd import _lib
     rng = np.random.default_rng(sum(sizes) + 5)
     mats = [_sym(nn, rng) for nn in sizes]
+    monkeypatch.setenv("DVR_TWOSTAGE_VEC", "0")    # eigenvectors from the one-stage reduction
     monkeypatch.setenv("DVR_TRI", "blocked")
     ws, zs = _lib.syev_batch(mats)
     for a, w, z in zip(mats, ws, zs):
@@ -128,3 +129,56 @@ def test_syev_batch_partial_and_clusters():
     ws, zs = _lib.syev_batch([a, a[:120, :120]], nvec=[40, 7])
     _check(a, ws[0], zs[0], 40, tol=2e-12)
     _check(a[:120, :120], ws[1], zs[1], 7, tol=2e-12)
+
+
+@pytest.mark.parametrize("sizes,nvec", [([256], [256]), ([300, 280], [300, 50]), ([517], [200]),
+                                        ([1027], [333]), ([2203], [100])])
+def test_syev_twostage_vectors(monkeypatch, sizes, nvec, solver_kind):
+    """Eigenvectors through the two-stage reduction (every problem of order
+    >= 256): inverse iteration on the tridiagonal matrix, then back through
+    the bulge chase (blocks of 64 sweeps on wavefronts) and the band
+    reduction (three GEMMs per panel).  Against LAPACK (values, residuals,
+    orthogonality), with all vectors of a 256 matrix and partial sets at
+    sizes whose last panel is short (300, 517, 1027, 2203), and against the
+    one-stage reduction's vectors (DVR_TWOSTAGE_VEC=0) up to sign."""
+    if solver_kind != "multi":
+        pytest.skip("the two-stage path does not depend on the one-stage solver kind")
+    from dvr3d_amd import _lib
+    rng = np.random.default_rng(sum(sizes) + 11)
+    mats = [_sym(nn, rng) for nn in sizes]
+    ws, zs = _lib.syev_batch(mats, nvec=nvec)
+    for a, w, z, nv in zip(mats, ws, zs, nvec):
+        _check(a, w, z, nv)
+    monkeypatch.setenv("DVR_TWOSTAGE_VEC", "0")
+    ws0, zs0 = _lib.syev_batch(mats, nvec=nvec)
+    for a, w, z, w0, z0 in zip(mats, ws, zs, ws0, zs0):
+        nrm = max(1.0, np.abs(w0).max())
+        assert np.abs(w - w0).max() <= 1e-12 * nrm * a.shape[0] ** 0.5
+        # well separated levels: the same vector up to sign
+        gap = np.minimum(np.diff(np.r_[-np.inf, w0]), np.diff(np.r_[w0, np.inf]))
+        sep = gap > 1e-4 * nrm
+        assert sep.sum() >= len(w0) // 2
+        ov = np.abs(np.sum(z * z0, axis=0))
+        assert np.abs(ov[sep] - 1.0).max() <= 1e-9
+
+
+def test_syev_twostage_vectors_clusters(solver_kind):
+    """Two-stage eigenvectors of a 400 matrix with a near-degenerate pair, an
+    exact triple and an already tridiagonal 300 matrix (every reflector the
+    identity) among the wanted levels."""
+    if solver_kind != "multi":
+        pytest.skip("the two-stage path does not depend on the one-stage solver kind")
+    from dvr3d_amd import _lib
+    rng = np.random.default_rng(17)
+    n = 400
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.sort(rng.uniform(0.0, 1.0, n))
+    lam[10] = lam[9] + 1e-13
+    lam[20:23] = lam[19]
+    lam = np.sort(lam)
+    a = (q * lam) @ q.T
+    a = 0.5 * (a + a.T)
+    t = np.diag([2.0] * 300) + np.diag([-1.0] * 299, -1) + np.diag([-1.0] * 299, 1)
+    ws, zs = _lib.syev_batch([a, t], nvec=[40, 30])
+    _check(a, ws[0], zs[0], 40, tol=2e-12)
+    _check(t, ws[1], zs[1], 30, tol=2e-12)
