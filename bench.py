@@ -867,6 +867,8 @@ def main():
     ap.add_argument("--train-steps", type=int, default=40,
                     help="c4: training steps of the learned preconditioner per dense instance")
     ap.add_argument("--solver", default="davidson", choices=["dense", "davidson"])
+    ap.add_argument("--nvec", type=int, default=0,
+                    help="eigenvectors (and wavefunction records) of the timed solve")
     ap.add_argument("--instances", type=int, default=256, help="c4: number of instances")
     ap.add_argument("--no-vectors-leg", action="store_true",
                     help="skip the chosen-vs-plain comparison with every level's eigenvector")
@@ -1000,7 +1002,7 @@ def main():
         return out
 
     progress(f"{cfg}: {args.solver} solve, {args.warmup} warm-up + {args.steps} timed steps")
-    main_run = measure(args.solver, args.steps, args.warmup, True)
+    main_run = measure(args.solver, args.steps, args.warmup, True, nvec=args.nvec)
     progress(f"{cfg}: {args.solver} {main_run['ms_per_step']:.1f} ms per step")
     ablation = None
     if args.solver == "davidson" and not args.no_ablation:
@@ -1042,7 +1044,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{cfg}: DVR3DRJZ theta-first J=0, npnt={job.npnt1}, nalf={job.nalf}, "
                                f"nham3={b.sz.max3d}, lowest {b.sz.neval} levels",
-                   "solver": args.solver, "levels": main_run["levels"],
+                   "solver": args.solver, "eigenvectors": args.nvec, "levels": main_run["levels"],
                    "converged": main_run["converged"], "max_err_cm": main_run["max_err_cm"],
                    "stage_ms": main_run["stage_ms"],
                    "solver_stats": main_run.get("solver_stats")},

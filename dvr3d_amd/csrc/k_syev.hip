@@ -1041,6 +1041,278 @@ k_stein_lds(EigBatch b, const Cluster* __restrict__ cl, int ncl,
                                 (long)nm * TW, TW, Z + zoff[p], (unsigned long long)p);
 }
 
+// Inverse iteration with one wavefront per cluster, for problems whose
+// scratch does not fit in LDS (large n): the same arithmetic as
+// inverse_iteration_cluster, with the recurrences (factorisation and the two
+// sweeps of each of the three iterations) on lane 0, but their operands
+// moving through LDS in chunks of SW elements that the whole wavefront
+// loads and stores with coalesced accesses, the next chunk's loads in flight
+// while lane 0 works on the current one.  Scaling, Gram-Schmidt against the
+// cluster's earlier vectors, norms and the final store are wavefront-
+// parallel.  With one thread per cluster (k_stein) a c3-size problem (20000
+// rows, 1000 vectors) runs 16 wavefronts whose every burst waits on memory.
+// Per cluster the scratch arrays FL, FD, FU, FU2, X, PV (nmax doubles each)
+// are contiguous.
+constexpr int SW = 256;
+constexpr int SWL = SW / 64;
+
+__device__ inline double wave_bcast_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return __shfl(v, 0, 64);
+}
+
+__global__ void __launch_bounds__(64)
+k_stein_wave(EigBatch b, const Cluster* __restrict__ cl, int ncl,
+             const double* __restrict__ eval, const long* __restrict__ eoff,
+             double* __restrict__ Z, const long* __restrict__ zoff,
+             double* __restrict__ scratch) {
+  __shared__ double in[4][SW + 1];
+  __shared__ double out[5][SW];
+  __shared__ double bc[2];
+  const int ci = blockIdx.x;
+  if (ci >= ncl) return;
+  const int lane = threadIdx.x;
+  const Cluster C = cl[ci];
+  const int p = C.p;
+  const int n = b.n[p];
+  const long vo = b.voff[p];
+  const double* __restrict__ d = b.d + vo;
+  const double* __restrict__ e = b.e + vo;
+  const double* __restrict__ lams = eval + eoff[p];
+  double* __restrict__ Zp = Z + zoff[p];
+  const long nm = b.nmax;
+  double* FL = scratch + (long)ci * 6 * nm;
+  double* FD = FL + nm;
+  double* FU = FD + nm;
+  double* FU2 = FU + nm;
+  double* X = FU2 + nm;
+  double* PV = X + nm;
+  const double tnn = b.tnrm[p] > 0.0 ? b.tnrm[p] : 1.0;
+  const double pertol = 10.0 * kEps * tnn, tiny = kEps * tnn;
+  const int nm1 = n - 1;
+  const unsigned long long seed = (unsigned long long)p;
+  // chunk [lo, lo + cnt) of a into registers (lane-strided), zero past cnt
+  auto ldc = [&](const double* a, long lo, int cnt, double(&r)[SWL]) {
+#pragma unroll
+    for (int t = 0; t < SWL; ++t) {
+      const int q = lane + 64 * t;
+      r[t] = q < cnt ? a[lo + q] : 0.0;
+    }
+  };
+  auto stl = [&](double* sdst, const double(&r)[SWL]) {
+#pragma unroll
+    for (int t = 0; t < SWL; ++t) sdst[lane + 64 * t] = r[t];
+  };
+  double lprev = 0.0;
+  for (int k = C.kb; k < C.ke; ++k) {
+    double lam = lams[k];
+    if (k > C.kb && lam - lprev < pertol) lam = lprev + pertol;
+    lprev = lam;
+    // ---- factorisation T - lam I = P L U (the dgttrf recurrence)
+    double dc = d[0] - lam, uc = n > 1 ? e[0] : 0.0;
+    {
+      double re[SWL], rd[SWL], rx = 0.0;
+      auto pre = [&](int j0) {
+        const int cnt = min(SW, nm1 - j0);
+        ldc(e, j0, cnt, re);
+        ldc(d + 1, j0, cnt, rd);
+        rx = (lane == 0 && j0 + SW < nm1) ? e[j0 + SW] : 0.0;
+      };
+      if (nm1 > 0) pre(0);
+      for (int i0 = 0; i0 < nm1; i0 += SW) {
+        const int cnt = min(SW, nm1 - i0);
+        __syncthreads();
+        stl(in[0], re);
+        stl(in[1], rd);
+        if (lane == 0) in[0][SW] = rx;
+        __syncthreads();
+        if (i0 + SW < nm1) pre(i0 + SW);
+        if (lane == 0) {
+          #pragma unroll 4
+          for (int q = 0; q < cnt; ++q) {
+            const int i = i0 + q;
+            const double li = in[0][q];
+            const double dn = in[1][q] - lam;
+            const double un = (i < n - 2) ? in[0][q + 1] : 0.0;
+            const bool sw = fabs(dc) < fabs(li);
+            const double den = sw ? li : dc, num = sw ? dc : li;
+            const double rden = rcp_refined(den == 0.0 ? tiny : den);
+            const double fl = num * rden;
+            out[0][q] = fl;
+            out[1][q] = rden;
+            out[2][q] = sw ? dn : uc;
+            out[3][q] = sw ? un : 0.0;
+            out[4][q] = sw ? 1.0 : 0.0;
+            dc = sw ? uc - fl * dn : dn - fl * uc;
+            uc = sw ? -fl * un : un;
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < SWL; ++t) {
+          const int q = lane + 64 * t;
+          if (q < cnt) {
+            FL[i0 + q] = out[0][q];
+            FD[i0 + q] = out[1][q];
+            FU[i0 + q] = out[2][q];
+            FU2[i0 + q] = out[3][q];
+            PV[i0 + q] = out[4][q];
+          }
+        }
+      }
+      if (lane == 0) FD[n - 1] = 1.0 / (dc == 0.0 ? tiny : dc);
+    }
+    __threadfence();
+    __syncthreads();
+    double scale = 1.0;
+    for (int it = 0; it < 3; ++it) {
+      const unsigned long long base = (seed << 40) ^ ((unsigned long long)k << 20);
+      // ---- forward sweep: L^-1 with the row interchanges
+      double xc = it == 0 ? ii_hash(base) : X[0] * scale;
+      {
+        double rl[SWL], rp[SWL], rn[SWL];
+        auto pre = [&](int j0) {
+          const int cnt = min(SW, nm1 - j0);
+          ldc(FL, j0, cnt, rl);
+          ldc(PV, j0, cnt, rp);
+          if (it == 0) {
+#pragma unroll
+            for (int t = 0; t < SWL; ++t)
+              rn[t] = ii_hash(base ^ (unsigned long long)(j0 + lane + 64 * t + 1));
+          } else {
+            ldc(X + 1, j0, cnt, rn);
+#pragma unroll
+            for (int t = 0; t < SWL; ++t) rn[t] *= scale;
+          }
+        };
+        if (nm1 > 0) pre(0);
+        for (int i0 = 0; i0 < nm1; i0 += SW) {
+          const int cnt = min(SW, nm1 - i0);
+          __syncthreads();
+          stl(in[0], rl);
+          stl(in[1], rp);
+          stl(in[2], rn);
+          __syncthreads();
+          if (i0 + SW < nm1) pre(i0 + SW);
+          if (lane == 0) {
+            #pragma unroll 4
+            for (int q = 0; q < cnt; ++q) {
+              const bool sflag = in[1][q] != 0.0;
+              const double xn = in[2][q];
+              const double a = sflag ? xn : xc, bq = sflag ? xc : xn;
+              xc = bq - in[0][q] * a;
+              out[0][q] = a;
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int t = 0; t < SWL; ++t) {
+            const int q = lane + 64 * t;
+            if (q < cnt) X[i0 + q] = out[0][q];
+          }
+        }
+      }
+      __threadfence();
+      __syncthreads();
+      // ---- backward sweep: U^-1, tracking max |x| (lane 0)
+      double x1 = 0.0, x0 = 0.0, mxv = 0.0;
+      if (lane == 0) {
+        x1 = xc * FD[n - 1];
+        X[n - 1] = x1;
+        mxv = fabs(x1);
+        if (n > 1) {
+          x0 = (X[n - 2] - FU[n - 2] * x1) * FD[n - 2];
+          X[n - 2] = x0;
+          mxv = fmax(mxv, fabs(x0));
+        }
+      }
+      {
+        double rx[SWL], ru[SWL], ru2[SWL], rf[SWL];
+        auto pre = [&](int hi) {
+          const int lo = max(0, hi - SW + 1), cnt = hi - lo + 1;
+          ldc(X, lo, cnt, rx);
+          ldc(FU, lo, cnt, ru);
+          ldc(FU2, lo, cnt, ru2);
+          ldc(FD, lo, cnt, rf);
+        };
+        if (n - 3 >= 0) pre(n - 3);
+        for (int i1 = n - 3; i1 >= 0; i1 -= SW) {
+          const int lo = max(0, i1 - SW + 1), cnt = i1 - lo + 1;
+          __syncthreads();
+          stl(in[0], rx);
+          stl(in[1], ru);
+          stl(in[2], ru2);
+          stl(in[3], rf);
+          __syncthreads();
+          if (i1 - SW >= 0) pre(i1 - SW);
+          if (lane == 0) {
+            #pragma unroll 4
+            for (int q = cnt - 1; q >= 0; --q) {
+              const double xi = (in[0][q] - in[1][q] * x0 - in[2][q] * x1) * in[3][q];
+              mxv = fmax(mxv, fabs(xi));
+              x1 = x0;
+              x0 = xi;
+              out[0][q] = xi;
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int t = 0; t < SWL; ++t) {
+            const int q = lane + 64 * t;
+            if (q < cnt) X[lo + q] = out[0][q];
+          }
+        }
+      }
+      if (lane == 0) bc[0] = mxv;
+      __threadfence();
+      __syncthreads();
+      mxv = bc[0];
+      const double sc = mxv > 0.0 ? 1.0 / mxv : 1.0;
+      if (k > C.kb) {
+        for (int i = lane; i < n; i += 64) X[i] *= sc;
+        for (int q = C.kb; q < k; ++q) {
+          const double* zq = Zp + (long)q * n;
+          double dot = 0.0;
+          for (int i = lane; i < n; i += 64) dot += X[i] * zq[i];
+          dot = wave_bcast_sum(dot);
+          for (int i = lane; i < n; i += 64) X[i] -= dot * zq[i];
+        }
+        double nrm = 0.0;
+        for (int i = lane; i < n; i += 64) nrm += X[i] * X[i];
+        nrm = wave_bcast_sum(nrm);
+        scale = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
+      } else {
+        double nrm = 0.0;
+        for (int i = lane; i < n; i += 64) {
+          const double t = X[i] * sc;
+          nrm += t * t;
+        }
+        nrm = wave_bcast_sum(nrm);
+        scale = nrm > 0.0 ? sc / sqrt(nrm) : 0.0;
+      }
+      __threadfence();
+      __syncthreads();
+    }
+    double* zk = Zp + (long)k * n;
+    for (int i = lane; i < n; i += 64) zk[i] = X[i] * scale;
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// One wavefront per cluster where the clusters are too few for the
+// one-thread-per-cluster kernel to fill the GPU (at most 4096, i.e. 64 of its
+// wavefronts: c3's 3D solve, 980 clusters of 20000 rows, 234 -> 30 ms); a
+// batch with more (c3's 2D stage, ~19000 clusters of <= 1116 rows) keeps
+// one thread each (16.0 ms there against 18.8 ms).  DVR_STEIN=thread (or
+// global) never takes the wavefront kernel, DVR_STEIN=wave always does.
+bool stein_use_wave(int ncl) {
+  const char* v = getenv("DVR_STEIN");
+  const std::string t = v ? v : "";
+  if (t == "thread" || t == "global") return false;
+  return t == "wave" || ncl <= 4096;
+}
+
 void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
                   int ncl, const double* eval, const long* eoff, double* Z,
                   const long* zoff, double* scratch, int nthreads) {
@@ -1052,6 +1324,12 @@ void launch_stein(hipStream_t s, const EigBatch& b, const Cluster* cl,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     DVR_LAUNCH("stein", 0.0, 0.0, k_stein_lds, dim3((ncl + tw - 1) / tw), dim3(64), lds, s, b, cl,
                ncl, eval, eoff, Z, zoff, tw);
+    DVR_LAUNCH_CHECK();
+    return;
+  }
+  if (stein_use_wave(ncl) && nthreads >= ncl) {
+    DVR_LAUNCH("stein", 0.0, 0.0, k_stein_wave, dim3(ncl), dim3(64), 0, s, b, cl, ncl, eval, eoff,
+               Z, zoff, scratch);
     DVR_LAUNCH_CHECK();
     return;
   }
