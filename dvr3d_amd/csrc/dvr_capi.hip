@@ -362,8 +362,8 @@ void run_pipeline(dvr_ctx_t* C) {
   C->cint.ensure(sizeof(double) * (size_t)G * n3);
   const CBlock* d_cb = upload(C->cblk, cb, s);
   const int* d_blk3 = upload(C->blk3, blk3, s);
-  launch_cint(s, C->v1.as<double>(), C->Z2.as<double>(), C->cint.as<double>(),
-              d_cb, d_blk3, d_iv1off, na, nb, n3);
+  launch_cint(s, C->v1.as<double>(), C->Z2.as<double>(), C->cint.as<double>(), d_cb, cb,
+              d_iv1off, na, nb);
   C->H3.ensure(sizeof(double) * (size_t)n3 * n3);
   {
     const int TM = syrk_tile();

@@ -152,10 +152,10 @@ struct CBlock {
 // Expansion of the chosen 2D vectors on the (a, b) grid of each c block:
 //   cint[g + G*(col0+j)] = sum_{k<iv1(c,b)} v1[(rowbase+off+k)*na + a]
 //                                          * c2[z2off + off+k + j*nham2]
-// with g = a*nb + b and off = iv1off[c*(nb+1)+b].
-void launch_cint(hipStream_t s, const double* v1, const double* c2,
-                 double* cint, const CBlock* cb, const int* blk3,
-                 const int* iv1off, int na, int nb, int n3);
+// with g = b*na + a and off = iv1off[c*(nb+1)+b]; hcb: the blocks on the host.
+void launch_cint(hipStream_t s, const double* v1, const double* c2, double* cint,
+                 const CBlock* cb, const std::vector<CBlock>& hcb, const int* iv1off, int na,
+                 int nb);
 
 struct DevBuf;
 // Wavefunction amplitudes of nvec levels on the DVR grid, in the IWAVE

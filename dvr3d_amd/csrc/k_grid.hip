@@ -72,79 +72,103 @@ void launch_pack_1d(hipStream_t s, const double* v1full, const double* e1full,
   DVR_LAUNCH_CHECK();
 }
 
-// One thread per (g, column) of cint.  Consecutive threads walk g with b
-// fastest, matching the column-major cint layout, so the stores coalesce.
+// cint (mkham3, D:2362-2411): the chosen 2D vectors of every c block on its
+// (a, b) grid, row g = b na + a.  For each (c, b) the rows b na .. b na + na - 1
+// of the block's columns are the product of the kept 1D vectors of (c, b)
+// (na x cnt, rows of v1) and rows off .. off + cnt - 1 of the block's 2D
+// vectors (cnt x iv2_c): one work-group per (64 columns, b, c), both factors
+// staged through LDS in chunks of CI_K 1D levels, each thread one a and 16
+// columns per 64-row slice of a, every element written once, coalesced along
+// a.  The row order is free: the assembly and the refinement only contract
+// over g, and the wavefunction scatter reads it back in this order.
+constexpr int CI_K = 32;
+
 __global__ void __launch_bounds__(256)
 k_cint(const double* __restrict__ v1, const double* __restrict__ c2,
        double* __restrict__ cint, const CBlock* __restrict__ cb,
-       const int* __restrict__ blk3, const int* __restrict__ iv1off, int na,
-       int nb, int n3) {
+       const int* __restrict__ iv1off, int na, int nb) {
+  __shared__ double As[CI_K][64];
+  __shared__ double Bs[CI_K][65];
+  const int c = blockIdx.z, b = blockIdx.y;
+  const CBlock B = cb[c];
+  const int j0 = blockIdx.x * 64;
+  if (j0 >= B.iv2) return;                       // uniform over the work-group
+  const int jn = min(64, B.iv2 - j0);
+  const int off = iv1off[c * (nb + 1) + b];
+  const int cnt = iv1off[c * (nb + 1) + b + 1] - off;
   const long G = (long)na * nb;
-  const long total = G * n3;
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (long)gridDim.x * blockDim.x) {
-    const int col = (int)(idx / G);
-    const int g = (int)(idx % G);
-    const int a = g / nb;
-    const int b = g % nb;
-    const int c = blk3[col];
-    const CBlock B = cb[c];
-    const int j = col - B.col0;
-    const int off = iv1off[c * (nb + 1) + b];
-    const int cnt = iv1off[c * (nb + 1) + b + 1] - off;
-    const double* vr = v1 + (long)(B.rowbase + off) * na + a;
-    const double* cr = c2 + B.z2off + off + (long)j * B.nham2;
-    double sum = 0.0;
-    for (int k = 0; k < cnt; ++k) sum += cr[k] * vr[(long)k * na];
-    cint[idx] = sum;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const double* vrow = v1 + (long)(B.rowbase + off) * na;        // 1D vector k at vrow + k na
+  const double* c2b = c2 + B.z2off + off + (long)j0 * B.nham2;   // (k, j) at c2b + k + j nham2
+  double* out = cint + (long)b * na + G * (B.col0 + j0);         // (a, j) at out + a + j G
+  for (int a0 = 0; a0 < na; a0 += 64) {
+    double acc[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+    for (int k0 = 0; k0 < cnt; k0 += CI_K) {
+      const int kc = min(CI_K, cnt - k0);
+      __syncthreads();
+      for (int x = t; x < CI_K * 64; x += 256) {
+        const int kk = x >> 6, aa = x & 63;
+        As[kk][aa] = (kk < kc && a0 + aa < na) ? vrow[(long)(k0 + kk) * na + a0 + aa] : 0.0;
+        const int kb = x % CI_K, jb = x / CI_K;
+        Bs[kb][jb] = (kb < kc && jb < jn) ? c2b[k0 + kb + (long)jb * B.nham2] : 0.0;
+      }
+      __syncthreads();
+      for (int kk = 0; kk < kc; ++kk) {
+        const double av = As[kk][lane];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = fma(av, Bs[kk][w * 16 + q], acc[q]);
+      }
+    }
+    if (a0 + lane < na) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int j = w * 16 + q;
+        if (j < jn) out[a0 + lane + (long)j * G] = acc[q];
+      }
+    }
   }
 }
 
-void launch_cint(hipStream_t s, const double* v1, const double* c2,
-                 double* cint, const CBlock* cb, const int* blk3,
-                 const int* iv1off, int na, int nb, int n3) {
-  const long total = (long)na * nb * n3;
-  if (total <= 0) return;
-  int grid = (int)((total + 255) / 256);
-  if (grid > 65535 * 16) grid = 65535 * 16;
-  DVR_LAUNCH("cint", 8.0 * total, 0.0, k_cint, dim3(grid), dim3(256), 0, s, v1, c2, cint, cb,
-                   blk3, iv1off, na, nb, n3);
+void launch_cint(hipStream_t s, const double* v1, const double* c2, double* cint,
+                 const CBlock* cb, const std::vector<CBlock>& hcb, const int* iv1off, int na,
+                 int nb) {
+  int jt = 0, nc = (int)hcb.size();
+  double fl = 0.0;
+  for (const CBlock& B : hcb) {
+    jt = std::max(jt, (B.iv2 + 63) / 64);
+    fl += 2.0 * na * (double)B.nham2 * B.iv2;
+  }
+  if (jt == 0 || nb <= 0 || nc <= 0) return;
+  long n3 = 0;
+  for (const CBlock& B : hcb) n3 += B.iv2;
+  DVR_LAUNCH("cint", 8.0 * (double)na * nb * n3, fl, k_cint, dim3(jt, nb, nc), dim3(256), 0, s,
+             v1, c2, cint, cb, iv1off, na, nb);
   DVR_LAUNCH_CHECK();
 }
 
 // Wavefunction amplitudes on the DVR grid (trans, D:3650-3845): for every
-// c block, the (a, b) grid values of the levels are cint_c (G x iv2_c,
-// g = a nb + b) times the block's rows of the 3D eigenvectors, one FP64 MFMA
-// GEMM per block into T[g + G (c + nc l)]; a tiled scatter then writes them
-// in the IWAVE record layout psi(idvr, npnt1, npnt2) per level:
-// theta-first with zr2r1 -> (c = r1, b = r2), else (c = r2, b = r1).
-// T (G x nc) x l: 64 x 64 tiles of (a, b) through LDS, read along b and
-// written along a; blocks without chosen functions stay zero.
+// c block, the grid values of the levels are cint_c (G x iv2_c, g = b na + a)
+// times the block's rows of the 3D eigenvectors, one FP64 MFMA GEMM per block
+// into T[g + G (c + nc l)]; the scatter then copies each run of na values
+// (contiguous in a on both sides) into the IWAVE record layout
+// psi(idvr, npnt1, npnt2) per level: theta-first with zr2r1 -> (c = r1,
+// b = r2), else (c = r2, b = r1).  Blocks without chosen functions stay zero.
 __global__ void __launch_bounds__(256)
 k_trans_scatter(const double* __restrict__ T, int na, int nb, int nc, long nlev,
                 const CBlock* __restrict__ cb, double* __restrict__ psi, long npsi, long sb,
                 long sc) {
-  __shared__ double tl[64][65];
   const long G = (long)na * nb;
-  const int nta = (na + 63) / 64;
-  const int a0 = (blockIdx.x % nta) * 64, b0 = (blockIdx.x / nta) * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (long y = blockIdx.y; y < (long)nc * nlev; y += gridDim.y) {
+  const long total = G * nc * nlev;
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < total; x += (long)gridDim.x * 256) {
+    const long y = x / G;                 // c + nc l
+    const long g = x - y * G;             // b na + a
     const int c = (int)(y % nc);
+    if (cb[c].iv2 <= 0) continue;
     const long l = y / nc;
-    if (cb[c].iv2 <= 0) continue;       // uniform over the work-group
-    const double* src = T + G * y;
-    for (int i = ty; i < 64; i += 4) {
-      const int a = a0 + i, b = b0 + tx;
-      if (a < na && b < nb) tl[i][tx] = src[(long)a * nb + b];
-    }
-    __syncthreads();
-    double* dst = psi + npsi * l + sc * c;
-    for (int j = ty; j < 64; j += 4) {
-      const int a = a0 + tx, b = b0 + j;
-      if (a < na && b < nb) dst[a + sb * b] = tl[tx][j];
-    }
-    __syncthreads();
+    const int b = (int)(g / na), a = (int)(g - (long)b * na);
+    psi[npsi * l + sc * c + sb * b + a] = T[x];
   }
 }
 
@@ -164,10 +188,10 @@ void launch_trans(hipStream_t s, const double* cint, const double* c3, double* p
   const long npsi = (long)idvr * npnt1 * npnt2;
   const long sb = zr2r1 ? (long)idvr * npnt1 : (long)idvr;
   const long sc = zr2r1 ? (long)idvr : (long)idvr * npnt1;
-  const unsigned gx = (unsigned)(((na + 63) / 64) * ((nb + 63) / 64));
-  const unsigned gy = (unsigned)std::min<long>((long)nc * nvec, 65535);
-  DVR_LAUNCH("trans_scatter", 16.0 * G * nc * nvec, 0.0, k_trans_scatter, dim3(gx, gy),
-             dim3(256), 0, s, (const double*)T, na, nb, nc, (long)nvec, cb, psi, npsi, sb, sc);
+  const long tot = G * nc * nvec;
+  DVR_LAUNCH("trans_scatter", 16.0 * tot, 0.0, k_trans_scatter,
+             dim3((unsigned)std::min<long>((tot + 255) / 256, 65535L * 16)), dim3(256), 0, s,
+             (const double*)T, na, nb, nc, (long)nvec, cb, psi, npsi, sb, sc);
   DVR_LAUNCH_CHECK();
 }
 
