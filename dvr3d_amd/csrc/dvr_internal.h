@@ -565,6 +565,7 @@ struct GemmArgs {
   // A + z sa, B + z sb, C + z sc
   long sa = 0, sb = 0, sc = 0;
   int nbatch = 1;
+  int st = 16;   // super-tile width (tiles) of the persistent lower-tile update
 };
 // C = alpha op(A) op(B) + beta C (column-major).  ksplit > 1 splits K over
 // work-groups; work then holds gemm_work_doubles(M, N, ksplit) doubles.

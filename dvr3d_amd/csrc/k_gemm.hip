@@ -264,44 +264,44 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g0) {
 }
 
 // The lower 64 x 64 tiles (ti >= tj) of an nt x nt tile grid, enumerated in
-// super-tiles of 16 x 16 tiles: super-rows top to bottom, their super-tiles
+// super-tiles of S x S tiles: super-rows top to bottom, their super-tiles
 // left to right, tiles column by column inside (the diagonal super-tile's
 // lower triangle likewise).  lower64_count gives their number, lower64_tile
 // tile number idx.
-__device__ inline long lower64_count(int nt) {
+__device__ inline long lower64_count(int nt, int S) {
   long n = 0;
-  for (int si = 0; 16 * si < nt; ++si) {
-    const int h = min(16, nt - 16 * si);
-    n += 16L * h * si + (long)h * (h + 1) / 2;
+  for (int si = 0; S * si < nt; ++si) {
+    const int h = min(S, nt - S * si);
+    n += (long)S * h * si + (long)h * (h + 1) / 2;
   }
   return n;
 }
-__device__ inline void lower64_tile(long idx, int nt, int& ti, int& tj) {
+__device__ inline void lower64_tile(long idx, int nt, int S, int& ti, int& tj) {
   int si = 0;
   long base = 0;
   for (;;) {
-    const int h = min(16, nt - 16 * si);
-    const long cnt = 16L * h * si + (long)h * (h + 1) / 2;
+    const int h = min(S, nt - S * si);
+    const long cnt = (long)S * h * si + (long)h * (h + 1) / 2;
     if (idx < base + cnt) break;
     base += cnt;
     ++si;
   }
-  const int h = min(16, nt - 16 * si);
+  const int h = min(S, nt - S * si);
   long rem = idx - base;
-  if (rem < 16L * h * si) {
-    const int sj = (int)(rem / (16 * h));
-    const int loc = (int)(rem % (16 * h));
-    tj = 16 * sj + loc / h;
-    ti = 16 * si + loc % h;
+  if (rem < (long)S * h * si) {
+    const int sj = (int)(rem / (S * h));
+    const int loc = (int)(rem % (S * h));
+    tj = S * sj + loc / h;
+    ti = S * si + loc % h;
   } else {
-    rem -= 16L * h * si;
+    rem -= (long)S * h * si;
     int c = 0;
     while (rem >= h - c) {
       rem -= h - c;
       ++c;
     }
-    tj = 16 * si + c;
-    ti = 16 * si + c + (int)rem;
+    tj = S * si + c;
+    ti = S * si + c + (int)rem;
   }
 }
 
@@ -340,14 +340,15 @@ k_gemm_lower_w64(GemmArgs g) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int p = lane & 15, q = lane >> 4;
   const int M = g.M, nt = (M + 63) >> 6;
-  const long ntiles = lower64_count(nt);
+  const int S = g.st;
+  const long ntiles = lower64_count(nt, S);
   const int x = blockIdx.x & 7;
   const long W = (long)(gridDim.x >> 3) * 4;   // wavefronts per XCD
   const long end = ntiles * (x + 1) / 8;
   long idx = ntiles * x / 8 + (long)(blockIdx.x >> 3) * 4 + wave;
   if (idx >= end) return;
   int ti, tj;
-  lower64_tile(idx, nt, ti, tj);
+  lower64_tile(idx, nt, S, ti, tj);
   // a tile of the ragged last tile row (M % 64 != 0) computes the 64 rows
   // (and, on the diagonal, columns) ending at M, so that every load stays
   // inside the matrix, and stores only its own rows and columns
@@ -373,7 +374,7 @@ k_gemm_lower_w64(GemmArgs g) {
     const long in = idx + W;
     const bool more = in < end;
     int tin = ti, tjn = tj;
-    if (more) lower64_tile(in, nt, tin, tjn);
+    if (more) lower64_tile(in, nt, S, tin, tjn);
     const double* pan = g.A + base(tin) + p + (long)q * g.lda;
     const double* pbn = g.B + base(tjn) + p + (long)q * g.ldb;
     const int lr = base(ti), lc = base(tj);
@@ -544,6 +545,14 @@ void launch_gemm_var(hipStream_t s, const char* name, bool ta, bool tb, int M, i
                  kt, cp, xcd_lower);
 }
 
+// super-tile width (in 64-row tiles) of the persistent lower-tile update;
+// DVR_LDL_ST overrides it
+int ldl_super_tile() {
+  const char* v = getenv("DVR_LDL_ST");
+  const int st = v ? atoi(v) : 16;
+  return st >= 1 ? st : 16;
+}
+
 void launch_gemm_lower_nt(hipStream_t s, const char* name, int M, int K, double alpha,
                           const double* A, long lda, const double* B, long ldb, double beta,
                           double* C, long ldc, int wgs_per_xcd, bool xcd_lower) {
@@ -558,6 +567,7 @@ void launch_gemm_lower_nt(hipStream_t s, const char* name, int M, int K, double 
   g.M = M; g.N = M; g.K = K; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb;
   g.C = C; g.ldc = ldc; g.alpha = alpha; g.beta = beta;
   g.ksplit = 1; g.kchunk = K; g.work = nullptr; g.lower = 1; g.ntile = (M + 63) / 64;
+  g.st = ldl_super_tile();
   const dim3 grid(8u * (unsigned)wgs_per_xcd);
   const double flops = (double)M * M * K;   // lower half: 2 M^2 K / 2
   const double bytes = 8.0 * (2.0 * M * K + (beta != 0.0 ? 2.0 : 1.0) * 0.5 * M * M);
