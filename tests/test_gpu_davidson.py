@@ -75,9 +75,10 @@ def test_ldl_panel_widths_agree(monkeypatch):
 def test_ldl_persistent_update_same_bits(monkeypatch, n):
     """The trailing update on the persistent wavefront-tile kernel (32, 28 and
     5 work-groups per XCD; K = 256, 128, 64 from panels of 4, 2, 1
-    sub-panels; its M % 64 ragged rows on the 64 x 64 kernel) sums in the
-    order of the 64 x 64 kernel: inertia (LAPACK's count) and solves
-    bit-identical to DVR_LDL_WIDE=0.  (The shift sits mid-spectrum, where
+    sub-panels; its M % 64 ragged rows on the 64 x 64 kernel; tile orders in
+    super-tiles of 16, 7 and 24 tiles) sums in the order of the 64 x 64
+    kernel: inertia (LAPACK's count) and solves bit-identical to
+    DVR_LDL_WIDE=0.  (The shift sits mid-spectrum, where
     the unpivoted factorisation has element growth: the accuracy of the
     solves is tested on the cases above and below.)"""
     from dvr3d_amd import _lib
@@ -95,6 +96,13 @@ def test_ldl_persistent_update_same_bits(monkeypatch, n):
             monkeypatch.setenv("DVR_LDL_WIDE", wgs)
             nu, t = _lib.ldl_solve(a, sigma, r)
             assert nu == nu0 and np.array_equal(t, t0), (sub, wgs)
+        # other super-tile widths of the tile order: same tiles, same sums
+        monkeypatch.setenv("DVR_LDL_WIDE", "32")
+        for st in ("7", "24"):
+            monkeypatch.setenv("DVR_LDL_ST", st)
+            nu, t = _lib.ldl_solve(a, sigma, r)
+            assert nu == nu0 and np.array_equal(t, t0), (sub, st)
+        monkeypatch.delenv("DVR_LDL_ST")
 
 
 @pytest.mark.parametrize("sb", ["512", "1024", "2048"])
